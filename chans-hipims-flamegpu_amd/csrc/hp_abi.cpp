@@ -1,0 +1,19 @@
+// Library-wide C-ABI entry points: version and last-error message.
+#include <string>
+
+#include "hp_common.hpp"
+
+namespace hp {
+std::string& last_error_slot() {
+  static thread_local std::string msg;
+  return msg;
+}
+}  // namespace hp
+
+extern "C" {
+
+int hp_abi_version(void) { return HP_ABI_VERSION; }
+
+const char* hp_last_error(void) { return hp::last_error_slot().c_str(); }
+
+}  // extern "C"

@@ -1,0 +1,581 @@
+// Structured-grid HiPIMS step for gfx950.
+//
+// Layout: every field is a plane of ny rows x pitch elements (pitch is a
+// multiple of 64, so each row starts 256-byte aligned in float).  Raster
+// (row r, col c) of the reference's input maps to plane element r*pitch + c,
+// so the raster's own nodata ring doubles as the stencil apron.
+//
+// Per step:
+//   k_flux      tile of 64 x TY cells + 2-cell cross halo staged in LDS; every
+//               edge touching the tile is solved once (swe_physics.hpp) and
+//               both cells take their share; listed cells (the reference's
+//               fluxMask rule) write dh/dqx/dqy.
+//   k_update    one pass over the planes: rain, implicit friction + update,
+//               h_max, the CFL minimum (wave64 shuffle + LDS), FP64 volume /
+//               flux / rain sums; one partial record per block.
+//   k_finalize  one block folds the partials and advances dt, t and the
+//               export clamp in the device control block (SWE_CUDA.py:818-831).
+// No host synchronisation anywhere in the step.
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+#include <string>
+
+#include "../../include/hipims_abi.h"
+#include "hp_common.hpp"
+#include "swe_physics.hpp"
+
+namespace hp {
+
+constexpr int kTX = 64;
+constexpr int kThreads = 256;
+constexpr int kUpdateMaxBlocks = 2048;
+
+template <class T> struct TileRows;
+template <> struct TileRows<float> { static constexpr int v = 8; };
+template <> struct TileRows<double> { static constexpr int v = 4; };
+
+// --- boundary series ------------------------------------------------------
+template <class T>
+__device__ inline int series_row(const T* tab, int rows, int w, T t) {
+  int r = 0;
+  while (r < rows - 1 && t > tab[r * w]) r++;
+  return r;
+}
+
+template <class T>
+__device__ inline T series_at(const T* tab, int rows, int w, int row, int col) {
+  if (tab == nullptr || col < 0 || col >= w || row < 0 || row >= rows) return T(0);
+  return tab[row * w + col];
+}
+
+struct SeriesRows {
+  int d, w, q;
+};
+
+template <class T> struct BcValues {
+  int type;
+  T gd, gwl, gqx, gqy;
+};
+
+template <class T>
+__device__ inline BcValues<T> bc_values(const hp_grid_desc& a, int slot, const SeriesRows& rows) {
+  BcValues<T> b;
+  int id;
+  decode_code(a.slot_code[slot], b.type, id);
+  b.gd = series_at<T>((const T*)a.given_depth, a.gd_rows, a.gd_w, rows.d, id + 1);
+  b.gwl = series_at<T>((const T*)a.given_wl, a.gwl_rows, a.gwl_w, rows.w, id + 1);
+  b.gqx = series_at<T>((const T*)a.given_q, a.gq_rows, a.gq_w, rows.q, 1 + 2 * id);
+  b.gqy = series_at<T>((const T*)a.given_q, a.gq_rows, a.gq_w, rows.q, 2 + 2 * id);
+  return b;
+}
+
+// Global (qx, qy) of an edge-frame momentum flux (fn, ft) for direction d.
+template <class T> __device__ inline void to_global(T fn, T ft, int d, T& gx, T& gy) {
+  switch (d) {
+    case 0: gx = fn; gy = ft; break;
+    case 1: gx = -ft; gy = fn; break;
+    case 2: gx = -fn; gy = -ft; break;
+    default: gx = ft; gy = -fn; break;
+  }
+}
+
+// LDS tile of one flux block.
+template <class T, int TY> struct FluxTile {
+  static constexpr int LW = kTX + 4, LH = TY + 4;
+  T h[LH][LW], qx[LH][LW], qy[LH][LW], z[LH][LW];
+  uint8_t ty[LH][LW];
+  // per edge: 0 mass, 1 qx, 2 qy (from side A's view, to subtract at A and
+  // add at B), 3 bed term of A, 4 bed term of B (each to subtract along the
+  // edge axis).  A = west / north cell, B = east / south cell.
+  T ex[TY][kTX + 1][5];
+  T ey[TY + 1][kTX][5];
+};
+
+template <class T, int TY>
+__device__ inline FrameCell<T> frame_cell(const FluxTile<T, TY>& s, int r, int c, int d, T zref) {
+  FrameCell<T> f;
+  const T qx = s.qx[r][c], qy = s.qy[r][c];
+  f.h = s.h[r][c];
+  f.qn = rot_n(qx, qy, d);
+  f.qt = rot_t(qx, qy, d);
+  f.z = s.z[r][c] - zref;
+  return f;
+}
+
+// Solve the edge between LDS cells A=(ra,ca) and B=(rb,cb), where B lies in
+// direction dAB from A (0 E for x-edges, 3 S for y-edges).  (rl,cl) is A's
+// neighbour opposite to B, (rr2,cr2) is B's neighbour beyond B.
+template <class T, int ORDER, int TY>
+__device__ inline void solve_tile_edge(const FluxTile<T, TY>& s, const hp_grid_desc& a, const SeriesRows& rows,
+                                       int ra, int ca, int rb, int cb, int rl, int cl, int rr2, int cr2, int dAB,
+                                       T dx, T dt, T* out) {
+  out[0] = out[1] = out[2] = out[3] = out[4] = T(0);
+  const int tA = s.ty[ra][ca], tB = s.ty[rb][cb];
+  if (tA == 0 && tB == 0) return;
+  const int dBA = (dAB + 2) & 3;
+  const bool axis_x = (dAB == 0 || dAB == 2);
+  if (tA != 0 && tB != 0) {
+    const T zref = Prec<T>::local_datum ? s.z[ra][ca] : T(0);
+    const FrameCell<T> c = frame_cell(s, ra, ca, dAB, zref);
+    const FrameCell<T> r = frame_cell(s, rb, cb, dAB, zref);
+    const FrameCell<T> l = s.ty[rl][cl] ? frame_cell(s, rl, cl, dAB, zref) : c;
+    const FrameCell<T> rr = s.ty[rr2][cr2] ? frame_cell(s, rr2, cr2, dAB, zref) : r;
+    const EdgeOut<T> o = solve_edge<T, ORDER>(c, l, r, rr, false, c.h + c.z, r.h + r.z, dx, dt);
+    if (!o.active) return;
+    T gx, gy;
+    to_global(o.fn, o.ft, dAB, gx, gy);
+    out[0] = o.fh;
+    out[1] = gx;
+    out[2] = gy;
+    const T nA = T(axis_x ? nrm_x(dAB) : nrm_y(dAB));
+    out[3] = o.bed_c * nA;
+    out[4] = o.bed_r * (-nA);
+  } else if (tA != 0) {
+    const T zref = Prec<T>::local_datum ? s.z[ra][ca] : T(0);
+    const FrameCell<T> c = frame_cell(s, ra, ca, dAB, zref);
+    const BcValues<T> bv = bc_values<T>(a, tA, rows);
+    const FrameCell<T> g = bc_ghost<T>(bv.type, c, dAB, bv.gd, bv.gwl - zref, bv.gqx, bv.gqy);
+    const EdgeOut<T> o = solve_edge<T, ORDER>(c, c, g, g, true, c.h + c.z, g.h + g.z, dx, dt);
+    if (!o.active) return;
+    T gx, gy;
+    to_global(o.fn, o.ft, dAB, gx, gy);
+    out[0] = o.fh;
+    out[1] = gx;
+    out[2] = gy;
+    out[3] = o.bed_c * T(axis_x ? nrm_x(dAB) : nrm_y(dAB));
+  } else {
+    const T zref = Prec<T>::local_datum ? s.z[rb][cb] : T(0);
+    const FrameCell<T> c = frame_cell(s, rb, cb, dBA, zref);
+    const BcValues<T> bv = bc_values<T>(a, tB, rows);
+    const FrameCell<T> g = bc_ghost<T>(bv.type, c, dBA, bv.gd, bv.gwl - zref, bv.gqx, bv.gqy);
+    const EdgeOut<T> o = solve_edge<T, ORDER>(c, c, g, g, true, c.h + c.z, g.h + g.z, dx, dt);
+    if (!o.active) return;
+    T gx, gy;
+    to_global(o.fn, o.ft, dBA, gx, gy);
+    out[0] = -o.fh;
+    out[1] = -gx;
+    out[2] = -gy;
+    out[4] = o.bed_c * T(axis_x ? nrm_x(dBA) : nrm_y(dBA));
+  }
+}
+
+template <class T, int ORDER, int TY>
+__global__ __launch_bounds__(kThreads) void k_flux(hp_grid_desc a) {
+  __shared__ FluxTile<T, TY> s;
+  constexpr int LW = FluxTile<T, TY>::LW, LH = FluxTile<T, TY>::LH;
+  const int tid = threadIdx.x;
+  const int x0 = blockIdx.x * kTX, y0 = blockIdx.y * TY;
+  const int nx = a.nx, ny = a.ny;
+  const int64_t pitch = a.pitch;
+  const T* __restrict__ h = (const T*)a.h;
+  const T* __restrict__ qx = (const T*)a.qx;
+  const T* __restrict__ qy = (const T*)a.qy;
+  const T* __restrict__ z = (const T*)a.z;
+  const uint8_t* __restrict__ ty = a.type_slot;
+
+  for (int e = tid; e < LH * LW; e += kThreads) {
+    const int r = e / LW, c = e - r * LW;
+    const int gy = y0 - 2 + r, gx = x0 - 2 + c;
+    const bool in = gy >= 0 && gy < ny && gx >= 0 && gx < nx;
+    const int64_t gi = in ? (int64_t)gy * pitch + gx : 0;
+    s.ty[r][c] = in ? ty[gi] : (uint8_t)0;
+    s.h[r][c] = in ? h[gi] : T(0);
+    s.qx[r][c] = in ? qx[gi] : T(0);
+    s.qy[r][c] = in ? qy[gi] : T(0);
+    s.z[r][c] = in ? z[gi] : T(0);
+  }
+
+  const double t_d = a.ctrl->t;
+  const T t = T(t_d);
+  const T dt = T(a.ctrl->dt);
+  const T dx = T(a.dx);
+  SeriesRows rows;
+  rows.d = a.given_depth ? series_row<T>((const T*)a.given_depth, a.gd_rows, a.gd_w, t) : 0;
+  rows.w = a.given_wl ? series_row<T>((const T*)a.given_wl, a.gwl_rows, a.gwl_w, t) : 0;
+  rows.q = a.given_q ? series_row<T>((const T*)a.given_q, a.gq_rows, a.gq_w, t) : 0;
+  __syncthreads();
+
+  // x-edges: row ly, between local columns lx-1 (A) and lx (B), lx = 0..64.
+  for (int e = tid; e < TY * (kTX + 1); e += kThreads) {
+    const int ly = e / (kTX + 1), lx = e - ly * (kTX + 1);
+    const int r = ly + 2, ca = lx + 1, cb = lx + 2;
+    solve_tile_edge<T, ORDER, TY>(s, a, rows, r, ca, r, cb, r, ca - 1, r, cb + 1, 0, dx, dt, s.ex[ly][lx]);
+  }
+  // y-edges: column lx, between local rows ly-1 (A, north) and ly (B, south).
+  for (int e = tid; e < (TY + 1) * kTX; e += kThreads) {
+    const int ly = e / kTX, lx = e - ly * kTX;
+    const int c = lx + 2, ra = ly + 1, rb = ly + 2;
+    solve_tile_edge<T, ORDER, TY>(s, a, rows, ra, c, rb, c, ra - 1, c, rb + 1, c, 3, dx, dt, s.ey[ly][lx]);
+  }
+  __syncthreads();
+
+  const T hs = T(kHs);
+  const bool t_zero = (t == T(0));
+  T* __restrict__ dh = (T*)a.dh;
+  T* __restrict__ dqx = (T*)a.dqx;
+  T* __restrict__ dqy = (T*)a.dqy;
+  for (int e = tid; e < TY * kTX; e += kThreads) {
+    const int ly = e / kTX, lx = e - ly * kTX;
+    const int gy = y0 + ly, gx = x0 + lx;
+    if (gy >= ny || gx >= nx) continue;
+    const int r = ly + 2, c = lx + 2;
+    T ah = T(0), aqx = T(0), aqy = T(0);
+    if (s.ty[r][c] != 0) {
+      bool listed = t_zero || s.h[r][c] > hs;
+      // neighbours E, N, W, S; compact cell 0 never counts (fluxMaskGenerator_Kernel.cu:36)
+      const int nr[4] = {r, r - 1, r, r + 1};
+      const int nc[4] = {c + 1, c, c - 1, c};
+      const int ngx[4] = {gx + 1, gx, gx - 1, gx};
+      const int ngy[4] = {gy, gy - 1, gy, gy + 1};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const bool is0 = (ngx[k] == a.cell0_x && ngy[k] == a.cell0_y);
+        listed = listed || (s.ty[nr[k]][nc[k]] != 0 && !is0 && s.h[nr[k]][nc[k]] > hs);
+      }
+      if (listed) {
+        const T* eE = s.ex[ly][lx + 1];  // this cell is A
+        const T* eN = s.ey[ly][lx];      // this cell is B
+        const T* eW = s.ex[ly][lx];      // this cell is B
+        const T* eS = s.ey[ly + 1][lx];  // this cell is A
+        // E
+        ah -= eE[0]; aqx -= eE[1]; aqx -= eE[3]; aqy -= eE[2];
+        // N
+        ah -= -eN[0]; aqx -= -eN[1]; aqy -= -eN[2]; aqy -= eN[4];
+        // W
+        ah -= -eW[0]; aqx -= -eW[1]; aqx -= eW[4]; aqy -= -eW[2];
+        // S
+        ah -= eS[0]; aqx -= eS[1]; aqy -= eS[2]; aqy -= eS[3];
+      }
+    }
+    const int64_t gi = (int64_t)gy * pitch + gx;
+    dh[gi] = ah;
+    dqx[gi] = aqx;
+    dqy[gi] = aqy;
+  }
+}
+
+// --- update pass ------------------------------------------------------------
+struct Partial {
+  double min_dt, vol, flux, rain;
+};
+
+template <class T> __device__ inline T wave_min(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+constexpr int kMaxStations = 1024;
+
+template <class T>
+__global__ __launch_bounds__(kThreads) void k_update(hp_grid_desc a) {
+  __shared__ double s_rain[kMaxStations];
+  __shared__ double s_red[4][kThreads / 64];
+  const int tid = threadIdx.x;
+  const double t_now = a.ctrl->t;
+  const T dt = T(a.ctrl->dt);
+  const T hs = T(kHs);
+  const T cfl_dx = T(a.cfl) * T(a.dx);
+
+  // station rain values at the current time (SWE_CUDA.py:472-495)
+  int n_st = 0;
+  if (a.rain_mode == HP_RAIN_STATION && a.rain_table != nullptr && a.rain_rows > 0) {
+    const int rows = a.rain_rows, cols = a.rain_cols;
+    n_st = cols - 1;
+    const double* tab = a.rain_table;
+    const bool after = !(t_now < tab[(rows - 1) * cols]);
+    int i = 0;
+    while (i + 1 < rows - 1 && !(t_now < tab[(i + 1) * cols])) i++;
+    for (int k = tid; k < n_st && k < kMaxStations; k += kThreads) {
+      double v = 0.0;
+      if (!after && rows > 1) {
+        const double per = (t_now - tab[i * cols]) / (tab[(i + 1) * cols] - tab[i * cols]);
+        v = tab[i * cols + 1 + k] + per * (tab[(i + 1) * cols + 1 + k] - tab[i * cols + 1 + k]);
+      }
+      s_rain[k] = v * a.rain_scale;
+    }
+  }
+  __syncthreads();
+
+  const T* __restrict__ dhp = (const T*)a.dh;
+  const T* __restrict__ dqxp = (const T*)a.dqx;
+  const T* __restrict__ dqyp = (const T*)a.dqy;
+  T* __restrict__ hp_ = (T*)a.h;
+  T* __restrict__ qxp = (T*)a.qx;
+  T* __restrict__ qyp = (T*)a.qy;
+  T* __restrict__ hmp = (T*)a.h_max;
+  const uint8_t* __restrict__ lu = a.landuse;
+  const T* __restrict__ man = (const T*)a.manning;
+  const int16_t* __restrict__ st = a.station;
+  const float* __restrict__ rf = a.rain_field;
+  const int mode = a.rain_mode;
+  const T rscale = T(a.rain_scale);
+
+  T mn = T(INFINITY);
+  double vol = 0.0, fsum = 0.0, rsum = 0.0;
+  const int64_t total = a.pitch * (int64_t)a.ny;
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + tid; i < total; i += (int64_t)gridDim.x * kThreads) {
+    const int cls = lu[i];
+    if (cls == 255) continue;
+    const T dhf = dhp[i];
+    T r = T(0);
+    if (mode == HP_RAIN_STATION) {
+      const int k = st ? (int)st[i] : 0;
+      r = (k >= 0 && k < n_st) ? T(s_rain[k]) : T(0);
+    } else if (mode == HP_RAIN_FIELD) {
+      r = T(rf[i]) * rscale;
+    }
+    const T rdt = r * dt;
+    const T dht = dhf + rdt;
+    T h = hp_[i];
+    if (!(fabs(dht) > T(0) || h >= T(0))) {
+      vol += (double)h;
+      continue;
+    }
+    h += dht;
+    T qx, qy;
+    if (h < hs) {
+      qx = T(0);
+      qy = T(0);
+    } else {
+      qx = qxp[i] + dqxp[i];
+      qy = qyp[i] + dqyp[i];
+      implicit_friction<T>(h, man[cls], dt, qx, qy);
+    }
+    hp_[i] = h;
+    qxp[i] = qx;
+    qyp[i] = qy;
+    hmp[i] = fmax(h, hmp[i]);
+    mn = fmin(mn, cfl_candidate<T>(h, qx, qy, cfl_dx));
+    vol += (double)h;
+    fsum += (double)dhf;
+    rsum += (double)rdt;
+  }
+
+  const int lane = tid & 63, w = tid >> 6;
+  mn = wave_min(mn);
+  vol = wave_sum(vol);
+  fsum = wave_sum(fsum);
+  rsum = wave_sum(rsum);
+  if (lane == 0) {
+    s_red[0][w] = (double)mn;
+    s_red[1][w] = vol;
+    s_red[2][w] = fsum;
+    s_red[3][w] = rsum;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    Partial p;
+    p.min_dt = s_red[0][0];
+    p.vol = s_red[1][0];
+    p.flux = s_red[2][0];
+    p.rain = s_red[3][0];
+    for (int k = 1; k < kThreads / 64; ++k) {
+      p.min_dt = fmin(p.min_dt, s_red[0][k]);
+      p.vol += s_red[1][k];
+      p.flux += s_red[2][k];
+      p.rain += s_red[3][k];
+    }
+    ((Partial*)a.work)[blockIdx.x] = p;
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_finalize(hp_ctrl* ctrl, const Partial* parts, int n, double dx) {
+  __shared__ double s_red[4][16];
+  const int tid = threadIdx.x;
+  double mn = INFINITY, vol = 0.0, fs = 0.0, rs = 0.0;
+  for (int k = tid; k < n; k += 1024) {
+    const Partial p = parts[k];
+    mn = fmin(mn, p.min_dt);
+    vol += p.vol;
+    fs += p.flux;
+    rs += p.rain;
+  }
+  mn = wave_min(mn);
+  vol = wave_sum(vol);
+  fs = wave_sum(fs);
+  rs = wave_sum(rs);
+  const int lane = tid & 63, w = tid >> 6;
+  if (lane == 0) {
+    s_red[0][w] = mn;
+    s_red[1][w] = vol;
+    s_red[2][w] = fs;
+    s_red[3][w] = rs;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    for (int k = 1; k < 16; ++k) {
+      mn = fmin(s_red[0][0], s_red[0][k]);
+      s_red[0][0] = mn;
+      s_red[1][0] += s_red[1][k];
+      s_red[2][0] += s_red[2][k];
+      s_red[3][0] += s_red[3][k];
+    }
+    mn = s_red[0][0];
+    const double area = dx * dx;
+    const double t = ctrl->t;
+    const double dt_old = ctrl->dt;
+    double dt_new = isinf(mn) ? dt_old : mn;  // no listed cell: keep dt (SWE_CUDA.py:818-822)
+    const double upper = dt_old * 10.0;
+    dt_new = (upper < dt_new) ? upper : dt_new;
+    const double next_export = (ctrl->export_n + 1.0) * ctrl->export_T;
+    if (dt_new + t >= next_export) {
+      dt_new = next_export - t;
+      ctrl->export_n += 1.0;
+      ctrl->export_pending += 1.0;
+    }
+    ctrl->dt = dt_new;
+    ctrl->t = t + dt_new;
+    ctrl->volume = s_red[1][0] * area;
+    ctrl->flux_volume += s_red[2][0] * area;
+    ctrl->rain_volume += s_red[3][0] * area;
+    ctrl->steps += 1.0;
+  }
+}
+
+// --- host side -------------------------------------------------------------
+static int update_blocks(const hp_grid_desc* g) {
+  const int64_t total = g->pitch * (int64_t)g->ny;
+  int64_t b = (total + kThreads - 1) / kThreads;
+  if (b > kUpdateMaxBlocks) b = kUpdateMaxBlocks;
+  if (b < 1) b = 1;
+  return (int)b;
+}
+
+static int check_desc(const hp_grid_desc* g) {
+  if (!g) return fail(HP_EINVAL, "null descriptor");
+  if (g->dtype != HP_F32 && g->dtype != HP_F64) return fail(HP_EINVAL, "dtype must be HP_F32 or HP_F64");
+  if (g->order != 1 && g->order != 2) return fail(HP_EINVAL, "order must be 1 or 2");
+  if (g->nx <= 0 || g->ny <= 0) return fail(HP_EINVAL, "nx and ny must be positive");
+  if (g->pitch < g->nx) return fail(HP_EINVAL, "pitch must be >= nx");
+  if (!(g->dx > 0.0)) return fail(HP_EINVAL, "dx must be positive");
+  if (!g->h || !g->qx || !g->qy || !g->z || !g->h_max || !g->dh || !g->dqx || !g->dqy)
+    return fail(HP_EINVAL, "a state plane pointer is null");
+  if (!g->type_slot || !g->landuse || !g->slot_code || !g->manning || !g->ctrl)
+    return fail(HP_EINVAL, "type_slot, landuse, slot_code, manning and ctrl are required");
+  if (g->n_classes <= 0 || g->n_classes > 255) return fail(HP_EINVAL, "n_classes must be in 1..255");
+  if (g->rain_mode == HP_RAIN_STATION) {
+    if (!g->rain_table || g->rain_rows < 1 || g->rain_cols < 2)
+      return fail(HP_EINVAL, "station rain needs a table of >= 1 row and >= 2 columns");
+    if (g->rain_cols - 1 > kMaxStations) return fail(HP_EINVAL, "at most 1024 rain stations");
+  } else if (g->rain_mode == HP_RAIN_FIELD) {
+    if (!g->rain_field) return fail(HP_EINVAL, "field rain needs rain_field");
+  } else if (g->rain_mode != HP_RAIN_NONE) {
+    return fail(HP_EINVAL, "unknown rain_mode");
+  }
+  if ((g->given_depth && (g->gd_rows < 1 || g->gd_w < 1)) || (g->given_wl && (g->gwl_rows < 1 || g->gwl_w < 1)) ||
+      (g->given_q && (g->gq_rows < 1 || g->gq_w < 1)))
+    return fail(HP_EINVAL, "boundary series tables need >= 1 row and column");
+  return HP_OK;
+}
+
+template <class T, int ORDER>
+static int launch_flux_t(const hp_grid_desc* g, hipStream_t st) {
+  constexpr int TY = TileRows<T>::v;
+  dim3 grid((g->nx + kTX - 1) / kTX, (g->ny + TY - 1) / TY);
+  hipLaunchKernelGGL((k_flux<T, ORDER, TY>), grid, dim3(kThreads), 0, st, *g);
+  return check_launch("k_flux");
+}
+
+static int launch_flux(const hp_grid_desc* g, hipStream_t st) {
+  if (g->dtype == HP_F32) return g->order == 2 ? launch_flux_t<float, 2>(g, st) : launch_flux_t<float, 1>(g, st);
+  return g->order == 2 ? launch_flux_t<double, 2>(g, st) : launch_flux_t<double, 1>(g, st);
+}
+
+static int launch_update(const hp_grid_desc* g, hipStream_t st) {
+  const int nb = update_blocks(g);
+  if (!g->work || g->work_bytes < (int64_t)nb * (int64_t)sizeof(Partial))
+    return fail(HP_ESPACE, "workspace smaller than hp_grid_workspace_bytes()");
+  if (g->dtype == HP_F32) hipLaunchKernelGGL((k_update<float>), dim3(nb), dim3(kThreads), 0, st, *g);
+  else hipLaunchKernelGGL((k_update<double>), dim3(nb), dim3(kThreads), 0, st, *g);
+  int rc = check_launch("k_update");
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, st, g->ctrl, (const Partial*)g->work, nb, g->dx);
+  return check_launch("k_finalize");
+}
+
+template <class T>
+__global__ void k_gather(const T* __restrict__ plane, const int64_t* __restrict__ off, int64_t n, T* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = plane[off[i]];
+}
+template <class T>
+__global__ void k_scatter(T* __restrict__ plane, const int64_t* __restrict__ off, int64_t n, const T* __restrict__ in) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) plane[off[i]] = in[i];
+}
+
+}  // namespace hp
+
+using namespace hp;
+
+extern "C" {
+
+int64_t hp_grid_workspace_bytes(const hp_grid_desc* g) {
+  if (!g || g->pitch <= 0 || g->ny <= 0) return 0;
+  return (int64_t)update_blocks(g) * (int64_t)sizeof(Partial);
+}
+
+int hp_grid_flux(const hp_grid_desc* g, void* stream) {
+  int rc = check_desc(g);
+  if (rc) return rc;
+  return launch_flux(g, (hipStream_t)stream);
+}
+
+int hp_grid_update(const hp_grid_desc* g, void* stream) {
+  int rc = check_desc(g);
+  if (rc) return rc;
+  return launch_update(g, (hipStream_t)stream);
+}
+
+int hp_grid_step(const hp_grid_desc* g, int32_t nsteps, void* stream) {
+  int rc = check_desc(g);
+  if (rc) return rc;
+  if (nsteps < 0) return fail(HP_EINVAL, "nsteps must be >= 0");
+  for (int32_t k = 0; k < nsteps; ++k) {
+    rc = launch_flux(g, (hipStream_t)stream);
+    if (rc) return rc;
+    rc = launch_update(g, (hipStream_t)stream);
+    if (rc) return rc;
+  }
+  return HP_OK;
+}
+
+int hp_gather(int dtype, const void* plane, const int64_t* offsets, int64_t n, void* out, void* stream) {
+  if (dtype != HP_F32 && dtype != HP_F64) return fail(HP_EINVAL, "dtype must be HP_F32 or HP_F64");
+  if (n < 0) return fail(HP_EINVAL, "n must be >= 0");
+  if (n == 0) return HP_OK;
+  if (!plane || !offsets || !out) return fail(HP_EINVAL, "null pointer");
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  if (dtype == HP_F32)
+    hipLaunchKernelGGL(k_gather<float>, dim3(nb), dim3(256), 0, (hipStream_t)stream, (const float*)plane, offsets, n,
+                       (float*)out);
+  else
+    hipLaunchKernelGGL(k_gather<double>, dim3(nb), dim3(256), 0, (hipStream_t)stream, (const double*)plane, offsets,
+                       n, (double*)out);
+  return check_launch("k_gather");
+}
+
+int hp_scatter(int dtype, void* plane, const int64_t* offsets, int64_t n, const void* in, void* stream) {
+  if (dtype != HP_F32 && dtype != HP_F64) return fail(HP_EINVAL, "dtype must be HP_F32 or HP_F64");
+  if (n < 0) return fail(HP_EINVAL, "n must be >= 0");
+  if (n == 0) return HP_OK;
+  if (!plane || !offsets || !in) return fail(HP_EINVAL, "null pointer");
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  if (dtype == HP_F32)
+    hipLaunchKernelGGL(k_scatter<float>, dim3(nb), dim3(256), 0, (hipStream_t)stream, (float*)plane, offsets, n,
+                       (const float*)in);
+  else
+    hipLaunchKernelGGL(k_scatter<double>, dim3(nb), dim3(256), 0, (hipStream_t)stream, (double*)plane, offsets, n,
+                       (const double*)in);
+  return check_launch("k_scatter");
+}
+
+}  // extern "C"

@@ -1,0 +1,352 @@
+// Device-side physics of the HiPIMS shallow-water step, written for gfx950.
+//
+// Everything here works in an *edge frame*: an edge between cell c and its
+// neighbour r in direction d (E, N, W, S; N is raster row - 1) is solved with
+// the normal momentum qn = q . n and the tangential momentum qt = q . t,
+// t = (-n_y, n_x).  One call produces the edge's mass flux and both sides'
+// bed-slope terms, so a stencil kernel can solve every edge once and hand the
+// negated flux to the other cell.
+//
+// Behaviour follows the reference kernels:
+//   minmod ............ CHANS-HiPIMS/hipims/cuda/fluxCal_jh_modified_surface_Kernel.cu:321-328
+//   order-1 bed recon . fluxCal_jh_modified_surface_Kernel.cu:579-592
+//   order-1 hydro recon fluxCal_jh_modified_surface_Kernel.cu:96-139
+//   order-2 MUSCL ..... flucCal_2ndOrder_jh_improved_Kernel.cu:83-107
+//   order-2 hydro recon flucCal_2ndOrder_jh_improved_Kernel.cu:43-72
+//   HLLC .............. fluxCal_jh_modified_surface_Kernel.cu:173-260
+//   bed-slope source .. fluxCal_jh_modified_surface_Kernel.cu:603-610,
+//                       flucCal_2ndOrder_jh_improved_Kernel.cu:494-501
+//   boundary ghosts ... fluxCal_jh_modified_surface_Kernel.cu:262-435
+//   implicit friction . friction_implicit_andUpdate_jh_Kernel.cu:44-139
+//   CFL accelerator ... timeControl_Kernel.cu:186-189
+//
+// Precision policy (template parameter T):
+//   double: every expression is evaluated in the reference's order with IEEE
+//           division and square root, so results agree with the reference
+//           update to rounding.
+//   float:  the reconstruction works relative to the cell's own bed (a local
+//           datum) so thin films are not lost in the rounding of a 50 m
+//           water level; the implicit-friction quotient uses its
+//           cancellation-free form 2q / (1 + sqrt(1 + x)); divisions and
+//           square roots use the hardware reciprocal / square-root (<= 1 ulp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hp {
+
+constexpr double kG = 9.81;
+constexpr double kHs = 1.0e-6;
+
+template <class T> struct Prec;
+template <> struct Prec<double> {
+  static constexpr bool local_datum = false;
+  __device__ static inline double div(double a, double b) { return a / b; }
+  __device__ static inline double sqrt_(double x) { return __builtin_sqrt(x); }
+  __device__ static inline double rcbrt_friction(double h) { return pow(h, -1.0 / 3.0); }
+};
+template <> struct Prec<float> {
+  static constexpr bool local_datum = true;
+  __device__ static inline float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+  __device__ static inline float sqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
+  __device__ static inline float rcbrt_friction(float h) { return rcbrtf(h); }
+};
+
+template <class T> __device__ inline T minmod(T x, T y) {
+  if (x * y <= T(0)) return T(0);
+  return fabs(x) < fabs(y) ? x : y;
+}
+
+// A cell seen from an edge frame.
+template <class T> struct FrameCell {
+  T h, qn, qt, z;
+};
+
+// Normal of direction d (E, N, W, S) as small integers.
+__device__ inline int nrm_x(int d) { return d == 0 ? 1 : (d == 2 ? -1 : 0); }
+__device__ inline int nrm_y(int d) { return d == 1 ? 1 : (d == 3 ? -1 : 0); }
+
+// q . n and q . t with t = (-n_y, n_x); exact for unit axis normals.
+template <class T> __device__ inline T rot_n(T qx, T qy, int d) {
+  switch (d) {
+    case 0: return qx;
+    case 1: return qy;
+    case 2: return -qx;
+    default: return -qy;
+  }
+}
+template <class T> __device__ inline T rot_t(T qx, T qy, int d) {
+  switch (d) {
+    case 0: return qy;
+    case 1: return -qx;
+    case 2: return -qy;
+    default: return qx;
+  }
+}
+
+// Boundary ghost in the edge frame, built from the cell state c (frame
+// values) and the decoded boundary type.  The reference builds the ghost in
+// the global frame (boundary_WallNonSlip etc.); the frame form below is the
+// same map rotated:
+//   3 "rigid":  E/W edges reverse qx and qy; on N/S edges the reference's
+//               expressions reduce to the identity (the ghost copies c).
+//   4 slip:     reverse the normal component only.
+//   5 open:     copy.
+//   6 H given:  h = given depth, q copied.
+//   7 Q given:  q = given (global qx, qy), h copied.
+//   8 WL given: h = max(given_wl, z) - z, q copied.
+//   9 fall:     h = 0, q = 0.
+// Any other type: copy (the reference leaves the ghost undefined there).
+// gwl is passed relative to the same datum as c.z.
+template <class T>
+__device__ inline FrameCell<T> bc_ghost(int type, const FrameCell<T>& c, int d, T gd, T gwl, T gqx, T gqy) {
+  FrameCell<T> b = c;
+  switch (type) {
+    case 3:
+      if (d == 0 || d == 2) { b.qn = -c.qn; b.qt = -c.qt; }
+      break;
+    case 4:
+      b.qn = -c.qn;
+      break;
+    case 6:
+      b.h = gd;
+      break;
+    case 7:
+      b.qn = rot_n(gqx, gqy, d);
+      b.qt = rot_t(gqx, gqy, d);
+      break;
+    case 8: {
+      T wlb = gwl > c.z ? gwl : c.z;
+      b.h = wlb - c.z;
+      break;
+    }
+    case 9:
+      b.h = T(0); b.qn = T(0); b.qt = T(0);
+      break;
+    default:
+      break;
+  }
+  return b;
+}
+
+// Result of one edge solve, already scaled by dt/dx as the reference
+// accumulates it:  e = (raw / dx) * dt.
+template <class T> struct EdgeOut {
+  T fh;      // mass flux leaving c through the edge
+  T fn, ft;  // normal / tangential momentum flux (edge frame)
+  T bed_c;   // bed-slope term of c along the normal (sign of n included below)
+  T bed_r;   // bed-slope term of r along the same normal
+  bool active;
+};
+
+// HLLC with the hydrostatic reconstruction applied first.
+// eL / eR: edge states before reconstruction (h, qn, qt, z) - velocities come
+// from these.  etaC / etaR: the cells' water levels used by the clamp.
+// On return eL.h/eL.z and eR.h/eR.z hold the reconstructed depth and bed.
+template <class T, int ORDER>
+__device__ inline void hllc_edge(FrameCell<T>& eL, FrameCell<T>& eR, T etaC, T etaR, T& fh, T& fn, T& ft) {
+  using P = Prec<T>;
+  const T hs = T(kHs), g = T(kG);
+  const T uL0 = eL.h < hs ? T(0) : P::div(eL.qn, eL.h);
+  const T vL0 = eL.h < hs ? T(0) : P::div(eL.qt, eL.h);
+  const T uR0 = eR.h < hs ? T(0) : P::div(eR.qn, eR.h);
+  const T vR0 = eR.h < hs ? T(0) : P::div(eR.qt, eR.h);
+
+  // hydrostatic reconstruction
+  T wL = eL.h + eL.z;
+  T wR = eR.h + eR.z;
+  const T emax = fmax(etaC, etaR), emin = fmin(etaC, etaR);
+  wL = fmax(fmin(wL, emax), emin);
+  wR = fmax(fmin(wR, emax), emin);
+  if (ORDER == 1) {
+    if ((etaC - etaR) * (wL - wR) < T(0)) {
+      T m = (wL + wR) / T(2);
+      wL = m;
+      wR = m;
+    }
+  } else {
+    if (etaC > etaR) wL = fmax(wL, wR);
+    else wR = fmax(wL, wR);
+  }
+  const T zbm = fmax(wL - eL.h, wR - eR.h);
+  eL.h = fmax(wL - zbm, T(0));
+  eR.h = fmax(wR - zbm, T(0));
+  eL.z = wL - eL.h;
+  eR.z = wR - eR.h;
+
+  fh = T(0); fn = T(0); ft = T(0);
+  if (eL.h < hs && eR.h < hs) return;
+
+  const T hL = eL.h, hR = eR.h;
+  const T qnL = uL0 * hL, qtL = vL0 * hL;
+  const T qnR = uR0 * hR, qtR = vR0 * hR;
+  const T aL = P::sqrt_(g * hL);
+  const T aR = P::sqrt_(g * hR);
+  const T base = (aL + aR) / T(2) + (uL0 - uR0) / T(4);
+  const T h_star = P::div(base * base, g);
+  const T u_star = (uL0 + uR0) / T(2) + aL - aR;
+  const T a_star = P::sqrt_(g * h_star);
+  const T sL = hL <= hs ? uR0 - T(2) * aR : fmin(uL0 - aL, u_star - a_star);
+  const T sR = hR <= hs ? uL0 + T(2) * aL : fmax(uR0 + aR, u_star + a_star);
+  const T sM = P::div(sL * hR * (uR0 - sR) - sR * hL * (uL0 - sL), hR * (uR0 - sR) - hL * (uL0 - sL));
+  if (T(0) <= sL) {
+    fh = qnL;
+    fn = uL0 * qnL + T(0.5) * g * hL * hL;
+    ft = uL0 * qtL;
+  } else if ((sL < T(0) && T(0) <= sM) || (sM < T(0) && T(0) <= sR)) {
+    const T qnfL = uL0 * qnL + T(0.5) * g * hL * hL;
+    const T qnfR = uR0 * qnR + T(0.5) * g * hR * hR;
+    fh = P::div(sR * qnL - sL * qnR + sL * sR * (hR - hL), sR - sL);
+    fn = P::div(sR * qnfL - sL * qnfR + sL * sR * (qnR - qnL), sR - sL);
+    ft = fh * ((sL < T(0) && T(0) <= sM) ? vL0 : vR0);
+  } else {
+    fh = qnR;
+    fn = uR0 * qnR + T(0.5) * g * hR * hR;
+    ft = uR0 * qtR;
+  }
+}
+
+// Solve one edge from cell c towards r (direction d).
+//   l:  c's neighbour opposite to r (c itself when that neighbour is nodata)
+//   rr: r's neighbour beyond r (r itself when nodata)
+//   r_is_ghost: r is a boundary ghost (c's neighbour in direction d is nodata);
+//               then l and rr are unused and no reconstruction is applied.
+// etaC / etaR: the two cells' water levels.  Order 1 uses h + z of the
+// unreconstructed cells (the reference rebuilds them from (h, z)); order 2
+// uses the stored cell water level wl (a ghost's wl for a boundary edge).
+// Frame values carry z relative to c's datum (c.z == 0 in float).
+// The returned terms are scaled by dt/dx as  (raw / dx) * dt.
+template <class T, int ORDER>
+__device__ inline EdgeOut<T> solve_edge(const FrameCell<T>& c, const FrameCell<T>& l, const FrameCell<T>& r,
+                                        const FrameCell<T>& rr, bool r_is_ghost, T etaC, T etaR, T dx, T dt) {
+  const T hs = T(kHs), g = T(kG);
+  EdgeOut<T> o;
+  o.fh = T(0); o.fn = T(0); o.ft = T(0); o.bed_c = T(0); o.bed_r = T(0);
+  o.active = !(c.h < hs && r.h < hs);
+  if (!o.active) return o;
+
+  FrameCell<T> eL, eR;
+  if (ORDER == 1) {
+    eL = c;
+    eR = r;
+    if (!r_is_ghost) {
+      T rb = minmod<T>(c.z - l.z, r.z - c.z);
+      if (r.h > hs) eL.z += T(0.5) * rb;
+      rb = minmod<T>(r.z - rr.z, c.z - r.z);
+      if (c.h > hs) eR.z += T(0.5) * rb;
+    }
+  } else {
+    if (r_is_ghost) {
+      eL = c;
+      eR = r;
+    } else {
+      eL.h = c.h + T(0.5) * minmod<T>(c.h - l.h, r.h - c.h);
+      eL.z = c.z + T(0.5) * minmod<T>(c.z - l.z, r.z - c.z);
+      eR.h = r.h + T(0.5) * minmod<T>(r.h - rr.h, c.h - r.h);
+      eR.z = r.z + T(0.5) * minmod<T>(r.z - rr.z, c.z - r.z);
+      if (c.h < hs) {
+        eL.qn = T(0); eL.qt = T(0);
+      } else {
+        eL.qn = c.qn + T(0.5) * minmod<T>(c.qn - l.qn, r.qn - c.qn);
+        eL.qt = c.qt + T(0.5) * minmod<T>(c.qt - l.qt, r.qt - c.qt);
+      }
+      if (r.h < hs) {
+        eR.qn = T(0); eR.qt = T(0);
+      } else {
+        eR.qn = r.qn + T(0.5) * minmod<T>(r.qn - rr.qn, c.qn - r.qn);
+        eR.qt = r.qt + T(0.5) * minmod<T>(r.qt - rr.qt, c.qt - r.qt);
+      }
+    }
+  }
+  T fh, fn, ft;
+  hllc_edge<T, ORDER>(eL, eR, etaC, etaR, fh, fn, ft);
+  o.fh = (fh / dx) * dt;
+  o.fn = (fn / dx) * dt;
+  o.ft = (ft / dx) * dt;
+  o.bed_c = ((T(0.5) * g * (eL.h + c.h) * (eL.z - c.z)) / dx) * dt;
+  o.bed_r = ((T(0.5) * g * (eR.h + r.h) * (eR.z - r.z)) / dx) * dt;
+  return o;
+}
+
+// Implicit Manning friction for one wet cell (h >= hs), updating qx, qy in
+// place after they have received their flux increments.
+template <class T>
+__device__ inline void implicit_friction(T h, T n, T dt, T& qx, T& qy) {
+  using P = Prec<T>;
+  const T g = T(kG);
+  const T C_f = g * n * n * Prec<T>::rcbrt_friction(h);
+  const T q_norm = P::sqrt_(qx * qx + qy * qy);
+  if (!(n > T(0))) return;
+  const T lim = T(5) * h * P::sqrt_(h * g);
+  if (P::local_datum) {
+    // cancellation-free: (1 - sqrt(1 + x)) / (-(x/2) / q_k) == 2 q_k / (1 + sqrt(1 + x))
+    const T x = P::div(T(4) * dt * C_f, h * h) * q_norm;
+    const T k = P::div(T(2), T(1) + P::sqrt_(T(1) + x));
+    if (fabs(qx) > T(1.0e-10)) {
+      const T tmp = k * qx;
+      if (fabs(tmp) > lim) {
+        qx = T(2) * tmp - qx;
+        if (qx * tmp <= T(0)) qx = T(0);
+      } else {
+        qx = tmp;
+      }
+    }
+    if (fabs(qy) > T(1.0e-10)) {
+      const T tmp = k * qy;
+      if (fabs(tmp) > lim) {
+        qy = T(2) * tmp - qy;
+        if (qy * tmp <= T(0)) qy = T(0);
+      } else {
+        qy = tmp;
+      }
+    }
+  } else {
+    if (fabs(qx) > T(1.0e-10)) {
+      const T tmp = (T(1) - P::sqrt_(T(1) + (T(4) * dt * C_f) / (h * h) * q_norm)) /
+                    (T(-2) * dt * C_f / (h * h) * (q_norm / qx));
+      if (fabs(tmp) > lim) {
+        qx = T(2) * tmp - qx;
+        if (qx * tmp <= T(0)) qx = T(0);
+      } else {
+        qx = tmp;
+      }
+    }
+    if (fabs(qy) > T(1.0e-10)) {
+      const T tmp = (T(1) - P::sqrt_(T(1) + (T(4) * dt * C_f) / (h * h) * q_norm)) /
+                    (T(-2) * dt * C_f / (h * h) * (q_norm / qy));
+      if (fabs(tmp) > lim) {
+        qy = T(2) * tmp - qy;
+        if (qy * tmp <= T(0)) qy = T(0);
+      } else {
+        qy = tmp;
+      }
+    }
+  }
+}
+
+// CFL time-step candidate of one updated cell (timeControl_Kernel.cu:186-189):
+// the square root is single precision in both instantiations, as in the
+// reference.
+template <class T>
+__device__ inline T cfl_candidate(T h, T qx, T qy, T cfl_dx) {
+  if (!(h > T(kHs))) return T(60.0);
+  const float a = __builtin_sqrtf((float)(T(kG) * h));
+  return cfl_dx / (T(a) + Prec<T>::div(fmax(fabs(qx), fabs(qy)), h));
+}
+
+// Mask-code decode (fluxCal_jh_modified_surface_Kernel.cu:528-535): leading
+// decimal digit = boundary type, remaining digits = series id.
+__host__ __device__ inline void decode_code(int32_t code, int& type, int& id) {
+  int t = code, n = 0;
+  while (t >= 10) {
+    t = (t - (t % 10)) / 10;
+    n++;
+  }
+  int p = 1;
+  for (int k = 0; k < n; ++k) p *= 10;
+  type = t;
+  id = code - t * p;
+}
+
+}  // namespace hp

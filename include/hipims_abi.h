@@ -1,0 +1,182 @@
+/*
+ * hipims_abi.h - C ABI of the MI355X (gfx950) HiPIMS shallow-water step.
+ *
+ * Every entry point takes caller-owned device pointers, plain sizes and an
+ * explicit hipStream_t (passed as void*), launches asynchronously on that
+ * stream, never synchronises and never allocates.  Each returns HP_OK (0) or
+ * a negative HP_E* code; hp_last_error() then holds a message for the calling
+ * thread.
+ *
+ * Two families:
+ *  1. The reference's torch-extension op surface, on the reference's compact
+ *     valid-cell layout (arrays of N valid cells in row-major raster order and
+ *     the int32[5*N] neighbour table of SWE_CUDA.py:173-222).  Each function
+ *     names the reference op it replaces.
+ *  2. The structured-grid path: state held as 2-D planes with a row pitch,
+ *     advanced by a stencil flux pass and a fused update pass with device-
+ *     resident time control (no host round trip per step).  These replace the
+ *     sequence Godunov.addFlux -> addStation_PrecipitationSource ->
+ *     time_friction_euler_update_cuda (SWE_CUDA.py:339-424, 497-502, 780-831).
+ */
+#ifndef HIPIMS_ABI_H
+#define HIPIMS_ABI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HP_ABI_VERSION 1
+
+enum hp_status {
+  HP_OK = 0,
+  HP_EINVAL = -1,   /* bad argument (null pointer, size, dtype, order) */
+  HP_ELAUNCH = -2,  /* the HIP runtime rejected a launch */
+  HP_ESPACE = -3    /* caller-provided workspace too small */
+};
+
+enum hp_dtype { HP_F32 = 0, HP_F64 = 1 };
+
+/* ABI version and the calling thread's last error message. */
+int hp_abi_version(void);
+const char* hp_last_error(void);
+
+/* ------------------------------------------------------------------------
+ * 1. Reference op surface (compact layout).  `dtype` selects float/double for
+ *    every floating-point pointer of the call; scalars dx, t, dt, CFL are
+ *    1-element device arrays as in the reference.
+ * ------------------------------------------------------------------------ */
+
+/* Replaces fluxMask.update (CHANS-HiPIMS/hipims/cuda/fluxMaskGenerator.cpp:24-35,
+ * kernel fluxMaskGenerator_Kernel.cu:19-42).  mask: uint8[N] (torch.bool). */
+int hp_flux_mask(int dtype, int64_t N, uint8_t* mask, const void* h, const int32_t* index, const void* t,
+                 void* stream);
+
+/* Replaces fluxCalculation_jh_modified_surface.addFlux (order 1,
+ * fluxCal_jh_modified_surface.cpp:28-57) and fluxCal_2ndOrder_jh_improved.addFlux
+ * (order 2, fluxCal_2ndOrder_jh_improved.cpp:28-57).  Accumulates into
+ * dh/dqx/dqy for the M cells listed in wet.  normal: [4][2] (may be null:
+ * the E, N, W, S unit normals).  given_*: row-major series tables. */
+int hp_flux_compact(int dtype, int order, int64_t M, int64_t N, const int32_t* wet, void* dh, void* dqx, void* dqy,
+                    const void* h, const void* wl, const void* z, const void* qx, const void* qy,
+                    const int32_t* index, const void* normal, const void* given_depth, int32_t gd_rows,
+                    int32_t gd_w, const void* given_wl, int32_t gwl_rows, int32_t gwl_w, const void* given_q,
+                    int32_t gq_rows, int32_t gq_w, const void* dx, const void* t, const void* dt, void* stream);
+
+/* Replaces station_PrecipitationCalculation.addStation_Precipitation
+ * (stationPrecipitation_interface.cpp:26-43).  rain is float64 when
+ * rain_is_f64 (the reference driver always builds it so, SWE_CUDA.py:483-489),
+ * else of `dtype`. */
+int hp_station_rain(int dtype, int64_t N, void* dh, const int16_t* station, const void* rain, int rain_is_f64,
+                    const void* dt, void* stream);
+
+/* Replaces friction_implicit_andUpdate_jh.addFriction_eulerUpdate
+ * (friction_implicit_andUpdate_jh_interface.cpp:29-55). */
+int hp_friction_update_compact(int dtype, int64_t K, const int32_t* wet, void* dh, void* dqx, void* dqy, void* dz,
+                               const uint8_t* landuse, void* h, void* wl, void* qx, void* qy, void* z,
+                               const void* manning, const void* dt, void* stream);
+
+/* Replaces timeControl.updateTimestep (timeControl.cpp:27-45): h_max and the
+ * per-listed-cell CFL candidate accel[K]. */
+int hp_time_control_compact(int dtype, int64_t K, const int32_t* wet, void* accel, void* h_max, const void* h,
+                            const void* qx, const void* qy, const void* dx, const void* cfl, void* stream);
+
+/* Replaces euler_update.update (euler_update.cpp:27-47). */
+int hp_euler_update_compact(int dtype, int64_t K, const int32_t* wet, const void* dh, const void* dqx,
+                            const void* dqy, const void* dz, void* h, void* wl, void* z, void* qx, void* qy,
+                            void* stream);
+
+/* ------------------------------------------------------------------------
+ * 2. Structured-grid path.
+ * ------------------------------------------------------------------------ */
+
+/* Device-resident time control and mass accounting, all float64 so a host
+ * can view it as double[16].  The first two fields are the reference's
+ * Godunov.t and Godunov.dt. */
+typedef struct hp_ctrl {
+  double t;              /* simulated time                                   */
+  double dt;             /* step used by the next flux / update              */
+  double export_T;       /* export interval (Export_timeStep)                */
+  double export_n;       /* number of export boundaries crossed              */
+  double export_pending; /* boundaries crossed since the host last cleared it */
+  double volume;         /* sum of h over valid cells after the last update  */
+  double rain_volume;    /* cumulative sum of rain*dt (depth units)          */
+  double flux_volume;    /* cumulative sum of flux dh (net boundary inflow)  */
+  double steps;          /* completed steps                                  */
+  double reserved[7];
+} hp_ctrl;
+
+/* Rain source of the update pass. */
+enum hp_rain_mode {
+  HP_RAIN_NONE = 0,
+  HP_RAIN_STATION = 1, /* station table [rain_rows][rain_cols] = [t, r_0..r_S-1] m/s,
+                          linear in time, 0 after the last row (SWE_CUDA.py:472-495);
+                          station plane int16 (null: every cell station 0)     */
+  HP_RAIN_FIELD = 2    /* per-cell float32 plane in m/s (addRadarRainfall)        */
+};
+
+/* One structured domain.  Planes are ny rows of `pitch` elements; columns
+ * [nx, pitch) are padding.  Raster row r, column c of the reference's input
+ * rasters is plane element r*pitch + c.  The raster must have nodata on its
+ * outer ring (the reference's neighbour table wraps otherwise). */
+typedef struct hp_grid_desc {
+  int32_t dtype;  /* HP_F32 / HP_F64 */
+  int32_t order;  /* 1: jh modified surface, 2: MUSCL */
+  int32_t nx, ny; /* raster columns / rows */
+  int64_t pitch;  /* elements per plane row, >= nx */
+  double dx, cfl;
+  int32_t cell0_x, cell0_y; /* first valid cell in row-major order (-1: none) */
+  int32_t n_classes;        /* entries of manning */
+  int32_t rain_mode;        /* hp_rain_mode */
+  /* state planes (dtype) */
+  void *h, *qx, *qy, *z, *h_max;
+  /* flux accumulators written by the flux pass (dtype) */
+  void *dh, *dqx, *dqy;
+  /* cell class planes */
+  const uint8_t* type_slot; /* 0 nodata, else slot of slot_code */
+  const uint8_t* landuse;   /* Manning class, 255 for nodata */
+  const int32_t* slot_code; /* [256] mask code of each slot (device) */
+  const void* manning;      /* [n_classes] (dtype) */
+  /* boundary series (dtype), as Godunov._given_depth/_given_wl/_given_discharge */
+  const void* given_depth;
+  const void* given_wl;
+  const void* given_q;
+  int32_t gd_rows, gd_w, gwl_rows, gwl_w, gq_rows, gq_w;
+  /* rain */
+  const int16_t* station;
+  const double* rain_table;
+  int32_t rain_rows, rain_cols;
+  const float* rain_field;
+  double rain_scale; /* multiplies every rain source (1.0 = reference) */
+  /* control block and reduction workspace (device) */
+  hp_ctrl* ctrl;
+  void* work;
+  int64_t work_bytes;
+} hp_grid_desc;
+
+/* Bytes of device workspace `work` must provide for this domain. */
+int64_t hp_grid_workspace_bytes(const hp_grid_desc* g);
+
+/* Flux pass: dh/dqx/dqy = edge fluxes and bed-slope sources of every listed
+ * cell (Godunov.addFlux with the wet list of fluxMask, SWE_CUDA.py:370-423). */
+int hp_grid_flux(const hp_grid_desc* g, void* stream);
+
+/* Update pass + time control: rain source, implicit friction and update,
+ * h_max, CFL minimum, dt/t advance and export clamp on the device
+ * (SWE_CUDA.py:497-502 then :780-831). */
+int hp_grid_update(const hp_grid_desc* g, void* stream);
+
+/* nsteps x (flux, update). */
+int hp_grid_step(const hp_grid_desc* g, int32_t nsteps, void* stream);
+
+/* out[i] = plane[offsets[i]]  (compact gather, Godunov.get_h). */
+int hp_gather(int dtype, const void* plane, const int64_t* offsets, int64_t n, void* out, void* stream);
+/* plane[offsets[i]] = in[i]  (compact scatter). */
+int hp_scatter(int dtype, void* plane, const int64_t* offsets, int64_t n, const void* in, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HIPIMS_ABI_H */
