@@ -35,19 +35,6 @@ __device__ inline T c_series_at(const T* tab, int rows, int w, int row, int col)
   return tab[row * w + col];
 }
 
-// Water level of a boundary ghost as the reference stores it in field_b[1]
-// (fluxCal_jh_modified_surface_Kernel.cu:262-396): only H/Q/WL-given ghosts
-// get a new one; the others keep the cell's wl.
-template <class T>
-__device__ inline T ghost_wl(int type, T c_wl, const FrameCell<T>& g, T gwl_rel) {
-  switch (type) {
-    case 6:
-    case 7: return g.z + g.h;
-    case 8: return gwl_rel > g.z ? gwl_rel : g.z;
-    default: return c_wl;
-  }
-}
-
 template <class T, int ORDER>
 __global__ __launch_bounds__(kCThreads) void k_flux_compact(
     int64_t M, int64_t N, const int32_t* __restrict__ wet, T* __restrict__ dh, T* __restrict__ dqx,

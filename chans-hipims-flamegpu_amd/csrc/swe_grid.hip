@@ -137,7 +137,8 @@ __device__ inline void solve_tile_edge(const FluxTile<T, TY>& s, const hp_grid_d
     const FrameCell<T> c = frame_cell(s, ra, ca, dAB, zref);
     const BcValues<T> bv = bc_values<T>(a, tA, rows);
     const FrameCell<T> g = bc_ghost<T>(bv.type, c, dAB, bv.gd, bv.gwl - zref, bv.gqx, bv.gqy);
-    const EdgeOut<T> o = solve_edge<T, ORDER>(c, c, g, g, true, c.h + c.z, g.h + g.z, dx, dt);
+    const T etaG = ORDER == 1 ? g.h + g.z : ghost_wl<T>(bv.type, c.h + c.z, g, bv.gwl - zref);
+    const EdgeOut<T> o = solve_edge<T, ORDER>(c, c, g, g, true, c.h + c.z, etaG, dx, dt);
     if (!o.active) return;
     T gx, gy;
     to_global(o.fn, o.ft, dAB, gx, gy);
@@ -150,7 +151,8 @@ __device__ inline void solve_tile_edge(const FluxTile<T, TY>& s, const hp_grid_d
     const FrameCell<T> c = frame_cell(s, rb, cb, dBA, zref);
     const BcValues<T> bv = bc_values<T>(a, tB, rows);
     const FrameCell<T> g = bc_ghost<T>(bv.type, c, dBA, bv.gd, bv.gwl - zref, bv.gqx, bv.gqy);
-    const EdgeOut<T> o = solve_edge<T, ORDER>(c, c, g, g, true, c.h + c.z, g.h + g.z, dx, dt);
+    const T etaG = ORDER == 1 ? g.h + g.z : ghost_wl<T>(bv.type, c.h + c.z, g, bv.gwl - zref);
+    const EdgeOut<T> o = solve_edge<T, ORDER>(c, c, g, g, true, c.h + c.z, etaG, dx, dt);
     if (!o.active) return;
     T gx, gy;
     to_global(o.fn, o.ft, dBA, gx, gy);
@@ -274,7 +276,15 @@ __device__ inline double wave_sum(double v) {
 
 constexpr int kMaxStations = 1024;
 
-template <class T>
+// MODE selects which of the reference driver's update methods the pass is:
+//   HP_UPDATE_STEP      time_friction_euler_update_cuda (SWE_CUDA.py:780-831):
+//                       cells with |dh| > 0 or h >= 0; friction; CFL + dt/t
+//   HP_UPDATE_FRICTION  friction_euler_update_cuda (:766-778): same list, no CFL
+//   HP_UPDATE_EULER     euler_update (:712-724): cells with |dh| > 0 or
+//                       h > 1e-6; q += dq without friction; no CFL
+//   HP_UPDATE_CFL       time_update_cuda (:726-764): cells with h > 1e-6; no
+//                       state change and no rain; h_max, CFL + dt/t
+template <class T, int MODE>
 __global__ __launch_bounds__(kThreads) void k_update(hp_grid_desc a) {
   __shared__ double s_rain[kMaxStations];
   __shared__ double s_red[4][kThreads / 64];
@@ -324,6 +334,15 @@ __global__ __launch_bounds__(kThreads) void k_update(hp_grid_desc a) {
   for (int64_t i = (int64_t)blockIdx.x * kThreads + tid; i < total; i += (int64_t)gridDim.x * kThreads) {
     const int cls = lu[i];
     if (cls == 255) continue;
+    T h = hp_[i];
+    if (MODE == HP_UPDATE_CFL) {
+      if (h > hs) {
+        hmp[i] = fmax(h, hmp[i]);
+        mn = fmin(mn, cfl_candidate<T>(h, qxp[i], qyp[i], cfl_dx));
+      }
+      vol += (double)h;
+      continue;
+    }
     const T dhf = dhp[i];
     T r = T(0);
     if (mode == HP_RAIN_STATION) {
@@ -334,14 +353,17 @@ __global__ __launch_bounds__(kThreads) void k_update(hp_grid_desc a) {
     }
     const T rdt = r * dt;
     const T dht = dhf + rdt;
-    T h = hp_[i];
-    if (!(fabs(dht) > T(0) || h >= T(0))) {
+    const bool listed = (MODE == HP_UPDATE_EULER) ? (fabs(dht) > T(0) || h > hs) : (fabs(dht) > T(0) || h >= T(0));
+    if (!listed) {
       vol += (double)h;
       continue;
     }
     h += dht;
     T qx, qy;
-    if (h < hs) {
+    if (MODE == HP_UPDATE_EULER) {
+      qx = qxp[i] + dqxp[i];
+      qy = qyp[i] + dqyp[i];
+    } else if (h < hs) {
       qx = T(0);
       qy = T(0);
     } else {
@@ -352,8 +374,10 @@ __global__ __launch_bounds__(kThreads) void k_update(hp_grid_desc a) {
     hp_[i] = h;
     qxp[i] = qx;
     qyp[i] = qy;
-    hmp[i] = fmax(h, hmp[i]);
-    mn = fmin(mn, cfl_candidate<T>(h, qx, qy, cfl_dx));
+    if (MODE == HP_UPDATE_STEP) {
+      hmp[i] = fmax(h, hmp[i]);
+      mn = fmin(mn, cfl_candidate<T>(h, qx, qy, cfl_dx));
+    }
     vol += (double)h;
     fsum += (double)dhf;
     rsum += (double)rdt;
@@ -489,16 +513,30 @@ static int launch_flux(const hp_grid_desc* g, hipStream_t st) {
   return g->order == 2 ? launch_flux_t<double, 2>(g, st) : launch_flux_t<double, 1>(g, st);
 }
 
-static int launch_update(const hp_grid_desc* g, hipStream_t st) {
+template <int MODE>
+static void launch_update_mode(const hp_grid_desc* g, int nb, hipStream_t st) {
+  if (g->dtype == HP_F32) hipLaunchKernelGGL((k_update<float, MODE>), dim3(nb), dim3(kThreads), 0, st, *g);
+  else hipLaunchKernelGGL((k_update<double, MODE>), dim3(nb), dim3(kThreads), 0, st, *g);
+}
+
+static int launch_update(const hp_grid_desc* g, int mode, hipStream_t st) {
   const int nb = update_blocks(g);
   if (!g->work || g->work_bytes < (int64_t)nb * (int64_t)sizeof(Partial))
     return fail(HP_ESPACE, "workspace smaller than hp_grid_workspace_bytes()");
-  if (g->dtype == HP_F32) hipLaunchKernelGGL((k_update<float>), dim3(nb), dim3(kThreads), 0, st, *g);
-  else hipLaunchKernelGGL((k_update<double>), dim3(nb), dim3(kThreads), 0, st, *g);
+  switch (mode) {
+    case HP_UPDATE_STEP: launch_update_mode<HP_UPDATE_STEP>(g, nb, st); break;
+    case HP_UPDATE_FRICTION: launch_update_mode<HP_UPDATE_FRICTION>(g, nb, st); break;
+    case HP_UPDATE_EULER: launch_update_mode<HP_UPDATE_EULER>(g, nb, st); break;
+    case HP_UPDATE_CFL: launch_update_mode<HP_UPDATE_CFL>(g, nb, st); break;
+    default: return fail(HP_EINVAL, "unknown update mode");
+  }
   int rc = check_launch("k_update");
   if (rc) return rc;
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, st, g->ctrl, (const Partial*)g->work, nb, g->dx);
-  return check_launch("k_finalize");
+  if (mode == HP_UPDATE_STEP || mode == HP_UPDATE_CFL) {
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, st, g->ctrl, (const Partial*)g->work, nb, g->dx);
+    return check_launch("k_finalize");
+  }
+  return HP_OK;
 }
 
 template <class T>
@@ -529,10 +567,12 @@ int hp_grid_flux(const hp_grid_desc* g, void* stream) {
   return launch_flux(g, (hipStream_t)stream);
 }
 
-int hp_grid_update(const hp_grid_desc* g, void* stream) {
+int hp_grid_update(const hp_grid_desc* g, void* stream) { return hp_grid_update_mode(g, HP_UPDATE_STEP, stream); }
+
+int hp_grid_update_mode(const hp_grid_desc* g, int mode, void* stream) {
   int rc = check_desc(g);
   if (rc) return rc;
-  return launch_update(g, (hipStream_t)stream);
+  return launch_update(g, mode, (hipStream_t)stream);
 }
 
 int hp_grid_step(const hp_grid_desc* g, int32_t nsteps, void* stream) {
@@ -542,7 +582,7 @@ int hp_grid_step(const hp_grid_desc* g, int32_t nsteps, void* stream) {
   for (int32_t k = 0; k < nsteps; ++k) {
     rc = launch_flux(g, (hipStream_t)stream);
     if (rc) return rc;
-    rc = launch_update(g, (hipStream_t)stream);
+    rc = launch_update(g, HP_UPDATE_STEP, (hipStream_t)stream);
     if (rc) return rc;
   }
   return HP_OK;

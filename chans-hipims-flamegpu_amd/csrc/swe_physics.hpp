@@ -130,6 +130,19 @@ __device__ inline FrameCell<T> bc_ghost(int type, const FrameCell<T>& c, int d, 
   return b;
 }
 
+// Water level of a boundary ghost as the reference stores it in field_b[1]
+// (fluxCal_jh_modified_surface_Kernel.cu:262-396): only H/Q/WL-given ghosts
+// get a new one; the others keep the cell's wl.
+template <class T>
+__device__ inline T ghost_wl(int type, T c_wl, const FrameCell<T>& g, T gwl_rel) {
+  switch (type) {
+    case 6:
+    case 7: return g.z + g.h;
+    case 8: return gwl_rel > g.z ? gwl_rel : g.z;
+    default: return c_wl;
+  }
+}
+
 // Result of one edge solve, already scaled by dt/dx as the reference
 // accumulates it:  e = (raw / dx) * dt.
 template <class T> struct EdgeOut {

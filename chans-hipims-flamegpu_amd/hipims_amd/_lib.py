@@ -21,6 +21,7 @@ CSRC_DIR = os.path.join(PKG_DIR, "csrc")
 HP_OK, HP_EINVAL, HP_ELAUNCH, HP_ESPACE = 0, -1, -2, -3
 HP_F32, HP_F64 = 0, 1
 HP_RAIN_NONE, HP_RAIN_STATION, HP_RAIN_FIELD = 0, 1, 2
+HP_UPDATE_STEP, HP_UPDATE_FRICTION, HP_UPDATE_EULER, HP_UPDATE_CFL = 0, 1, 2, 3
 ABI_VERSION = 1
 
 _P = ctypes.c_void_p
@@ -72,6 +73,7 @@ SIGNATURES = {
     "hp_grid_workspace_bytes": (_I64, [ctypes.POINTER(HpGridDesc)]),
     "hp_grid_flux": (_I, [ctypes.POINTER(HpGridDesc), _P]),
     "hp_grid_update": (_I, [ctypes.POINTER(HpGridDesc), _P]),
+    "hp_grid_update_mode": (_I, [ctypes.POINTER(HpGridDesc), _I, _P]),
     "hp_grid_step": (_I, [ctypes.POINTER(HpGridDesc), _I32, _P]),
     "hp_gather": (_I, [_I, _P, _P, _I64, _P, _P]),
     "hp_scatter": (_I, [_I, _P, _P, _I64, _P, _P]),

@@ -92,7 +92,7 @@ class GridSolver:
 
     def __init__(self, mask, h, qx, qy, z, dx, *, cfl=0.5, order=2, dtype=torch.float64, device="cuda",
                  first_dt=1.0e-4, t=0.0, export_T=math.inf, export_n=0, manning=(0.0,), landuse=None,
-                 given_depth=None, given_wl=None, given_q=None):
+                 given_depth=None, given_wl=None, given_q=None, ctrl=None):
         self.device = torch.device(device)
         if self.device.type == "cuda" and self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())
@@ -124,11 +124,17 @@ class GridSolver:
         self.offsets = torch.from_numpy(L.offsets).to(dev)
         self.set_manning(manning)
         self.set_landuse(landuse)
-        self.ctrl = torch.zeros(_lib.CTRL_DOUBLES, dtype=torch.float64, device=dev)
-        self.ctrl[0] = float(t)
-        self.ctrl[1] = float(first_dt)
-        self.ctrl[2] = float(export_T)
-        self.ctrl[3] = float(export_n)
+        if ctrl is not None:
+            # caller-owned control block (t, dt, export_T, export_n already set)
+            if ctrl.dtype != torch.float64 or ctrl.numel() < _lib.CTRL_DOUBLES or ctrl.device != dev:
+                raise ValueError("ctrl must be a float64 tensor of 16 entries on the solver's device")
+            self.ctrl = ctrl
+        else:
+            self.ctrl = torch.zeros(_lib.CTRL_DOUBLES, dtype=torch.float64, device=dev)
+            self.ctrl[0] = float(t)
+            self.ctrl[1] = float(first_dt)
+            self.ctrl[2] = float(export_T)
+            self.ctrl[3] = float(export_n)
         self.given_depth = self.given_wl = self.given_q = None
         self.set_boundary(given_depth, given_wl, given_q)
         self.station = None
@@ -259,8 +265,16 @@ class GridSolver:
     def flux(self):
         check(lib().hp_grid_flux(ctypes.byref(self.desc), self._stream()), "hp_grid_flux")
 
-    def update(self):
-        check(lib().hp_grid_update(ctypes.byref(self.desc), self._stream()), "hp_grid_update")
+    def update(self, mode: int = _lib.HP_UPDATE_STEP, rain: bool = True):
+        """One update pass; `rain` False leaves the configured rain source out."""
+        d = self.desc
+        saved = d.rain_mode
+        if not rain:
+            d.rain_mode = _lib.HP_RAIN_NONE
+        try:
+            check(lib().hp_grid_update_mode(ctypes.byref(d), int(mode), self._stream()), "hp_grid_update_mode")
+        finally:
+            d.rain_mode = saved
 
     def step(self, nsteps: int = 1):
         check(lib().hp_grid_step(ctypes.byref(self.desc), int(nsteps), self._stream()), "hp_grid_step")

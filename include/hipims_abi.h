@@ -167,6 +167,17 @@ int hp_grid_flux(const hp_grid_desc* g, void* stream);
  * (SWE_CUDA.py:497-502 then :780-831). */
 int hp_grid_update(const hp_grid_desc* g, void* stream);
 
+/* Which reference driver method an update pass stands for. */
+enum hp_update_mode {
+  HP_UPDATE_STEP = 0,     /* time_friction_euler_update_cuda (SWE_CUDA.py:780-831) */
+  HP_UPDATE_FRICTION = 1, /* friction_euler_update_cuda (:766-778): no time control */
+  HP_UPDATE_EULER = 2,    /* euler_update (:712-724): no friction, no time control */
+  HP_UPDATE_CFL = 3       /* time_update_cuda (:726-764): time control only */
+};
+
+/* The update pass in the given mode (hp_grid_update == HP_UPDATE_STEP). */
+int hp_grid_update_mode(const hp_grid_desc* g, int mode, void* stream);
+
 /* nsteps x (flux, update). */
 int hp_grid_step(const hp_grid_desc* g, int32_t nsteps, void* stream);
 
