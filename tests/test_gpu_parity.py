@@ -1,0 +1,333 @@
+"""HIP path vs the CPU oracle on the same inputs (GPU tests).
+
+Bar: FP64 results within 1e-10 relative (max |a - b| / max |b| per field) of
+the oracle's restatement of the reference update, for the same state, t and
+dt.  Multi-step runs are compared in lockstep: before every step the HIP state
+is overwritten with the oracle's, so a rounding-level difference cannot grow
+through a wet/dry threshold into a different trajectory.
+"""
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import box, compact, rel_err, random_state
+from hipims_amd import _lib, ops
+from hipims_amd import generators as gen
+from hipims_amd.grid import GridSolver
+
+pytestmark = pytest.mark.gpu
+TOL64 = 1e-10
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def make_pair(orc, case, order, dtype=torch.float64, **kw):
+    g = GridSolver(case.mask, case.h, case.qx, case.qy, case.z, case.dx, cfl=case.cfl, order=order, dtype=dtype,
+                   first_dt=case.first_dt, manning=case.manning, landuse=case.landuse,
+                   given_depth=kw.get("gd"), given_wl=kw.get("gw"), given_q=kw.get("gq"))
+    o = orc.OracleSim(case.mask, case.h, case.qx, case.qy, case.z, case.dx, cfl=case.cfl, order=order,
+                      first_dt=case.first_dt, manning=case.manning, landuse=case.landuse,
+                      given_depth=kw.get("gd"), given_wl=kw.get("gw"), given_q=kw.get("gq"),
+                      rain_table=kw.get("rain"), station=kw.get("station"))
+    if kw.get("rain") is not None:
+        g.set_station_rain(kw["rain"], kw.get("station"))
+    return g, o
+
+
+def sync_to(g, o):
+    dt_ = g.dtype
+    for name, arr in (("h", o.h_c), ("qx", o.qx_c), ("qy", o.qy_c), ("h_max", o.h_max)):
+        g.load_compact(getattr(g, name), torch.from_numpy(arr).to(dt_))
+    g.ctrl[0] = o.time
+    g.ctrl[1] = o.dt
+
+
+def compare(g, o, tag, tol=TOL64):
+    torch.cuda.synchronize()
+    errs = {}
+    for name, ref in (("h", o.h_c), ("qx", o.qx_c), ("qy", o.qy_c), ("h_max", o.h_max)):
+        got = g.compact(getattr(g, name)).cpu().numpy()
+        errs[name] = rel_err(got, ref)
+    errs["dt"] = abs(g.dt - o.dt) / o.dt
+    errs["t"] = abs(g.t - o.time) / max(o.time, 1e-300)
+    bad = {k: v for k, v in errs.items() if not v <= tol}
+    assert not bad, f"{tag}: {bad}"
+    return errs
+
+
+def lockstep(g, o, steps, tag, warm=0):
+    o.run(warm)
+    worst = 0.0
+    for k in range(steps):
+        sync_to(g, o)
+        g.step(1)
+        o.run(1)
+        e = compare(g, o, f"{tag} step {k}")
+        worst = max(worst, max(e.values()))
+    return worst
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_dam_break_fp64_lockstep(oracle_lib, cuda, order):
+    say(f"[dam_break 64^2 order {order}]")
+    c = gen.dam_break(64, wall=30)
+    c.h[20:30, 10:20] += 0.3
+    g, o = make_pair(oracle_lib, c, order)
+    worst = lockstep(g, o, 30, f"dam_break o{order}")
+    say(f"  worst rel err {worst:.2e}")
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_rainfall_runoff_fp64_lockstep(oracle_lib, cuda, order):
+    say(f"[rainfall_runoff 96^2 order {order}]")
+    c = gen.rainfall_runoff(96, 5.0, rain_mmh=200.0, relief=10.0)
+    g, o = make_pair(oracle_lib, c, order, rain=c.rain_table, station=np.zeros_like(c.mask))
+    # warm the oracle into a wet/dry state first, then compare step by step
+    worst = lockstep(g, o, 25, f"runoff o{order}", warm=40)
+    say(f"  worst rel err {worst:.2e}, t = {o.time:.1f} s, wet cells {(o.h_c > 1e-6).sum()}")
+
+
+def bc_case(n=40, seed=9):
+    z, h, qx, qy = random_state(n, n + 7, seed=seed, wet=False, rough=0.3)
+    mask, H, QX, QY, Z = box(n, n + 7, wall=30, z=z, h=h, qx=qx, qy=qy)
+    # boundary segments of every type; the west edge is an H-given inflow (61),
+    # north has Q-given (70), WL-given (80), fall (9), open (50), slip (40)
+    mask[1:-1, 1] = 61
+    mask[1, 2:10] = 70
+    mask[1, 10:18] = 80
+    mask[1, 18:26] = 9
+    mask[-2, 2:20] = 50
+    mask[-2, 20:30] = 40
+    gd = np.array([[0.0, 0.0, 0.8], [5.0, 0.0, 1.2], [1e9, 0.0, 0.4]])
+    gw = np.array([[0.0, 0.9], [1e9, 1.1]])
+    gq = np.array([[0.0, 0.3, -0.6], [3.0, -0.2, -0.9], [1e9, 0.0, 0.0]])
+    c = gen.Case("bc_mix", mask, H, QX, QY, Z, 1.0, np.zeros_like(mask, dtype=np.int64), np.array([0.025]), 0.5, 0.01)
+    return c, gd, gw, gq
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_boundary_types_fp64_lockstep(oracle_lib, cuda, order):
+    say(f"[bc_mix order {order}]")
+    c, gd, gw, gq = bc_case()
+    g, o = make_pair(oracle_lib, c, order, gd=gd, gw=gw, gq=gq)
+    worst = lockstep(g, o, 40, f"bc o{order}")
+    say(f"  worst rel err {worst:.2e}, t = {o.time:.2f}")
+
+
+def test_cell0_quirk_and_station_rain(oracle_lib, cuda):
+    say("[cell0 + stations]")
+    n = 24
+    h = np.zeros((n, n))
+    h[0, 0] = 0.8             # compact cell 0 wet, its neighbours dry
+    h[10:14, 10:14] = 0.5
+    mask, H, QX, QY, Z = box(n, n, wall=40, h=h)
+    st = np.zeros_like(mask)
+    st[:, n // 2:] = 1
+    st[n // 2:, :] += 2        # 4 stations
+    table = np.array([[0.0, 1e-4, 2e-4, 0.0, 5e-5], [30.0, 3e-4, 0.0, 1e-4, 5e-5], [60.0, 0.0, 1e-4, 2e-4, 0.0]])
+    c = gen.Case("cell0", mask, H, QX, QY, Z, 1.0, np.zeros_like(mask, dtype=np.int64), np.array([0.03]), 0.5, 0.05)
+    g, o = make_pair(oracle_lib, c, 2, rain=table, station=st)
+    o.run(1)  # leave t == 0 (every cell listed there)
+    worst = lockstep(g, o, 30, "cell0")
+    say(f"  worst rel err {worst:.2e}")
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_compact_ops_match_oracle(oracle_lib, cuda, dtype):
+    say(f"[compact ops {dtype}]")
+    orc = oracle_lib
+    c, gd, gw, gq = bc_case(30, seed=4)
+    np_dt = np.float64
+    idx = orc.build_index(c.mask)
+    h, qx, qy, z = (compact(c.mask, a) for a in (c.h, c.qx, c.qy, c.z))
+    wl = h + z
+    N = h.size
+    tol = TOL64 if dtype == torch.float64 else 2e-4
+    T = lambda a, d=dtype: torch.from_numpy(np.ascontiguousarray(a)).to(device=cuda, dtype=d)
+    t_now, dt, dx = 2.0, 0.02, 1.0
+    # fluxMask
+    fm = torch.zeros(N, dtype=torch.bool, device=cuda)
+    ops.flux_mask_update(fm, T(h), T(idx, torch.int32), T([t_now]))
+    ref_list = orc.flux_list(h, idx, t_now)
+    assert np.array_equal(np.flatnonzero(fm.cpu().numpy()), ref_list)
+    wet = ref_list
+    for order in (1, 2):
+        dh, dqx, dqy = (np.full(N, 0.01) for _ in range(3))  # accumulators hold prior sources
+        gdh, gdqx, gdqy = T(dh), T(dqx), T(dqy)
+        fn = ops.add_flux_order1 if order == 1 else ops.add_flux_order2
+        fn(T(wet, torch.int32), gdh, gdqx, gdqy, T(h), T(wl), T(z), T(qx), T(qy), T(idx, torch.int32),
+           T(orc.NORMAL), T(gd), T(gw), T(gq), T([dx]), T([t_now]), T([dt]))
+        orc.flux(order, wet, dh, dqx, dqy, h, wl, z, qx, qy, idx, gd, gw, gq, dx, t_now, dt)
+        torch.cuda.synchronize()
+        for name, a, b in (("dh", gdh, dh), ("dqx", gdqx, dqx), ("dqy", gdqy, dqy)):
+            e = rel_err(a.cpu().numpy() - 0.01, b - 0.01)
+            assert e <= tol, f"flux o{order} {name}: {e:.2e}"
+    # station rain (float64 table, as the reference driver builds it)
+    st = (np.arange(N) % 3).astype(np.int16)
+    rain = np.array([1e-4, 2e-5, 0.0])
+    dh = np.linspace(-1e-3, 1e-3, N)
+    gdh = T(dh)
+    ops.add_station_precipitation(gdh, T(st, torch.int16), T(rain, torch.float64), T([dt]))
+    orc.station_rain(dh, st, rain, dt)
+    assert rel_err(gdh.cpu().numpy(), dh) <= tol
+    # friction + update, then time control, then euler
+    rng = np.random.Generator(np.random.PCG64(1))
+    dh = rng.normal(0, 1e-3, N)
+    dh[::7] = -h[::7] - 1e-3  # some cells dry out (h < 0 kept, as the reference)
+    dqx, dqy, dz = rng.normal(0, 1e-3, N), rng.normal(0, 1e-3, N), np.zeros(N)
+    lu = (np.arange(N) % 2).astype(np.uint8)
+    man = np.array([0.03, 0.06])
+    listed = orc.update_list(dh, h)
+    G = {k: T(v) for k, v in dict(dh=dh, dqx=dqx, dqy=dqy, dz=dz, h=h, wl=wl, qx=qx, qy=qy, z=z).items()}
+    ops.add_friction_euler_update(T(listed, torch.int32), G["dh"], G["dqx"], G["dqy"], G["dz"], T(lu, torch.uint8),
+                                  G["h"], G["wl"], G["qx"], G["qy"], G["z"], T(man), T([dt]))
+    R = {k: v.copy() for k, v in dict(dh=dh, dqx=dqx, dqy=dqy, dz=dz, h=h, wl=wl, qx=qx, qy=qy, z=z).items()}
+    orc.friction_update(listed, R["dh"], R["dqx"], R["dqy"], R["dz"], lu, R["h"], R["wl"], R["qx"], R["qy"], R["z"],
+                        man, dt)
+    torch.cuda.synchronize()
+    for k in ("h", "qx", "qy", "wl", "dh"):
+        assert rel_err(G[k].cpu().numpy(), R[k]) <= max(tol, 1e-12 if k == "dh" else tol), k
+    acc_g = torch.full((listed.size,), 60.0, dtype=dtype, device=cuda)
+    hmax_g = T(np.zeros(N))
+    ops.update_timestep(T(listed, torch.int32), acc_g, hmax_g, G["h"], G["qx"], G["qy"], T([dx]), T([0.5]),
+                        T([t_now]), T([dt]))
+    acc_r, hmax_r = np.full(listed.size, 60.0), np.zeros(N)
+    orc.time_control(listed, acc_r, hmax_r, R["h"], R["qx"], R["qy"], dx, 0.5)
+    torch.cuda.synchronize()
+    assert rel_err(acc_g.cpu().numpy(), acc_r) <= max(tol, 1e-7)  # single-precision sqrt in both
+    assert rel_err(hmax_g.cpu().numpy(), hmax_r) <= tol
+    ops.euler_update(T(listed, torch.int32), T(dh), T(dqx), T(dqy), T(dz), G["h"], G["wl"], G["z"], G["qx"], G["qy"])
+    orc.euler_update(listed, dh, dqx, dqy, dz, R["h"], R["wl"], R["z"], R["qx"], R["qy"])
+    torch.cuda.synchronize()
+    for k in ("h", "qx", "qy", "wl"):
+        assert rel_err(G[k].cpu().numpy(), R[k]) <= tol, k
+
+
+def test_update_modes_match_reference_methods(oracle_lib, cuda):
+    """friction_euler_update_cuda, euler_update and time_update_cuda."""
+    say("[update modes]")
+    orc = oracle_lib
+    c = gen.rainfall_runoff(48, 5.0, rain_mmh=100.0, relief=5.0)
+    g, o = make_pair(orc, c, 2, rain=c.rain_table, station=np.zeros_like(c.mask))
+    o.run(30)
+    for mode in (_lib.HP_UPDATE_FRICTION, _lib.HP_UPDATE_EULER, _lib.HP_UPDATE_CFL):
+        sync_to(g, o)
+        R = {k: getattr(o, k).copy() for k in ("h_c", "qx_c", "qy_c", "z_c", "h_max")}
+        idx = o.index
+        dt, t_now = o.dt, o.time
+        dh, dqx, dqy, dz = (np.zeros(o.N) for _ in range(4))
+        if mode != _lib.HP_UPDATE_CFL:
+            g.flux()
+            g.update(mode, rain=False)
+            orc.flux(2, orc.flux_list(R["h_c"], idx, t_now), dh, dqx, dqy, R["h_c"], R["h_c"] + R["z_c"], R["z_c"],
+                     R["qx_c"], R["qy_c"], idx, [[0.0, 0.0]], [[0.0, 0.0]], [[0.0, 0.0, 0.0]], c.dx, t_now, dt)
+            wl = R["h_c"] + R["z_c"]
+            if mode == _lib.HP_UPDATE_FRICTION:
+                lst = orc.update_list(dh, R["h_c"])
+                orc.friction_update(lst, dh, dqx, dqy, dz, compact(c.mask, c.landuse, np.uint8), R["h_c"], wl,
+                                    R["qx_c"], R["qy_c"], R["z_c"], c.manning, dt)
+            else:
+                lst = np.flatnonzero((np.abs(dh) > 0) | (R["h_c"] > 1e-6)).astype(np.int32)
+                orc.euler_update(lst, dh, dqx, dqy, dz, R["h_c"], wl, R["z_c"], R["qx_c"], R["qy_c"])
+            torch.cuda.synchronize()
+            assert abs(g.dt - dt) == 0.0 and abs(g.t - t_now) == 0.0   # no time control
+        else:
+            g.update(mode, rain=False)
+            lst = np.flatnonzero(R["h_c"] > 1e-6).astype(np.int32)
+            acc = np.full(lst.size, 60.0)
+            orc.time_control(lst, acc, R["h_max"], R["h_c"], R["qx_c"], R["qy_c"], c.dx, c.cfl)
+            dt_new = min(acc.min() if lst.size else dt, 10 * dt)
+            torch.cuda.synchronize()
+            assert abs(g.dt - dt_new) <= 1e-15 * dt_new and abs(g.t - (t_now + dt_new)) <= 1e-12 * g.t
+        for k, plane in (("h_c", g.h), ("qx_c", g.qx), ("qy_c", g.qy), ("h_max", g.h_max)):
+            e = rel_err(g.compact(plane).cpu().numpy(), R[k])
+            assert e <= TOL64, f"mode {mode} {k}: {e:.2e}"
+
+
+def test_godunov_dropin_server_sequence(oracle_lib, cuda, tmp_path):
+    """The server's calls (CHANS-HiPIMS_server.py:246-256) through the drop-in
+    class, with exports, vs the oracle driver."""
+    say("[Godunov drop-in]")
+    from hipims_amd.SWE_CUDA import Godunov
+    c = gen.rainfall_runoff(40, 5.0, rain_mmh=300.0, relief=3.0)
+    dev = cuda
+    num = Godunov(dev, c.dx, c.cfl, 120.0, t=0.0, export_n=0, firstTimeStep=1.0, secondOrder=False,
+                  tensorType=torch.float64)
+    num.setOutPutPath(str(tmp_path))
+    T = lambda a: torch.as_tensor(np.asarray(a))
+    num.init__fluidField_tensor(T(c.mask), T(c.h), T(c.qx), T(c.qy), T(c.h + c.z), T(c.z), dev)
+    num.set__frictionField_tensor(c.manning, dev)
+    num.set_landuse(T(c.mask), T(c.landuse), dev)
+    num.set_distributed_rainfall_station_Mask(T(c.mask), T(np.zeros_like(c.mask)), dev)
+    num.set_boundary_tensor({}, dev)
+    o = oracle_lib.OracleSim(c.mask, c.h, c.qx, c.qy, c.z, c.dx, cfl=c.cfl, order=1, first_dt=1.0, export_T=120.0,
+                             manning=c.manning, landuse=c.landuse, rain_table=c.rain_table,
+                             station=np.zeros_like(c.mask))
+    target = np.array([3, 7, 12, 20]) + (c.mask[1:-1, 1:-1].size // 2)
+    idx_abm = torch.from_numpy(target)
+    for k in range(60):
+        num.addFlux()
+        num.addStation_PrecipitationSource(c.rain_table, dev)
+        num.time_friction_euler_update_cuda(dev)
+        o.run(1)
+        res = num.get_h()[idx_abm.to(dev)].cpu().numpy()
+        assert rel_err(res, o.h_c[target]) <= 1e-9, k
+        assert abs(num.t.item() - o.time) <= 1e-9 * o.time and abs(num.dt.item() - o.dt) <= 1e-9 * o.dt
+    assert o.exports >= 1
+    names = sorted(os.listdir(tmp_path))
+    assert any(n.startswith("h_tensor([120.") for n in names), names
+    assert {n.split("_tensor")[0] for n in names} == {"h", "qx", "qy", "wl", "h_max"}
+
+
+def test_fp32_error_against_fp64_oracle(oracle_lib, cuda):
+    """FP32 structured path vs the FP64 oracle over a rainfall-runoff run:
+    max depth / discharge error and volume-balance error (stated, bounded)."""
+    say("[fp32 vs fp64 oracle]")
+    c = gen.rainfall_runoff(128, 5.0, rain_mmh=100.0)
+    g, o = make_pair(oracle_lib, c, 2, dtype=torch.float32, rain=c.rain_table, station=np.zeros_like(c.mask))
+    steps = 300
+    g.step(steps)
+    o.run(steps)
+    torch.cuda.synchronize()
+    h32 = g.compact(g.h).double().cpu().numpy()
+    q32 = np.hypot(g.compact(g.qx).double().cpu().numpy(), g.compact(g.qy).double().cpu().numpy())
+    q64 = np.hypot(o.qx_c, o.qy_c)
+    dh = np.abs(h32 - o.h_c).max()
+    dq = np.abs(q32 - q64).max()
+    ctrl = g.ctrl_dict()
+    area = c.dx * c.dx
+    v_gpu = ctrl["volume"]
+    balance = (v_gpu - (ctrl["rain_volume"] + ctrl["flux_volume"])) / ctrl["rain_volume"]
+    v_orc = o.h_c.sum() * area
+    say(f"  t32 {g.t:.1f} t64 {o.time:.1f}; max|dh| {dh:.3e} m (max h {o.h_c.max():.3f}), max|dq| {dq:.3e} m2/s, "
+        f"volume fp32 {v_gpu:.6e} fp64 {v_orc:.6e} (rel {abs(v_gpu - v_orc) / v_orc:.2e}), balance err {balance:.2e}")
+    assert abs(g.t - o.time) / o.time < 1e-3
+    assert dh < 5e-3 and dq < 5e-3
+    assert abs(balance) < 1e-4
+    assert abs(v_gpu - v_orc) / v_orc < 1e-3
+
+
+def test_full_size_fp32_properties(cuda):
+    """2048^2 rainfall-runoff, FP32: finite state, volume closure."""
+    say("[2048^2 fp32 properties]")
+    c = gen.rainfall_runoff(2048, 5.0, rain_mmh=50.0)
+    g = GridSolver(c.mask, c.h, c.qx, c.qy, c.z, c.dx, cfl=0.5, order=2, dtype=torch.float32, first_dt=c.first_dt,
+                   manning=c.manning, landuse=c.landuse)
+    g.set_station_rain(c.rain_table)
+    g.step(40)
+    torch.cuda.synchronize()
+    ctrl = g.ctrl_dict()
+    assert ctrl["steps"] == 40
+    h = g.h[:, : g.layout.nx]
+    assert torch.isfinite(h).all() and torch.isfinite(g.qx).all() and torch.isfinite(g.qy).all()
+    balance = (ctrl["volume"] - ctrl["rain_volume"] - ctrl["flux_volume"]) / ctrl["rain_volume"]
+    say(f"  t {ctrl['t']:.1f} s, volume {ctrl['volume']:.4e} m3, rain {ctrl['rain_volume']:.4e}, "
+        f"balance err {balance:.2e}")
+    assert abs(balance) < 1e-5
