@@ -6,21 +6,29 @@
 // so the raster's own nodata ring doubles as the stencil apron.
 //
 // Per step:
-//   k_flux      tile of 64 x TY cells + 2-cell cross halo staged in LDS; every
-//               edge touching the tile is solved once (swe_physics.hpp) and
-//               both cells take their share; listed cells (the reference's
-//               fluxMask rule) write dh/dqx/dqy.
-//   k_update    one pass over the planes: rain, implicit friction + update,
-//               h_max, the CFL minimum (wave64 shuffle + LDS), FP64 volume /
-//               flux / rain sums; one partial record per block.
-//   k_finalize  one block folds the partials and advances dt, t and the
-//               export clamp in the device control block (SWE_CUDA.py:818-831).
+//   flux    tile of 64 x TY cells + 2-cell cross halo staged in LDS; every
+//           edge touching the tile is solved once (swe_physics.hpp) and both
+//           cells take their share; listed cells (the reference's fluxMask
+//           rule) write dh/dqx/dqy.
+//   update  a contiguous chunk of plane elements per block: rain, implicit
+//           friction + update, h_max, the CFL minimum (wave64 shuffle + LDS),
+//           FP64 volume / flux / rain sums; one partial record per block.
+//   finalize one block per domain folds its partials and advances dt, t and
+//           the export clamp in the domain's control block (SWE_CUDA.py:818-831).
 // No host synchronisation anywhere in the step.
+//
+// Each pass is a device function over (domain, tile or chunk).  Single-domain
+// launches take the domain descriptor as a kernel argument; ensemble launches
+// take a device array of descriptors plus a (domain, local index) map per
+// block, so one launch advances every domain of an ensemble by its own dt.
+// A domain whose control block has t_stop > 0 and t >= t_stop is skipped.
 
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
+#include <cstring>
 #include <string>
+#include <vector>
 
 #include "../../include/hipims_abi.h"
 #include "hp_common.hpp"
@@ -30,11 +38,11 @@ namespace hp {
 
 constexpr int kTX = 64;
 constexpr int kThreads = 256;
-constexpr int kUpdateMaxBlocks = 2048;
+constexpr int kUpdateIters = 16;  // plane elements per thread per update block
+constexpr int kChunk = kThreads * kUpdateIters;
+constexpr int kMaxStations = 1024;
 
-template <class T> struct TileRows;
-template <> struct TileRows<float> { static constexpr int v = 8; };
-template <> struct TileRows<double> { static constexpr int v = 4; };
+__device__ inline bool stopped(const hp_ctrl* c) { return c->t_stop > 0.0 && c->t >= c->t_stop; }
 
 // --- boundary series ------------------------------------------------------
 template <class T>
@@ -84,13 +92,15 @@ template <class T> __device__ inline void to_global(T fn, T ft, int d, T& gx, T&
 // LDS tile of one flux block.
 template <class T, int TY> struct FluxTile {
   static constexpr int LW = kTX + 4, LH = TY + 4;
+  static constexpr int NX = TY * (kTX + 1);  // x-edges
+  static constexpr int NY = (TY + 1) * kTX;  // y-edges
   T h[LH][LW], qx[LH][LW], qy[LH][LW], z[LH][LW];
   uint8_t ty[LH][LW];
   // per edge: 0 mass, 1 qx, 2 qy (from side A's view, to subtract at A and
   // add at B), 3 bed term of A, 4 bed term of B (each to subtract along the
   // edge axis).  A = west / north cell, B = east / south cell.
-  T ex[TY][kTX + 1][5];
-  T ey[TY + 1][kTX][5];
+  // x-edge (ly, lx) is record ly*(TX+1)+lx; y-edge (ly, lx) is NX + ly*TX+lx.
+  T e[NX + NY][5];
 };
 
 template <class T, int TY>
@@ -132,43 +142,36 @@ __device__ inline void solve_tile_edge(const FluxTile<T, TY>& s, const hp_grid_d
     const T nA = T(axis_x ? nrm_x(dAB) : nrm_y(dAB));
     out[3] = o.bed_c * nA;
     out[4] = o.bed_r * (-nA);
-  } else if (tA != 0) {
-    const T zref = Prec<T>::local_datum ? s.z[ra][ca] : T(0);
-    const FrameCell<T> c = frame_cell(s, ra, ca, dAB, zref);
-    const BcValues<T> bv = bc_values<T>(a, tA, rows);
-    const FrameCell<T> g = bc_ghost<T>(bv.type, c, dAB, bv.gd, bv.gwl - zref, bv.gqx, bv.gqy);
-    const T etaG = ORDER == 1 ? g.h + g.z : ghost_wl<T>(bv.type, c.h + c.z, g, bv.gwl - zref);
-    const EdgeOut<T> o = solve_edge<T, ORDER>(c, c, g, g, true, c.h + c.z, etaG, dx, dt);
-    if (!o.active) return;
-    T gx, gy;
-    to_global(o.fn, o.ft, dAB, gx, gy);
-    out[0] = o.fh;
-    out[1] = gx;
-    out[2] = gy;
-    out[3] = o.bed_c * T(axis_x ? nrm_x(dAB) : nrm_y(dAB));
   } else {
-    const T zref = Prec<T>::local_datum ? s.z[rb][cb] : T(0);
-    const FrameCell<T> c = frame_cell(s, rb, cb, dBA, zref);
-    const BcValues<T> bv = bc_values<T>(a, tB, rows);
-    const FrameCell<T> g = bc_ghost<T>(bv.type, c, dBA, bv.gd, bv.gwl - zref, bv.gqx, bv.gqy);
+    // boundary edge, solved from the valid side with its outward normal
+    const bool from_a = tA != 0;
+    const int rc = from_a ? ra : rb, cc = from_a ? ca : cb, d = from_a ? dAB : dBA;
+    const T zref = Prec<T>::local_datum ? s.z[rc][cc] : T(0);
+    const FrameCell<T> c = frame_cell(s, rc, cc, d, zref);
+    const BcValues<T> bv = bc_values<T>(a, from_a ? tA : tB, rows);
+    const FrameCell<T> g = bc_ghost<T>(bv.type, c, d, bv.gd, bv.gwl - zref, bv.gqx, bv.gqy);
     const T etaG = ORDER == 1 ? g.h + g.z : ghost_wl<T>(bv.type, c.h + c.z, g, bv.gwl - zref);
     const EdgeOut<T> o = solve_edge<T, ORDER>(c, c, g, g, true, c.h + c.z, etaG, dx, dt);
     if (!o.active) return;
     T gx, gy;
-    to_global(o.fn, o.ft, dBA, gx, gy);
-    out[0] = -o.fh;
-    out[1] = -gx;
-    out[2] = -gy;
-    out[4] = o.bed_c * T(axis_x ? nrm_x(dBA) : nrm_y(dBA));
+    to_global(o.fn, o.ft, d, gx, gy);
+    const T bed = o.bed_c * T(axis_x ? nrm_x(d) : nrm_y(d));
+    if (from_a) {
+      out[0] = o.fh; out[1] = gx; out[2] = gy; out[3] = bed;
+    } else {
+      out[0] = -o.fh; out[1] = -gx; out[2] = -gy; out[4] = bed;
+    }
   }
 }
 
 template <class T, int ORDER, int TY>
-__global__ __launch_bounds__(kThreads) void k_flux(hp_grid_desc a) {
+__device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by) {
   __shared__ FluxTile<T, TY> s;
-  constexpr int LW = FluxTile<T, TY>::LW, LH = FluxTile<T, TY>::LH;
+  using Tile = FluxTile<T, TY>;
+  constexpr int LW = Tile::LW, LH = Tile::LH;
+  if (stopped(a.ctrl)) return;
   const int tid = threadIdx.x;
-  const int x0 = blockIdx.x * kTX, y0 = blockIdx.y * TY;
+  const int x0 = bx * kTX, y0 = by * TY;
   const int nx = a.nx, ny = a.ny;
   const int64_t pitch = a.pitch;
   const T* __restrict__ h = (const T*)a.h;
@@ -189,8 +192,7 @@ __global__ __launch_bounds__(kThreads) void k_flux(hp_grid_desc a) {
     s.z[r][c] = in ? z[gi] : T(0);
   }
 
-  const double t_d = a.ctrl->t;
-  const T t = T(t_d);
+  const T t = T(a.ctrl->t);
   const T dt = T(a.ctrl->dt);
   const T dx = T(a.dx);
   SeriesRows rows;
@@ -199,17 +201,24 @@ __global__ __launch_bounds__(kThreads) void k_flux(hp_grid_desc a) {
   rows.q = a.given_q ? series_row<T>((const T*)a.given_q, a.gq_rows, a.gq_w, t) : 0;
   __syncthreads();
 
-  // x-edges: row ly, between local columns lx-1 (A) and lx (B), lx = 0..64.
-  for (int e = tid; e < TY * (kTX + 1); e += kThreads) {
-    const int ly = e / (kTX + 1), lx = e - ly * (kTX + 1);
-    const int r = ly + 2, ca = lx + 1, cb = lx + 2;
-    solve_tile_edge<T, ORDER, TY>(s, a, rows, r, ca, r, cb, r, ca - 1, r, cb + 1, 0, dx, dt, s.ex[ly][lx]);
-  }
-  // y-edges: column lx, between local rows ly-1 (A, north) and ly (B, south).
-  for (int e = tid; e < (TY + 1) * kTX; e += kThreads) {
-    const int ly = e / kTX, lx = e - ly * kTX;
-    const int c = lx + 2, ra = ly + 1, rb = ly + 2;
-    solve_tile_edge<T, ORDER, TY>(s, a, rows, ra, c, rb, c, ra - 1, c, rb + 1, c, 3, dx, dt, s.ey[ly][lx]);
+  // every x-edge (row ly, between local columns lx-1 = A and lx = B) and
+  // y-edge (column lx, between local rows ly-1 = A and ly = B) in one flat
+  // loop so the 4 waves share the work evenly
+  for (int e = tid; e < Tile::NX + Tile::NY; e += kThreads) {
+    int ra, ca, rb, cb, rl, cl, rr2, cr2, d;
+    if (e < Tile::NX) {
+      const int ly = e / (kTX + 1), lx = e - ly * (kTX + 1);
+      ra = rb = rl = rr2 = ly + 2;
+      ca = lx + 1; cb = lx + 2; cl = lx; cr2 = lx + 3;
+      d = 0;
+    } else {
+      const int f = e - Tile::NX;
+      const int ly = f / kTX, lx = f - ly * kTX;
+      ca = cb = cl = cr2 = lx + 2;
+      ra = ly + 1; rb = ly + 2; rl = ly; rr2 = ly + 3;
+      d = 3;
+    }
+    solve_tile_edge<T, ORDER, TY>(s, a, rows, ra, ca, rb, cb, rl, cl, rr2, cr2, d, dx, dt, s.e[e]);
   }
   __syncthreads();
 
@@ -237,17 +246,14 @@ __global__ __launch_bounds__(kThreads) void k_flux(hp_grid_desc a) {
         listed = listed || (s.ty[nr[k]][nc[k]] != 0 && !is0 && s.h[nr[k]][nc[k]] > hs);
       }
       if (listed) {
-        const T* eE = s.ex[ly][lx + 1];  // this cell is A
-        const T* eN = s.ey[ly][lx];      // this cell is B
-        const T* eW = s.ex[ly][lx];      // this cell is B
-        const T* eS = s.ey[ly + 1][lx];  // this cell is A
-        // E
+        const T* eE = s.e[ly * (kTX + 1) + lx + 1];         // this cell is A
+        const T* eN = s.e[Tile::NX + ly * kTX + lx];        // this cell is B
+        const T* eW = s.e[ly * (kTX + 1) + lx];             // this cell is B
+        const T* eS = s.e[Tile::NX + (ly + 1) * kTX + lx];  // this cell is A
+        // the reference's order: E, N, W, S; per edge HLLC terms then bed term
         ah -= eE[0]; aqx -= eE[1]; aqx -= eE[3]; aqy -= eE[2];
-        // N
         ah -= -eN[0]; aqx -= -eN[1]; aqy -= -eN[2]; aqy -= eN[4];
-        // W
         ah -= -eW[0]; aqx -= -eW[1]; aqx -= eW[4]; aqy -= -eW[2];
-        // S
         ah -= eS[0]; aqx -= eS[1]; aqy -= eS[2]; aqy -= eS[3];
       }
     }
@@ -256,6 +262,20 @@ __global__ __launch_bounds__(kThreads) void k_flux(hp_grid_desc a) {
     dqx[gi] = aqx;
     dqy[gi] = aqy;
   }
+}
+
+template <class T, int ORDER, int TY>
+__global__ __launch_bounds__(kThreads) void k_flux(hp_grid_desc a) {
+  flux_tile<T, ORDER, TY>(a, blockIdx.x, blockIdx.y);
+}
+
+template <class T, int ORDER, int TY>
+__global__ __launch_bounds__(kThreads) void k_flux_multi(const hp_grid_desc* __restrict__ doms,
+                                                         const int2* __restrict__ map) {
+  const int2 m = map[blockIdx.x];
+  const hp_grid_desc& a = doms[m.x];
+  const int ntx = (a.nx + kTX - 1) / kTX;
+  flux_tile<T, ORDER, TY>(a, m.y % ntx, m.y / ntx);
 }
 
 // --- update pass ------------------------------------------------------------
@@ -274,8 +294,6 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
-constexpr int kMaxStations = 1024;
-
 // MODE selects which of the reference driver's update methods the pass is:
 //   HP_UPDATE_STEP      time_friction_euler_update_cuda (SWE_CUDA.py:780-831):
 //                       cells with |dh| > 0 or h >= 0; friction; CFL + dt/t
@@ -284,11 +302,17 @@ constexpr int kMaxStations = 1024;
 //                       h > 1e-6; q += dq without friction; no CFL
 //   HP_UPDATE_CFL       time_update_cuda (:726-764): cells with h > 1e-6; no
 //                       state change and no rain; h_max, CFL + dt/t
+// Processes plane elements [chunk*kChunk, (chunk+1)*kChunk) and writes one
+// partial record.
 template <class T, int MODE>
-__global__ __launch_bounds__(kThreads) void k_update(hp_grid_desc a) {
+__device__ inline void update_chunk(const hp_grid_desc& a, int64_t chunk, Partial* __restrict__ part) {
   __shared__ double s_rain[kMaxStations];
   __shared__ double s_red[4][kThreads / 64];
   const int tid = threadIdx.x;
+  if (stopped(a.ctrl)) {
+    if (tid == 0) *part = Partial{INFINITY, 0.0, 0.0, 0.0};
+    return;
+  }
   const double t_now = a.ctrl->t;
   const T dt = T(a.ctrl->dt);
   const T hs = T(kHs);
@@ -296,7 +320,8 @@ __global__ __launch_bounds__(kThreads) void k_update(hp_grid_desc a) {
 
   // station rain values at the current time (SWE_CUDA.py:472-495)
   int n_st = 0;
-  if (a.rain_mode == HP_RAIN_STATION && a.rain_table != nullptr && a.rain_rows > 0) {
+  const int mode = a.rain_mode;
+  if (mode == HP_RAIN_STATION && a.rain_table != nullptr && a.rain_rows > 0) {
     const int rows = a.rain_rows, cols = a.rain_cols;
     n_st = cols - 1;
     const double* tab = a.rain_table;
@@ -325,13 +350,16 @@ __global__ __launch_bounds__(kThreads) void k_update(hp_grid_desc a) {
   const T* __restrict__ man = (const T*)a.manning;
   const int16_t* __restrict__ st = a.station;
   const float* __restrict__ rf = a.rain_field;
-  const int mode = a.rain_mode;
   const T rscale = T(a.rain_scale);
 
   T mn = T(INFINITY);
   double vol = 0.0, fsum = 0.0, rsum = 0.0;
   const int64_t total = a.pitch * (int64_t)a.ny;
-  for (int64_t i = (int64_t)blockIdx.x * kThreads + tid; i < total; i += (int64_t)gridDim.x * kThreads) {
+  const int64_t base = chunk * kChunk;
+#pragma unroll 4
+  for (int it = 0; it < kUpdateIters; ++it) {
+    const int64_t i = base + (int64_t)it * kThreads + tid;
+    if (i >= total) break;
     const int cls = lu[i];
     if (cls == 255) continue;
     T h = hp_[i];
@@ -396,26 +424,36 @@ __global__ __launch_bounds__(kThreads) void k_update(hp_grid_desc a) {
   }
   __syncthreads();
   if (tid == 0) {
-    Partial p;
-    p.min_dt = s_red[0][0];
-    p.vol = s_red[1][0];
-    p.flux = s_red[2][0];
-    p.rain = s_red[3][0];
+    Partial p{s_red[0][0], s_red[1][0], s_red[2][0], s_red[3][0]};
     for (int k = 1; k < kThreads / 64; ++k) {
       p.min_dt = fmin(p.min_dt, s_red[0][k]);
       p.vol += s_red[1][k];
       p.flux += s_red[2][k];
       p.rain += s_red[3][k];
     }
-    ((Partial*)a.work)[blockIdx.x] = p;
+    *part = p;
   }
 }
 
-__global__ __launch_bounds__(1024) void k_finalize(hp_ctrl* ctrl, const Partial* parts, int n, double dx) {
+template <class T, int MODE>
+__global__ __launch_bounds__(kThreads) void k_update(hp_grid_desc a) {
+  update_chunk<T, MODE>(a, blockIdx.x, (Partial*)a.work + blockIdx.x);
+}
+
+template <class T, int MODE>
+__global__ __launch_bounds__(kThreads) void k_update_multi(const hp_grid_desc* __restrict__ doms,
+                                                           const int2* __restrict__ map, Partial* parts) {
+  const int2 m = map[blockIdx.x];
+  update_chunk<T, MODE>(doms[m.x], m.y, parts + blockIdx.x);
+}
+
+// Fold a domain's partials and advance its time control.
+__device__ inline void finalize_domain(hp_ctrl* ctrl, const Partial* parts, int n, double dx) {
   __shared__ double s_red[4][16];
   const int tid = threadIdx.x;
+  if (stopped(ctrl)) return;
   double mn = INFINITY, vol = 0.0, fs = 0.0, rs = 0.0;
-  for (int k = tid; k < n; k += 1024) {
+  for (int k = tid; k < n; k += blockDim.x) {
     const Partial p = parts[k];
     mn = fmin(mn, p.min_dt);
     vol += p.vol;
@@ -427,6 +465,7 @@ __global__ __launch_bounds__(1024) void k_finalize(hp_ctrl* ctrl, const Partial*
   fs = wave_sum(fs);
   rs = wave_sum(rs);
   const int lane = tid & 63, w = tid >> 6;
+  const int nw = (blockDim.x + 63) >> 6;
   if (lane == 0) {
     s_red[0][w] = mn;
     s_red[1][w] = vol;
@@ -435,14 +474,16 @@ __global__ __launch_bounds__(1024) void k_finalize(hp_ctrl* ctrl, const Partial*
   }
   __syncthreads();
   if (tid == 0) {
-    for (int k = 1; k < 16; ++k) {
-      mn = fmin(s_red[0][0], s_red[0][k]);
-      s_red[0][0] = mn;
-      s_red[1][0] += s_red[1][k];
-      s_red[2][0] += s_red[2][k];
-      s_red[3][0] += s_red[3][k];
-    }
     mn = s_red[0][0];
+    vol = s_red[1][0];
+    fs = s_red[2][0];
+    rs = s_red[3][0];
+    for (int k = 1; k < nw; ++k) {
+      mn = fmin(mn, s_red[0][k]);
+      vol += s_red[1][k];
+      fs += s_red[2][k];
+      rs += s_red[3][k];
+    }
     const double area = dx * dx;
     const double t = ctrl->t;
     const double dt_old = ctrl->dt;
@@ -457,20 +498,38 @@ __global__ __launch_bounds__(1024) void k_finalize(hp_ctrl* ctrl, const Partial*
     }
     ctrl->dt = dt_new;
     ctrl->t = t + dt_new;
-    ctrl->volume = s_red[1][0] * area;
-    ctrl->flux_volume += s_red[2][0] * area;
-    ctrl->rain_volume += s_red[3][0] * area;
+    ctrl->volume = vol * area;
+    ctrl->flux_volume += fs * area;
+    ctrl->rain_volume += rs * area;
     ctrl->steps += 1.0;
   }
 }
 
+__global__ __launch_bounds__(1024) void k_finalize(hp_ctrl* ctrl, const Partial* parts, int n, double dx) {
+  finalize_domain(ctrl, parts, n, dx);
+}
+
+__global__ __launch_bounds__(256) void k_finalize_multi(const hp_grid_desc* __restrict__ doms,
+                                                        const int2* __restrict__ ranges, const Partial* parts) {
+  const int2 r = ranges[blockIdx.x];
+  const hp_grid_desc& a = doms[blockIdx.x];
+  finalize_domain(a.ctrl, parts + r.x, r.y, a.dx);
+}
+
 // --- host side -------------------------------------------------------------
-static int update_blocks(const hp_grid_desc* g) {
+static int64_t update_blocks(const hp_grid_desc* g) {
   const int64_t total = g->pitch * (int64_t)g->ny;
-  int64_t b = (total + kThreads - 1) / kThreads;
-  if (b > kUpdateMaxBlocks) b = kUpdateMaxBlocks;
-  if (b < 1) b = 1;
-  return (int)b;
+  return (total + kChunk - 1) / kChunk;
+}
+
+static int flux_rows(const hp_grid_desc* g) {
+  if (g->dtype == HP_F32) return g->tile_rows == 4 ? 4 : 8;
+  return g->tile_rows == 2 ? 2 : 4;
+}
+
+static int64_t flux_tiles(const hp_grid_desc* g) {
+  const int ty = flux_rows(g);
+  return (int64_t)((g->nx + kTX - 1) / kTX) * (int64_t)((g->ny + ty - 1) / ty);
 }
 
 static int check_desc(const hp_grid_desc* g) {
@@ -497,15 +556,22 @@ static int check_desc(const hp_grid_desc* g) {
   if ((g->given_depth && (g->gd_rows < 1 || g->gd_w < 1)) || (g->given_wl && (g->gwl_rows < 1 || g->gwl_w < 1)) ||
       (g->given_q && (g->gq_rows < 1 || g->gq_w < 1)))
     return fail(HP_EINVAL, "boundary series tables need >= 1 row and column");
+  if (flux_tiles(g) > 0x7fffffffLL || update_blocks(g) > 0x7fffffffLL) return fail(HP_EINVAL, "domain too large");
   return HP_OK;
+}
+
+template <class T, int ORDER, int TY>
+static int launch_flux_single(const hp_grid_desc* g, hipStream_t st) {
+  dim3 grid((g->nx + kTX - 1) / kTX, (g->ny + TY - 1) / TY);
+  hipLaunchKernelGGL((k_flux<T, ORDER, TY>), grid, dim3(kThreads), 0, st, *g);
+  return check_launch("k_flux");
 }
 
 template <class T, int ORDER>
 static int launch_flux_t(const hp_grid_desc* g, hipStream_t st) {
-  constexpr int TY = TileRows<T>::v;
-  dim3 grid((g->nx + kTX - 1) / kTX, (g->ny + TY - 1) / TY);
-  hipLaunchKernelGGL((k_flux<T, ORDER, TY>), grid, dim3(kThreads), 0, st, *g);
-  return check_launch("k_flux");
+  const int ty = flux_rows(g);
+  if (sizeof(T) == 4) return ty == 4 ? launch_flux_single<T, ORDER, 4>(g, st) : launch_flux_single<T, ORDER, 8>(g, st);
+  return ty == 2 ? launch_flux_single<T, ORDER, 2>(g, st) : launch_flux_single<T, ORDER, 4>(g, st);
 }
 
 static int launch_flux(const hp_grid_desc* g, hipStream_t st) {
@@ -514,14 +580,14 @@ static int launch_flux(const hp_grid_desc* g, hipStream_t st) {
 }
 
 template <int MODE>
-static void launch_update_mode(const hp_grid_desc* g, int nb, hipStream_t st) {
-  if (g->dtype == HP_F32) hipLaunchKernelGGL((k_update<float, MODE>), dim3(nb), dim3(kThreads), 0, st, *g);
-  else hipLaunchKernelGGL((k_update<double, MODE>), dim3(nb), dim3(kThreads), 0, st, *g);
+static void launch_update_mode(const hp_grid_desc* g, int64_t nb, hipStream_t st) {
+  if (g->dtype == HP_F32) hipLaunchKernelGGL((k_update<float, MODE>), dim3((unsigned)nb), dim3(kThreads), 0, st, *g);
+  else hipLaunchKernelGGL((k_update<double, MODE>), dim3((unsigned)nb), dim3(kThreads), 0, st, *g);
 }
 
 static int launch_update(const hp_grid_desc* g, int mode, hipStream_t st) {
-  const int nb = update_blocks(g);
-  if (!g->work || g->work_bytes < (int64_t)nb * (int64_t)sizeof(Partial))
+  const int64_t nb = update_blocks(g);
+  if (!g->work || g->work_bytes < nb * (int64_t)sizeof(Partial))
     return fail(HP_ESPACE, "workspace smaller than hp_grid_workspace_bytes()");
   switch (mode) {
     case HP_UPDATE_STEP: launch_update_mode<HP_UPDATE_STEP>(g, nb, st); break;
@@ -533,10 +599,66 @@ static int launch_update(const hp_grid_desc* g, int mode, hipStream_t st) {
   int rc = check_launch("k_update");
   if (rc) return rc;
   if (mode == HP_UPDATE_STEP || mode == HP_UPDATE_CFL) {
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, st, g->ctrl, (const Partial*)g->work, nb, g->dx);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, st, g->ctrl, (const Partial*)g->work, (int)nb, g->dx);
     return check_launch("k_finalize");
   }
   return HP_OK;
+}
+
+// ---- ensemble ---------------------------------------------------------------
+struct EnsembleLayout {
+  int64_t off_doms, off_fmap, off_umap, off_fin, off_parts, total;
+  int64_t n_ftiles, n_ublocks;
+};
+
+static int64_t align256(int64_t x) { return (x + 255) & ~int64_t(255); }
+
+static EnsembleLayout ensemble_layout(int32_t n, const hp_grid_desc* d) {
+  EnsembleLayout L{};
+  for (int k = 0; k < n; ++k) {
+    L.n_ftiles += flux_tiles(&d[k]);
+    L.n_ublocks += update_blocks(&d[k]);
+  }
+  L.off_doms = 0;
+  L.off_fmap = align256((int64_t)n * (int64_t)sizeof(hp_grid_desc));
+  L.off_umap = L.off_fmap + align256(L.n_ftiles * 8);
+  L.off_fin = L.off_umap + align256(L.n_ublocks * 8);
+  L.off_parts = L.off_fin + align256((int64_t)n * 8);
+  L.total = L.off_parts + align256(L.n_ublocks * (int64_t)sizeof(Partial));
+  return L;
+}
+
+template <class T, int ORDER, int TY>
+static void launch_flux_multi(const hp_ensemble* e, hipStream_t st) {
+  hipLaunchKernelGGL((k_flux_multi<T, ORDER, TY>), dim3((unsigned)e->n_flux_tiles), dim3(kThreads), 0, st,
+                     (const hp_grid_desc*)e->doms, (const int2*)e->flux_map);
+}
+
+template <class T>
+static void launch_update_multi(const hp_ensemble* e, hipStream_t st) {
+  hipLaunchKernelGGL((k_update_multi<T, HP_UPDATE_STEP>), dim3((unsigned)e->n_update_blocks), dim3(kThreads), 0, st,
+                     (const hp_grid_desc*)e->doms, (const int2*)e->update_map, (Partial*)e->partials);
+}
+
+static int ensemble_step_once(const hp_ensemble* e, hipStream_t st) {
+  const bool f32 = e->dtype == HP_F32;
+  const int ty = e->tile_rows;
+  if (f32) {
+    if (e->order == 2) (ty == 4 ? launch_flux_multi<float, 2, 4> : launch_flux_multi<float, 2, 8>)(e, st);
+    else (ty == 4 ? launch_flux_multi<float, 1, 4> : launch_flux_multi<float, 1, 8>)(e, st);
+  } else {
+    if (e->order == 2) (ty == 2 ? launch_flux_multi<double, 2, 2> : launch_flux_multi<double, 2, 4>)(e, st);
+    else (ty == 2 ? launch_flux_multi<double, 1, 2> : launch_flux_multi<double, 1, 4>)(e, st);
+  }
+  int rc = check_launch("k_flux_multi");
+  if (rc) return rc;
+  if (f32) launch_update_multi<float>(e, st);
+  else launch_update_multi<double>(e, st);
+  rc = check_launch("k_update_multi");
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_finalize_multi, dim3((unsigned)e->n_domains), dim3(256), 0, st,
+                     (const hp_grid_desc*)e->doms, (const int2*)e->finalize_map, (const Partial*)e->partials);
+  return check_launch("k_finalize_multi");
 }
 
 template <class T>
@@ -558,7 +680,7 @@ extern "C" {
 
 int64_t hp_grid_workspace_bytes(const hp_grid_desc* g) {
   if (!g || g->pitch <= 0 || g->ny <= 0) return 0;
-  return (int64_t)update_blocks(g) * (int64_t)sizeof(Partial);
+  return update_blocks(g) * (int64_t)sizeof(Partial);
 }
 
 int hp_grid_flux(const hp_grid_desc* g, void* stream) {
@@ -583,6 +705,71 @@ int hp_grid_step(const hp_grid_desc* g, int32_t nsteps, void* stream) {
     rc = launch_flux(g, (hipStream_t)stream);
     if (rc) return rc;
     rc = launch_update(g, HP_UPDATE_STEP, (hipStream_t)stream);
+    if (rc) return rc;
+  }
+  return HP_OK;
+}
+
+int64_t hp_ensemble_workspace_bytes(int32_t n, const hp_grid_desc* descs) {
+  if (n <= 0 || !descs) return 0;
+  return ensemble_layout(n, descs).total;
+}
+
+int hp_ensemble_init(int32_t n, const hp_grid_desc* descs, void* work, int64_t work_bytes, hp_ensemble* out,
+                     void* stream) {
+  if (n <= 0 || !descs || !out) return fail(HP_EINVAL, "need n > 0 descriptors and an output handle");
+  for (int k = 0; k < n; ++k) {
+    int rc = check_desc(&descs[k]);
+    if (rc) return fail(rc, "domain " + std::to_string(k) + ": " + last_error_slot());
+    if (descs[k].dtype != descs[0].dtype || descs[k].order != descs[0].order ||
+        flux_rows(&descs[k]) != flux_rows(&descs[0]))
+      return fail(HP_EINVAL, "every domain of an ensemble needs the same dtype, order and tile_rows");
+  }
+  const EnsembleLayout L = ensemble_layout(n, descs);
+  if (!work || work_bytes < L.total) return fail(HP_ESPACE, "workspace smaller than hp_ensemble_workspace_bytes()");
+  if (L.n_ftiles > 0x7fffffffLL || L.n_ublocks > 0x7fffffffLL) return fail(HP_EINVAL, "ensemble too large");
+  std::vector<uint8_t> host((size_t)L.off_parts, 0);
+  std::memcpy(host.data() + L.off_doms, descs, sizeof(hp_grid_desc) * (size_t)n);
+  int32_t* fm = (int32_t*)(host.data() + L.off_fmap);
+  int32_t* um = (int32_t*)(host.data() + L.off_umap);
+  int32_t* fin = (int32_t*)(host.data() + L.off_fin);
+  int64_t fi = 0, ui = 0;
+  for (int k = 0; k < n; ++k) {
+    const int64_t nt = flux_tiles(&descs[k]), nu = update_blocks(&descs[k]);
+    for (int64_t j = 0; j < nt; ++j, ++fi) {
+      fm[2 * fi] = k;
+      fm[2 * fi + 1] = (int32_t)j;
+    }
+    fin[2 * k] = (int32_t)ui;
+    fin[2 * k + 1] = (int32_t)nu;
+    for (int64_t j = 0; j < nu; ++j, ++ui) {
+      um[2 * ui] = k;
+      um[2 * ui + 1] = (int32_t)j;
+    }
+  }
+  hipError_t err = hipMemcpyAsync(work, host.data(), host.size(), hipMemcpyHostToDevice, (hipStream_t)stream);
+  if (err == hipSuccess) err = hipStreamSynchronize((hipStream_t)stream);
+  if (err != hipSuccess) return fail(HP_ELAUNCH, std::string("ensemble table upload: ") + hipGetErrorString(err));
+  uint8_t* w = (uint8_t*)work;
+  out->n_domains = n;
+  out->dtype = descs[0].dtype;
+  out->order = descs[0].order;
+  out->tile_rows = flux_rows(&descs[0]);
+  out->n_flux_tiles = (int32_t)L.n_ftiles;
+  out->n_update_blocks = (int32_t)L.n_ublocks;
+  out->doms = w + L.off_doms;
+  out->flux_map = w + L.off_fmap;
+  out->update_map = w + L.off_umap;
+  out->finalize_map = w + L.off_fin;
+  out->partials = w + L.off_parts;
+  return HP_OK;
+}
+
+int hp_ensemble_step(const hp_ensemble* e, int32_t nsteps, void* stream) {
+  if (!e || e->n_domains <= 0 || !e->doms) return fail(HP_EINVAL, "ensemble handle not initialised");
+  if (nsteps < 0) return fail(HP_EINVAL, "nsteps must be >= 0");
+  for (int32_t k = 0; k < nsteps; ++k) {
+    int rc = ensemble_step_once(e, (hipStream_t)stream);
     if (rc) return rc;
   }
   return HP_OK;

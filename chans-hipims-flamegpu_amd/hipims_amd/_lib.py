@@ -34,10 +34,12 @@ class HpCtrl(ctypes.Structure):
     _fields_ = [("t", ctypes.c_double), ("dt", ctypes.c_double), ("export_T", ctypes.c_double),
                 ("export_n", ctypes.c_double), ("export_pending", ctypes.c_double), ("volume", ctypes.c_double),
                 ("rain_volume", ctypes.c_double), ("flux_volume", ctypes.c_double), ("steps", ctypes.c_double),
-                ("reserved", ctypes.c_double * 7)]
+                ("t_stop", ctypes.c_double), ("reserved", ctypes.c_double * 6)]
 
 
-CTRL_FIELDS = [f for f, _ in HpCtrl._fields_[:9]]
+CTRL_FIELDS = [f for f, _ in HpCtrl._fields_[:10]]
+CTRL_T, CTRL_DT, CTRL_EXPORT_T, CTRL_EXPORT_N, CTRL_PENDING, CTRL_VOLUME = 0, 1, 2, 3, 4, 5
+CTRL_RAIN, CTRL_FLUX, CTRL_STEPS, CTRL_T_STOP = 6, 7, 8, 9
 CTRL_DOUBLES = 16
 
 
@@ -46,6 +48,7 @@ class HpGridDesc(ctypes.Structure):
         ("dtype", _I32), ("order", _I32), ("nx", _I32), ("ny", _I32), ("pitch", _I64),
         ("dx", ctypes.c_double), ("cfl", ctypes.c_double),
         ("cell0_x", _I32), ("cell0_y", _I32), ("n_classes", _I32), ("rain_mode", _I32),
+        ("tile_rows", _I32), ("reserved_i32", _I32),
         ("h", _P), ("qx", _P), ("qy", _P), ("z", _P), ("h_max", _P),
         ("dh", _P), ("dqx", _P), ("dqy", _P),
         ("type_slot", _P), ("landuse", _P), ("slot_code", _P), ("manning", _P),
@@ -57,12 +60,19 @@ class HpGridDesc(ctypes.Structure):
     ]
 
 
+class HpEnsemble(ctypes.Structure):
+    _fields_ = [("n_domains", _I32), ("dtype", _I32), ("order", _I32), ("tile_rows", _I32),
+                ("n_flux_tiles", _I32), ("n_update_blocks", _I32),
+                ("doms", _P), ("flux_map", _P), ("update_map", _P), ("finalize_map", _P), ("partials", _P)]
+
+
 _lib = None
 
 # name -> (restype, argtypes); every symbol include/hipims_abi.h declares
 SIGNATURES = {
     "hp_abi_version": (_I, []),
     "hp_last_error": (ctypes.c_char_p, []),
+    "hp_struct_sizes": (None, [ctypes.POINTER(ctypes.c_int64)]),
     "hp_flux_mask": (_I, [_I, _I64, _P, _P, _P, _P, _P]),
     "hp_flux_compact": (_I, [_I, _I, _I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                              _P, _I32, _I32, _P, _I32, _I32, _P, _I32, _I32, _P, _P, _P, _P]),
@@ -75,6 +85,9 @@ SIGNATURES = {
     "hp_grid_update": (_I, [ctypes.POINTER(HpGridDesc), _P]),
     "hp_grid_update_mode": (_I, [ctypes.POINTER(HpGridDesc), _I, _P]),
     "hp_grid_step": (_I, [ctypes.POINTER(HpGridDesc), _I32, _P]),
+    "hp_ensemble_workspace_bytes": (_I64, [_I32, ctypes.POINTER(HpGridDesc)]),
+    "hp_ensemble_init": (_I, [_I32, ctypes.POINTER(HpGridDesc), _P, _I64, ctypes.POINTER(HpEnsemble), _P]),
+    "hp_ensemble_step": (_I, [ctypes.POINTER(HpEnsemble), _I32, _P]),
     "hp_gather": (_I, [_I, _P, _P, _I64, _P, _P]),
     "hp_scatter": (_I, [_I, _P, _P, _I64, _P, _P]),
 }
@@ -102,6 +115,11 @@ def lib():
             fn.argtypes = args
         if L.hp_abi_version() != ABI_VERSION:
             raise RuntimeError("libhipims_amd.so ABI version mismatch")
+        sizes = (ctypes.c_int64 * 3)()
+        L.hp_struct_sizes(sizes)
+        mine = (ctypes.sizeof(HpCtrl), ctypes.sizeof(HpGridDesc), ctypes.sizeof(HpEnsemble))
+        if tuple(sizes) != mine:
+            raise RuntimeError(f"struct layout mismatch: library {tuple(sizes)} vs bindings {mine}")
         _lib = L
     return _lib
 

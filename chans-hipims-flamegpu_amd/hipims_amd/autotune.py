@@ -1,0 +1,49 @@
+"""Flux tile-shape autotuner.
+
+The flux pass stages 64 x TY cells plus a 2-cell halo in LDS.  A taller tile
+re-reads fewer halo rows but holds more LDS per block (fewer blocks per CU);
+which wins depends on dtype, order and domain size.  tune() times each legal
+TY on the caller's own domain state (restoring the state afterwards) with HIP
+events and keeps the fastest; results are memoised per (dtype, order, size
+class) for the process.
+"""
+from __future__ import annotations
+
+import torch
+
+from .grid import GridSolver
+
+CHOICES = {torch.float32: (4, 8), torch.float64: (2, 4)}
+_cache: dict = {}
+
+
+def _key(s: GridSolver):
+    cells = s.layout.ny * s.layout.pitch
+    return (s.dtype, s.order, max(0, cells.bit_length() - 18))
+
+
+def tune(s: GridSolver, reps: int = 20, apply: bool = True) -> int:
+    key = _key(s)
+    if key in _cache:
+        best = _cache[key]
+    else:
+        saved = [t.clone() for t in (s.dh, s.dqx, s.dqy)]
+        timings = {}
+        for ty in CHOICES[s.dtype]:
+            s.set_tile_rows(ty)
+            s.flux()  # warm-up (flux only reads state, writes the accumulators)
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            for _ in range(reps):
+                s.flux()
+            b.record()
+            torch.cuda.synchronize(s.device)
+            timings[ty] = a.elapsed_time(b) / reps
+        for t, v in zip((s.dh, s.dqx, s.dqy), saved):
+            t.copy_(v)
+        best = min(timings, key=timings.get)
+        _cache[key] = best
+        s.tune_timings = timings
+    if apply:
+        s.set_tile_rows(best)
+    return best

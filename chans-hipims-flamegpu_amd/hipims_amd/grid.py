@@ -92,7 +92,7 @@ class GridSolver:
 
     def __init__(self, mask, h, qx, qy, z, dx, *, cfl=0.5, order=2, dtype=torch.float64, device="cuda",
                  first_dt=1.0e-4, t=0.0, export_T=math.inf, export_n=0, manning=(0.0,), landuse=None,
-                 given_depth=None, given_wl=None, given_q=None, ctrl=None):
+                 given_depth=None, given_wl=None, given_q=None, ctrl=None, tile_rows=0, t_stop=0.0):
         self.device = torch.device(device)
         if self.device.type == "cuda" and self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())
@@ -135,6 +135,9 @@ class GridSolver:
             self.ctrl[1] = float(first_dt)
             self.ctrl[2] = float(export_T)
             self.ctrl[3] = float(export_n)
+        if t_stop:
+            self.ctrl[_lib.CTRL_T_STOP] = float(t_stop)
+        self.tile_rows = int(tile_rows)
         self.given_depth = self.given_wl = self.given_q = None
         self.set_boundary(given_depth, given_wl, given_q)
         self.station = None
@@ -217,6 +220,14 @@ class GridSolver:
         self.rain_table = self.station = self.rain_field = None
         self._desc = None
 
+    def set_tile_rows(self, rows: int):
+        self.tile_rows = int(rows)
+        self._desc = None
+
+    def set_t_stop(self, t_stop: float):
+        """The passes of this domain do nothing once t >= t_stop (0: never)."""
+        self.ctrl[_lib.CTRL_T_STOP] = float(t_stop)
+
     def set_rain_scale(self, s: float):
         self.rain_scale = float(s)
         self._desc = None
@@ -232,6 +243,7 @@ class GridSolver:
         d.cell0_x, d.cell0_y = L.cell0
         d.n_classes = int(self.manning.numel())
         d.rain_mode = self.rain_mode
+        d.tile_rows = self.tile_rows
         for f in ("h", "qx", "qy", "z", "h_max", "dh", "dqx", "dqy", "type_slot", "landuse", "slot_code",
                   "manning"):
             setattr(d, f, getattr(self, f).data_ptr())

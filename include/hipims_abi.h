@@ -41,6 +41,9 @@ enum hp_dtype { HP_F32 = 0, HP_F64 = 1 };
 /* ABI version and the calling thread's last error message. */
 int hp_abi_version(void);
 const char* hp_last_error(void);
+/* sizeof(hp_ctrl), sizeof(hp_grid_desc), sizeof(hp_ensemble) into out[0..2],
+ * so bindings can check their mirrors of these structs. */
+void hp_struct_sizes(int64_t* out);
 
 /* ------------------------------------------------------------------------
  * 1. Reference op surface (compact layout).  `dtype` selects float/double for
@@ -104,7 +107,8 @@ typedef struct hp_ctrl {
   double rain_volume;    /* cumulative sum of rain*dt (depth units)          */
   double flux_volume;    /* cumulative sum of flux dh (net boundary inflow)  */
   double steps;          /* completed steps                                  */
-  double reserved[7];
+  double t_stop;         /* > 0: the domain's passes do nothing once t >= t_stop */
+  double reserved[6];
 } hp_ctrl;
 
 /* Rain source of the update pass. */
@@ -129,6 +133,8 @@ typedef struct hp_grid_desc {
   int32_t cell0_x, cell0_y; /* first valid cell in row-major order (-1: none) */
   int32_t n_classes;        /* entries of manning */
   int32_t rain_mode;        /* hp_rain_mode */
+  int32_t tile_rows;        /* flux tile height: float 4 or 8, double 2 or 4 (0: 8 / 4) */
+  int32_t reserved_i32;
   /* state planes (dtype) */
   void *h, *qx, *qy, *z, *h_max;
   /* flux accumulators written by the flux pass (dtype) */
@@ -180,6 +186,30 @@ int hp_grid_update_mode(const hp_grid_desc* g, int mode, void* stream);
 
 /* nsteps x (flux, update). */
 int hp_grid_step(const hp_grid_desc* g, int32_t nsteps, void* stream);
+
+/* ---- Ensemble: many domains advanced by the same launches, each with its own
+ * dt and time control (no reference counterpart; the reference runs one
+ * domain per process).  All domains share dtype, order and tile_rows. */
+typedef struct hp_ensemble {
+  int32_t n_domains, dtype, order, tile_rows;
+  int32_t n_flux_tiles, n_update_blocks;
+  const void* doms;         /* device hp_grid_desc[n_domains]          */
+  const void* flux_map;     /* device int32[2 * n_flux_tiles]          */
+  const void* update_map;   /* device int32[2 * n_update_blocks]       */
+  const void* finalize_map; /* device int32[2 * n_domains]             */
+  void* partials;           /* device reduction records                */
+} hp_ensemble;
+
+/* Device workspace bytes hp_ensemble_init needs for these domains. */
+int64_t hp_ensemble_workspace_bytes(int32_t n, const hp_grid_desc* descs);
+/* Copy the descriptors and block maps into `work` (a one-off upload that
+ * synchronises `stream`) and fill the handle.  The domains' planes must stay
+ * allocated while the handle is used. */
+int hp_ensemble_init(int32_t n, const hp_grid_desc* descs, void* work, int64_t work_bytes, hp_ensemble* out,
+                     void* stream);
+/* nsteps x (flux of every domain, update of every domain, per-domain time
+ * control), three launches per step, graph-capturable. */
+int hp_ensemble_step(const hp_ensemble* e, int32_t nsteps, void* stream);
 
 /* out[i] = plane[offsets[i]]  (compact gather, Godunov.get_h). */
 int hp_gather(int dtype, const void* plane, const int64_t* offsets, int64_t n, void* out, void* stream);

@@ -16,7 +16,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_symbols():
     text = open(os.path.join(ROOT, "include", "hipims_abi.h")).read()
-    return sorted(set(re.findall(r"^\s*(?:int|int64_t|const char\*)\s+(hp_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|int64_t|void|const char\*)\s+(hp_\w+)\s*\(", text, re.M)))
 
 
 def test_every_declared_symbol_is_exported():
