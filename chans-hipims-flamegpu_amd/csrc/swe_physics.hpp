@@ -42,9 +42,9 @@ constexpr double kHs = 1.0e-6;
 template <class T> struct Prec;
 template <> struct Prec<double> {
   static constexpr bool local_datum = false;
-  __device__ static inline double div(double a, double b) { return a / b; }
-  __device__ static inline double sqrt_(double x) { return __builtin_sqrt(x); }
-  __device__ static inline double rcbrt_friction(double h) { return pow(h, -1.0 / 3.0); }
+  __host__ __device__ static inline double div(double a, double b) { return a / b; }
+  __host__ __device__ static inline double sqrt_(double x) { return __builtin_sqrt(x); }
+  __host__ __device__ static inline double rcbrt_friction(double h) { return pow(h, -1.0 / 3.0); }
 };
 template <> struct Prec<float> {
   static constexpr bool local_datum = true;
@@ -53,7 +53,7 @@ template <> struct Prec<float> {
   __device__ static inline float rcbrt_friction(float h) { return rcbrtf(h); }
 };
 
-template <class T> __device__ inline T minmod(T x, T y) {
+template <class T> __host__ __device__ inline T minmod(T x, T y) {
   if (x * y <= T(0)) return T(0);
   return fabs(x) < fabs(y) ? x : y;
 }
@@ -64,11 +64,11 @@ template <class T> struct FrameCell {
 };
 
 // Normal of direction d (E, N, W, S) as small integers.
-__device__ inline int nrm_x(int d) { return d == 0 ? 1 : (d == 2 ? -1 : 0); }
-__device__ inline int nrm_y(int d) { return d == 1 ? 1 : (d == 3 ? -1 : 0); }
+__host__ __device__ inline int nrm_x(int d) { return d == 0 ? 1 : (d == 2 ? -1 : 0); }
+__host__ __device__ inline int nrm_y(int d) { return d == 1 ? 1 : (d == 3 ? -1 : 0); }
 
 // q . n and q . t with t = (-n_y, n_x); exact for unit axis normals.
-template <class T> __device__ inline T rot_n(T qx, T qy, int d) {
+template <class T> __host__ __device__ inline T rot_n(T qx, T qy, int d) {
   switch (d) {
     case 0: return qx;
     case 1: return qy;
@@ -76,7 +76,7 @@ template <class T> __device__ inline T rot_n(T qx, T qy, int d) {
     default: return -qy;
   }
 }
-template <class T> __device__ inline T rot_t(T qx, T qy, int d) {
+template <class T> __host__ __device__ inline T rot_t(T qx, T qy, int d) {
   switch (d) {
     case 0: return qy;
     case 1: return -qx;
@@ -100,7 +100,7 @@ template <class T> __device__ inline T rot_t(T qx, T qy, int d) {
 // Any other type: copy (the reference leaves the ghost undefined there).
 // gwl is passed relative to the same datum as c.z.
 template <class T>
-__device__ inline FrameCell<T> bc_ghost(int type, const FrameCell<T>& c, int d, T gd, T gwl, T gqx, T gqy) {
+__host__ __device__ inline FrameCell<T> bc_ghost(int type, const FrameCell<T>& c, int d, T gd, T gwl, T gqx, T gqy) {
   FrameCell<T> b = c;
   switch (type) {
     case 3:
@@ -134,7 +134,7 @@ __device__ inline FrameCell<T> bc_ghost(int type, const FrameCell<T>& c, int d, 
 // (fluxCal_jh_modified_surface_Kernel.cu:262-396): only H/Q/WL-given ghosts
 // get a new one; the others keep the cell's wl.
 template <class T>
-__device__ inline T ghost_wl(int type, T c_wl, const FrameCell<T>& g, T gwl_rel) {
+__host__ __device__ inline T ghost_wl(int type, T c_wl, const FrameCell<T>& g, T gwl_rel) {
   switch (type) {
     case 6:
     case 7: return g.z + g.h;
@@ -158,7 +158,7 @@ template <class T> struct EdgeOut {
 // from these.  etaC / etaR: the cells' water levels used by the clamp.
 // On return eL.h/eL.z and eR.h/eR.z hold the reconstructed depth and bed.
 template <class T, int ORDER>
-__device__ inline void hllc_edge(FrameCell<T>& eL, FrameCell<T>& eR, T etaC, T etaR, T& fh, T& fn, T& ft) {
+__host__ __device__ inline void hllc_edge(FrameCell<T>& eL, FrameCell<T>& eR, T etaC, T etaR, T& fh, T& fn, T& ft) {
   using P = Prec<T>;
   const T hs = T(kHs), g = T(kG);
   const T uL0 = eL.h < hs ? T(0) : P::div(eL.qn, eL.h);
@@ -231,7 +231,7 @@ __device__ inline void hllc_edge(FrameCell<T>& eL, FrameCell<T>& eR, T etaC, T e
 // Frame values carry z relative to c's datum (c.z == 0 in float).
 // The returned terms are scaled by dt/dx as  (raw / dx) * dt.
 template <class T, int ORDER>
-__device__ inline EdgeOut<T> solve_edge(const FrameCell<T>& c, const FrameCell<T>& l, const FrameCell<T>& r,
+__host__ __device__ inline EdgeOut<T> solve_edge(const FrameCell<T>& c, const FrameCell<T>& l, const FrameCell<T>& r,
                                         const FrameCell<T>& rr, bool r_is_ghost, T etaC, T etaR, T dx, T dt) {
   const T hs = T(kHs), g = T(kG);
   EdgeOut<T> o;
@@ -285,7 +285,7 @@ __device__ inline EdgeOut<T> solve_edge(const FrameCell<T>& c, const FrameCell<T
 // Implicit Manning friction for one wet cell (h >= hs), updating qx, qy in
 // place after they have received their flux increments.
 template <class T>
-__device__ inline void implicit_friction(T h, T n, T dt, T& qx, T& qy) {
+__host__ __device__ inline void implicit_friction(T h, T n, T dt, T& qx, T& qy) {
   using P = Prec<T>;
   const T g = T(kG);
   const T C_f = g * n * n * Prec<T>::rcbrt_friction(h);
