@@ -36,7 +36,7 @@ def P(a):
     return a.ctypes.data_as(_P)
 
 
-def fd_check(f, x, idx, v, grad, eps_rel=1e-6, tol=1e-6, depth_idx=()):
+def fd_check(f, x, idx, v, grad, eps_rel=1e-6, tol=1e-6, depth_idx=(), kink=1e-5):
     """Compare grad[k] with the central difference of v . f along x[idx[k]].
     Depth inputs (depth_idx) are not differenced through 0 or across the
     1e-6 wet/dry threshold.  Returns (compared, skipped)."""
@@ -52,7 +52,7 @@ def fd_check(f, x, idx, v, grad, eps_rel=1e-6, tol=1e-6, depth_idx=()):
         xm[j] -= eps
         fp, fm = v @ f(xp), v @ f(xm)
         fwd, bwd, cen = (fp - f0) / eps, (f0 - fm) / eps, (fp - fm) / (2 * eps)
-        if abs(fwd - bwd) > 1e-5 * (abs(fwd) + abs(bwd)) + 1e-9:
+        if abs(fwd - bwd) > kink * (abs(fwd) + abs(bwd)) + 1e-9:
             skip += 1  # a branch switches inside the stencil
             continue
         assert abs(grad[k] - cen) <= tol * max(abs(cen), abs(grad[k])) + 1e-9, (j, grad[k], cen)
@@ -140,7 +140,7 @@ def test_friction_vjp_matches_finite_differences(ec):
             return out
         grad = np.zeros(4)
         ec.ec_friction_vjp(h, n, dt, q[0], q[1], P(a), P(grad))
-        ok, sk = fd_check(f, x, [0, 1, 2, 3], a, grad, eps_rel=1e-5, tol=1e-5, depth_idx=(2,))
+        ok, sk = fd_check(f, x, [0, 1, 2, 3], a, grad, eps_rel=1e-5, tol=1e-5, depth_idx=(2,), kink=1e-2)
         total_ok += ok
         total_skip += sk
     assert total_ok > 0.9 * (total_ok + total_skip), (total_ok, total_skip)
