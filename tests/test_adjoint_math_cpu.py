@@ -36,14 +36,14 @@ def P(a):
     return a.ctypes.data_as(_P)
 
 
-def fd_check(f, x, idx, v, grad, eps_rel=1e-6, tol=1e-6, depth_idx=(), kink=1e-5):
+def fd_check(f, x, idx, v, grad, eps_rel=1e-6, tol=1e-6, depth_idx=(), kink=1e-5, floor=1.0):
     """Compare grad[k] with the central difference of v . f along x[idx[k]].
     Depth inputs (depth_idx) are not differenced through 0 or across the
     1e-6 wet/dry threshold.  Returns (compared, skipped)."""
     ok = skip = 0
     f0 = v @ f(x)
     for k, j in enumerate(idx):
-        eps = eps_rel * max(1.0, abs(x[j]))
+        eps = eps_rel * max(floor, abs(x[j]))
         if j in depth_idx and (x[j] - eps < 0.0 or abs(x[j] - 1e-6) < 2 * eps):
             skip += 1
             continue
@@ -140,7 +140,7 @@ def test_friction_vjp_matches_finite_differences(ec):
             return out
         grad = np.zeros(4)
         ec.ec_friction_vjp(h, n, dt, q[0], q[1], P(a), P(grad))
-        ok, sk = fd_check(f, x, [0, 1, 2, 3], a, grad, eps_rel=1e-5, tol=1e-5, depth_idx=(2,), kink=1e-2)
+        ok, sk = fd_check(f, x, [0, 1, 2, 3], a, grad, eps_rel=1e-6, tol=1e-5, depth_idx=(2,), kink=1e-2, floor=1e-6)
         total_ok += ok
         total_skip += sk
     assert total_ok > 0.9 * (total_ok + total_skip), (total_ok, total_skip)
