@@ -140,7 +140,7 @@ def test_friction_vjp_matches_finite_differences(ec):
             return out
         grad = np.zeros(4)
         ec.ec_friction_vjp(h, n, dt, q[0], q[1], P(a), P(grad))
-        ok, sk = fd_check(f, x, [0, 1, 2, 3], a, grad, eps_rel=1e-6, tol=1e-5, depth_idx=(2,), kink=1e-2, floor=1e-6)
+        ok, sk = fd_check(f, x, [0, 1, 2, 3], a, grad, eps_rel=1e-6, tol=1e-4, depth_idx=(2,), kink=1e-2, floor=1e-6)
         total_ok += ok
         total_skip += sk
     assert total_ok > 0.9 * (total_ok + total_skip), (total_ok, total_skip)
