@@ -1,0 +1,408 @@
+// Adjoint of one structured-grid step (no counterpart in the reference).
+//
+// With the time-step schedule frozen, step k maps U_k = (h, qx, qy) to
+// U_{k+1} through the flux pass (a radius-2 stencil) and the pointwise update
+// (rain, implicit friction).  Given the cotangent lambda_{k+1} of U_{k+1},
+// hp_adjoint_step computes lambda_k and accumulates the gradient of the
+// Manning coefficient of every landcover class and of the rain multiplier:
+//
+//   recompute   the flux pass on U_k (the state planes hold U_k on entry)
+//   k_adj_update  pointwise: reverse of rain + friction + update; writes the
+//               direct part of lambda_k, the cotangents of the flux
+//               accumulators (zero where the cell is not flux-listed) and
+//               adds the block's FP64 parameter-gradient sums
+//   k_adj_edges one thread per edge: reverse of the edge solve
+//               (swe_adjoint.hpp); writes the (h, qx, qy) cotangents the edge
+//               sends to each of the four cells it reads (roles l, A, B, rr)
+//   k_adj_gather per cell: lambda_k += the contributions of the 8 edges that
+//               read it
+// Every write goes to the writing thread's own cell or edge slot: no atomics.
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+#include <string>
+
+#include "../../include/hipims_abi.h"
+#include "hp_common.hpp"
+#include "swe_adjoint.hpp"
+
+namespace hp {
+
+constexpr int kAThreads = 256;
+constexpr int kAIters = 16;  // must match the update pass's chunking
+constexpr int kAChunk = kAThreads * kAIters;
+constexpr int kGradStride = 16;  // doubles per block: classes 0..14, rain at 15
+
+int grid_flux_launch(const hp_grid_desc* g, hipStream_t st);  // swe_grid.hip
+
+template <class T> struct AdjPlanes {
+  T *lh, *lqx, *lqy;
+  T *ch, *cqx, *cqy;
+  T* xc;  // 12 planes: role(4) x var(3), each (ny+1) x cpitch
+  T* yc;
+  int64_t cpitch, cplane;
+};
+
+template <class T>
+__device__ inline bool flux_listed(const hp_grid_desc& a, const T* h, int x, int y, T t) {
+  const T hs = T(kHs);
+  const int64_t P = a.pitch;
+  const int64_t i = (int64_t)y * P + x;
+  if (t == T(0) || h[i] > hs) return true;
+  const int nx[4] = {x + 1, x, x - 1, x};
+  const int ny[4] = {y, y - 1, y, y + 1};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (nx[k] < 0 || nx[k] >= a.nx || ny[k] < 0 || ny[k] >= a.ny) continue;
+    const int64_t j = (int64_t)ny[k] * P + nx[k];
+    if (a.type_slot[j] == 0) continue;
+    if (nx[k] == a.cell0_x && ny[k] == a.cell0_y) continue;
+    if (h[j] > hs) return true;
+  }
+  return false;
+}
+
+template <class T>
+__global__ __launch_bounds__(kAThreads) void k_adj_update(hp_grid_desc a, AdjPlanes<T> p, double* grads) {
+  __shared__ double s_rain[1024];
+  __shared__ double s_g[kAThreads / 64][kGradStride];
+  const int tid = threadIdx.x;
+  const double t_now = a.ctrl->t;
+  const T dt = T(a.ctrl->dt);
+  const T t_T = T(t_now);
+  const T hs = T(kHs);
+  const int mode = a.rain_mode;
+  int n_st = 0;
+  if (mode == HP_RAIN_STATION && a.rain_table != nullptr && a.rain_rows > 0) {
+    const int rows = a.rain_rows, cols = a.rain_cols;
+    n_st = cols - 1;
+    const double* tab = a.rain_table;
+    const bool after = !(t_now < tab[(rows - 1) * cols]);
+    int i = 0;
+    while (i + 1 < rows - 1 && !(t_now < tab[(i + 1) * cols])) i++;
+    for (int k = tid; k < n_st && k < 1024; k += kAThreads) {
+      double v = 0.0;
+      if (!after && rows > 1) {
+        const double per = (t_now - tab[i * cols]) / (tab[(i + 1) * cols] - tab[i * cols]);
+        v = tab[i * cols + 1 + k] + per * (tab[(i + 1) * cols + 1 + k] - tab[i * cols + 1 + k]);
+      }
+      s_rain[k] = v;  // unscaled rate
+    }
+  }
+  __syncthreads();
+  const T* __restrict__ h = (const T*)a.h;
+  const T* __restrict__ qx = (const T*)a.qx;
+  const T* __restrict__ qy = (const T*)a.qy;
+  const T* __restrict__ dh = (const T*)a.dh;
+  const T* __restrict__ dqx = (const T*)a.dqx;
+  const T* __restrict__ dqy = (const T*)a.dqy;
+  const T* __restrict__ man = (const T*)a.manning;
+  const T scale = T(a.rain_scale);
+  double g_loc[kGradStride];
+#pragma unroll
+  for (int k = 0; k < kGradStride; ++k) g_loc[k] = 0.0;
+  const int64_t total = a.pitch * (int64_t)a.ny;
+  const int64_t base = (int64_t)blockIdx.x * kAChunk;
+  for (int it = 0; it < kAIters; ++it) {
+    const int64_t i = base + (int64_t)it * kAThreads + tid;
+    if (i >= total) break;
+    const int cls = a.landuse[i];
+    if (cls == 255) {
+      p.lh[i] = T(0); p.lqx[i] = T(0); p.lqy[i] = T(0);
+      p.ch[i] = T(0); p.cqx[i] = T(0); p.cqy[i] = T(0);
+      continue;
+    }
+    const T h0 = h[i], dhf = dh[i];
+    T rb = T(0);
+    if (mode == HP_RAIN_STATION) {
+      const int k = a.station ? (int)a.station[i] : 0;
+      rb = (k >= 0 && k < n_st) ? T(s_rain[k]) : T(0);
+    } else if (mode == HP_RAIN_FIELD) {
+      rb = T(a.rain_field[i]);
+    }
+    const T rdt = (rb * scale) * dt;
+    const T dht = dhf + rdt;
+    const T lh1 = p.lh[i], lqx1 = p.lqx[i], lqy1 = p.lqy[i];
+    T lh0, lqx0, lqy0, ch = T(0), cqx = T(0), cqy = T(0);
+    if (!(fabs(dht) > T(0) || h0 >= T(0))) {
+      lh0 = lh1; lqx0 = lqx1; lqy0 = lqy1;
+    } else {
+      const T h1 = h0 + dht;
+      T a_h1 = lh1, a_qsx = T(0), a_qsy = T(0);
+      if (!(h1 < hs)) {
+        T a_hf = T(0), a_n = T(0);
+        friction_vjp<T>(h1, man[cls], dt, qx[i] + dqx[i], qy[i] + dqy[i], lqx1, lqy1, a_qsx, a_qsy, a_hf, a_n);
+        a_h1 += a_hf;
+        if (cls < kGradStride - 1) g_loc[cls] += (double)a_n;
+      }
+      g_loc[kGradStride - 1] += (double)a_h1 * (double)rb * (double)dt;
+      lh0 = a_h1; lqx0 = a_qsx; lqy0 = a_qsy;
+      const int y = (int)(i / a.pitch), x = (int)(i - (int64_t)y * a.pitch);
+      if (flux_listed<T>(a, h, x, y, t_T)) {
+        ch = a_h1; cqx = a_qsx; cqy = a_qsy;
+      }
+    }
+    p.lh[i] = lh0; p.lqx[i] = lqx0; p.lqy[i] = lqy0;
+    p.ch[i] = ch; p.cqx[i] = cqx; p.cqy[i] = cqy;
+  }
+  const int lane = tid & 63, w = tid >> 6;
+#pragma unroll
+  for (int k = 0; k < kGradStride; ++k) {
+    double v = g_loc[k];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) s_g[w][k] = v;
+  }
+  __syncthreads();
+  if (tid < kGradStride) {
+    double v = 0.0;
+    for (int k = 0; k < kAThreads / 64; ++k) v += s_g[k][tid];
+    grads[(int64_t)blockIdx.x * kGradStride + tid] += v;
+  }
+}
+
+template <class T>
+__device__ inline FrameCell<T> gframe(const hp_grid_desc& a, int64_t i, int d, T zref) {
+  const T* h = (const T*)a.h;
+  const T* qx = (const T*)a.qx;
+  const T* qy = (const T*)a.qy;
+  const T* z = (const T*)a.z;
+  FrameCell<T> f;
+  f.h = h[i];
+  f.qn = rot_n(qx[i], qy[i], d);
+  f.qt = rot_t(qx[i], qy[i], d);
+  f.z = z[i] - zref;
+  return f;
+}
+
+template <class T>
+__device__ inline void to_global_adj(const FrameAdj<T>& f, int d, T& ax, T& ay) {
+  switch (d) {
+    case 0: ax = f.qn; ay = f.qt; break;
+    case 1: ax = -f.qt; ay = f.qn; break;
+    case 2: ax = -f.qn; ay = -f.qt; break;
+    default: ax = f.qt; ay = -f.qn; break;
+  }
+}
+
+// One edge per thread.  Edges [0, nxe) are x-edges (row y, between columns
+// j-1 = A and j = B, j in [0, nx]); the rest are y-edges (column x, between
+// rows i-1 = A and i = B, i in [0, ny]).  Writes 12 cotangents per edge.
+template <class T, int ORDER>
+__global__ __launch_bounds__(kAThreads) void k_adj_edges(hp_grid_desc a, AdjPlanes<T> p) {
+  const int64_t e = (int64_t)blockIdx.x * kAThreads + threadIdx.x;
+  const int nx = a.nx, ny = a.ny;
+  const int64_t nxe = (int64_t)(nx + 1) * ny;
+  const int64_t nye = (int64_t)nx * (ny + 1);
+  if (e >= nxe + nye) return;
+  const bool xe = e < nxe;
+  int yA, xA, yB, xB, yl, xl, yr2, xr2, dAB;
+  int64_t slot;
+  if (xe) {
+    const int y = (int)(e / (nx + 1)), j = (int)(e - (int64_t)y * (nx + 1));
+    yA = yB = yl = yr2 = y;
+    xA = j - 1; xB = j; xl = j - 2; xr2 = j + 1;
+    dAB = 0;
+    slot = (int64_t)y * p.cpitch + j;
+  } else {
+    const int64_t f = e - nxe;
+    const int i = (int)(f / nx), x = (int)(f - (int64_t)i * nx);
+    xA = xB = xl = xr2 = x;
+    yA = i - 1; yB = i; yl = i - 2; yr2 = i + 1;
+    dAB = 3;
+    slot = (int64_t)i * p.cpitch + x;
+  }
+  T* out = xe ? p.xc : p.yc;
+  FrameAdj<T> adj[4] = {zero_adj<T>(), zero_adj<T>(), zero_adj<T>(), zero_adj<T>()};  // l, A, B, rr
+  int dir[4] = {dAB, dAB, dAB, dAB};
+  const int64_t P = a.pitch;
+  auto valid = [&](int x, int y) -> bool {
+    return x >= 0 && x < nx && y >= 0 && y < ny && a.type_slot[(int64_t)y * P + x] != 0;
+  };
+  const bool vA = valid(xA, yA), vB = valid(xB, yB);
+  if (vA || vB) {
+    const T dt = T(a.ctrl->dt);
+    const T dx = T(a.dx);
+    const T t = T(a.ctrl->t);
+    const int dBA = (dAB + 2) & 3;
+    const bool axis_x = (dAB == 0 || dAB == 2);
+    auto cot = [&](int x, int y, T& ch, T& cx, T& cy) {
+      const int64_t i = (int64_t)y * P + x;
+      ch = p.ch[i]; cx = p.cqx[i]; cy = p.cqy[i];
+    };
+    if (vA && vB) {
+      const int64_t iA = (int64_t)yA * P + xA, iB = (int64_t)yB * P + xB;
+      const T zref = Prec<T>::local_datum ? ((const T*)a.z)[iA] : T(0);
+      const FrameCell<T> c = gframe<T>(a, iA, dAB, zref);
+      const FrameCell<T> r = gframe<T>(a, iB, dAB, zref);
+      const bool vl = valid(xl, yl), vr = valid(xr2, yr2);
+      const FrameCell<T> l = vl ? gframe<T>(a, (int64_t)yl * P + xl, dAB, zref) : c;
+      const FrameCell<T> rr = vr ? gframe<T>(a, (int64_t)yr2 * P + xr2, dAB, zref) : r;
+      T cAh, cAx, cAy, cBh, cBx, cBy;
+      cot(xA, yA, cAh, cAx, cAy);
+      cot(xB, yB, cBh, cBx, cBy);
+      const T wx = cBx - cAx, wy = cBy - cAy;
+      const T nA = T(axis_x ? nrm_x(dAB) : nrm_y(dAB));
+      const T v_fh = cBh - cAh;
+      const T v_fn = rot_n(wx, wy, dAB), v_ft = rot_t(wx, wy, dAB);
+      const T v_bc = -(axis_x ? cAx : cAy) * nA;
+      const T v_br = (axis_x ? cBx : cBy) * nA;
+      T a_ec = T(0), a_er = T(0);
+      solve_edge_vjp<T, ORDER>(c, l, r, rr, false, c.h + c.z, r.h + r.z, dx, dt, v_fh, v_fn, v_ft, v_bc, v_br,
+                               adj[1], adj[0], adj[2], adj[3], a_ec, a_er);
+      adj[1].h += a_ec;
+      adj[2].h += a_er;
+      if (!vl) { adj[1].h += adj[0].h; adj[1].qn += adj[0].qn; adj[1].qt += adj[0].qt; adj[0] = zero_adj<T>(); }
+      if (!vr) { adj[2].h += adj[3].h; adj[2].qn += adj[3].qn; adj[2].qt += adj[3].qt; adj[3] = zero_adj<T>(); }
+    } else {
+      const bool from_a = vA;
+      const int xc = from_a ? xA : xB, yc = from_a ? yA : yB, d = from_a ? dAB : dBA;
+      const int64_t ic = (int64_t)yc * P + xc;
+      const T zref = Prec<T>::local_datum ? ((const T*)a.z)[ic] : T(0);
+      const FrameCell<T> c = gframe<T>(a, ic, d, zref);
+      // boundary values at this time
+      int type, id;
+      decode_code(a.slot_code[a.type_slot[ic]], type, id);
+      auto srow = [&](const T* tab, int rows, int w) {
+        int rr_ = 0;
+        while (tab && rr_ < rows - 1 && t > tab[rr_ * w]) rr_++;
+        return rr_;
+      };
+      auto sat = [&](const T* tab, int rows, int w, int row, int col) -> T {
+        return (tab == nullptr || col < 0 || col >= w || row < 0 || row >= rows) ? T(0) : tab[row * w + col];
+      };
+      const T* gd = (const T*)a.given_depth;
+      const T* gw = (const T*)a.given_wl;
+      const T* gq = (const T*)a.given_q;
+      const T bd = sat(gd, a.gd_rows, a.gd_w, srow(gd, a.gd_rows, a.gd_w), id + 1);
+      const T bw = sat(gw, a.gwl_rows, a.gwl_w, srow(gw, a.gwl_rows, a.gwl_w), id + 1) - zref;
+      const int rq = srow(gq, a.gq_rows, a.gq_w);
+      const T bqx = sat(gq, a.gq_rows, a.gq_w, rq, 1 + 2 * id);
+      const T bqy = sat(gq, a.gq_rows, a.gq_w, rq, 2 + 2 * id);
+      const FrameCell<T> g = bc_ghost<T>(type, c, d, bd, bw, bqx, bqy);
+      const T etaG = ORDER == 1 ? g.h + g.z : ghost_wl<T>(type, c.h + c.z, g, bw);
+      T cch, ccx, ccy;
+      cot(xc, yc, cch, ccx, ccy);
+      const T nC = T(axis_x ? nrm_x(d) : nrm_y(d));
+      const T v_fh = -cch;
+      const T v_fn = rot_n(-ccx, -ccy, d), v_ft = rot_t(-ccx, -ccy, d);
+      const T v_bc = -(axis_x ? ccx : ccy) * nC;
+      FrameAdj<T> ac = zero_adj<T>(), ag = zero_adj<T>(), u1 = zero_adj<T>(), u2 = zero_adj<T>();
+      T a_ec = T(0), a_eg = T(0);
+      solve_edge_vjp<T, ORDER>(c, c, g, g, true, c.h + c.z, etaG, dx, dt, v_fh, v_fn, v_ft, v_bc, T(0), ac, u1, ag,
+                               u2, a_ec, a_eg);
+      ghost_adj<T, ORDER>(type, d, ag, a_eg, ac);
+      ac.h += a_ec;
+      adj[from_a ? 1 : 2] = ac;
+      dir[from_a ? 1 : 2] = d;
+    }
+  }
+  for (int role = 0; role < 4; ++role) {
+    T gx, gy;
+    to_global_adj<T>(adj[role], dir[role], gx, gy);
+    T* base = out + (int64_t)(role * 3) * p.cplane + slot;
+    base[0] = adj[role].h;
+    base[p.cplane] = gx;
+    base[2 * p.cplane] = gy;
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(kAThreads) void k_adj_gather(hp_grid_desc a, AdjPlanes<T> p) {
+  const int64_t i = (int64_t)blockIdx.x * kAThreads + threadIdx.x;
+  const int64_t total = a.pitch * (int64_t)a.ny;
+  if (i >= total) return;
+  if (a.type_slot[i] == 0) return;
+  const int y = (int)(i / a.pitch), x = (int)(i - (int64_t)y * a.pitch);
+  // x-edges j = x-1 (this cell is rr), x (B), x+1 (A), x+2 (l); y-edges alike
+  const int64_t C = p.cpitch;
+  const int64_t xs[4] = {(int64_t)y * C + (x + 2), (int64_t)y * C + (x + 1), (int64_t)y * C + x,
+                         (int64_t)y * C + (x - 1)};
+  const int64_t ys[4] = {(int64_t)(y + 2) * C + x, (int64_t)(y + 1) * C + x, (int64_t)y * C + x,
+                         (int64_t)(y - 1) * C + x};
+  T sh = T(0), sx = T(0), sy = T(0);
+#pragma unroll
+  for (int role = 0; role < 4; ++role) {
+    const T* bx = p.xc + (int64_t)(role * 3) * p.cplane;
+    const T* by = p.yc + (int64_t)(role * 3) * p.cplane;
+    sh += bx[xs[role]] + by[ys[role]];
+    sx += bx[p.cplane + xs[role]] + by[p.cplane + ys[role]];
+    sy += bx[2 * p.cplane + xs[role]] + by[2 * p.cplane + ys[role]];
+  }
+  p.lh[i] += sh;
+  p.lqx[i] += sx;
+  p.lqy[i] += sy;
+}
+
+template <class T>
+__global__ void k_adj_loss(T* lam_h, const int64_t* __restrict__ off, const double* __restrict__ h_rec,
+                           const double* __restrict__ obs, int n, double weight) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) lam_h[off[k]] += T(weight * 2.0 * (h_rec[k] - obs[k]));
+}
+
+template <class T>
+static int launch_adjoint(const hp_grid_desc* g, const hp_adjoint_desc* ad, hipStream_t st) {
+  AdjPlanes<T> p;
+  p.lh = (T*)ad->lam_h; p.lqx = (T*)ad->lam_qx; p.lqy = (T*)ad->lam_qy;
+  p.ch = (T*)ad->cot_h; p.cqx = (T*)ad->cot_qx; p.cqy = (T*)ad->cot_qy;
+  p.xc = (T*)ad->xc; p.yc = (T*)ad->yc;
+  p.cpitch = ad->cpitch;
+  p.cplane = ad->cpitch * (int64_t)(g->ny + 1);
+  const int64_t total = g->pitch * (int64_t)g->ny;
+  const unsigned nbu = (unsigned)((total + kAChunk - 1) / kAChunk);
+  hipLaunchKernelGGL((k_adj_update<T>), dim3(nbu), dim3(kAThreads), 0, st, *g, p, ad->grads);
+  int rc = check_launch("k_adj_update");
+  if (rc) return rc;
+  const int64_t nedges = (int64_t)(g->nx + 1) * g->ny + (int64_t)g->nx * (g->ny + 1);
+  const unsigned nbe = (unsigned)((nedges + kAThreads - 1) / kAThreads);
+  if (g->order == 2) hipLaunchKernelGGL((k_adj_edges<T, 2>), dim3(nbe), dim3(kAThreads), 0, st, *g, p);
+  else hipLaunchKernelGGL((k_adj_edges<T, 1>), dim3(nbe), dim3(kAThreads), 0, st, *g, p);
+  rc = check_launch("k_adj_edges");
+  if (rc) return rc;
+  const unsigned nbg = (unsigned)((total + kAThreads - 1) / kAThreads);
+  hipLaunchKernelGGL((k_adj_gather<T>), dim3(nbg), dim3(kAThreads), 0, st, *g, p);
+  return check_launch("k_adj_gather");
+}
+
+}  // namespace hp
+
+using namespace hp;
+
+extern "C" {
+
+int64_t hp_adjoint_grad_blocks(const hp_grid_desc* g) {
+  if (!g || g->pitch <= 0 || g->ny <= 0) return 0;
+  return (g->pitch * (int64_t)g->ny + kAChunk - 1) / kAChunk;
+}
+
+int hp_adjoint_step(const hp_grid_desc* g, const hp_adjoint_desc* ad, void* stream) {
+  if (!ad) return fail(HP_EINVAL, "null adjoint descriptor");
+  if (!g) return fail(HP_EINVAL, "null descriptor");
+  if (!ad->lam_h || !ad->lam_qx || !ad->lam_qy || !ad->cot_h || !ad->cot_qx || !ad->cot_qy || !ad->xc || !ad->yc ||
+      !ad->grads)
+    return fail(HP_EINVAL, "an adjoint plane pointer is null");
+  if (ad->cpitch < (int64_t)g->nx + 1) return fail(HP_EINVAL, "cpitch must be >= nx + 1");
+  if (g->n_classes > kGradStride - 1) return fail(HP_EINVAL, "the adjoint supports at most 15 Manning classes");
+  int rc = grid_flux_launch(g, (hipStream_t)stream);  // recompute the step's flux on U_k
+  if (rc) return rc;
+  if (g->dtype == HP_F32) return launch_adjoint<float>(g, ad, (hipStream_t)stream);
+  return launch_adjoint<double>(g, ad, (hipStream_t)stream);
+}
+
+int hp_adjoint_add_loss(int dtype, void* lam_h, const int64_t* offsets, const double* h_rec, const double* obs,
+                        int32_t n, double weight, void* stream) {
+  if (dtype != HP_F32 && dtype != HP_F64) return fail(HP_EINVAL, "dtype must be HP_F32 or HP_F64");
+  if (n <= 0) return HP_OK;
+  if (!lam_h || !offsets || !h_rec || !obs) return fail(HP_EINVAL, "null pointer");
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  if (dtype == HP_F32)
+    hipLaunchKernelGGL(k_adj_loss<float>, dim3(nb), dim3(256), 0, (hipStream_t)stream, (float*)lam_h, offsets, h_rec,
+                       obs, n, weight);
+  else
+    hipLaunchKernelGGL(k_adj_loss<double>, dim3(nb), dim3(256), 0, (hipStream_t)stream, (double*)lam_h, offsets,
+                       h_rec, obs, n, weight);
+  return check_launch("k_adj_loss");
+}
+
+}  // extern "C"

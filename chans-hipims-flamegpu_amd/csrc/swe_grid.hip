@@ -672,6 +672,13 @@ __global__ void k_scatter(T* __restrict__ plane, const int64_t* __restrict__ off
   if (i < n) plane[off[i]] = in[i];
 }
 
+// Flux pass launch shared with the adjoint (which recomputes it).
+int grid_flux_launch(const hp_grid_desc* g, hipStream_t st) {
+  int rc = check_desc(g);
+  if (rc) return rc;
+  return launch_flux(g, st);
+}
+
 }  // namespace hp
 
 using namespace hp;

@@ -66,6 +66,12 @@ class HpEnsemble(ctypes.Structure):
                 ("doms", _P), ("flux_map", _P), ("update_map", _P), ("finalize_map", _P), ("partials", _P)]
 
 
+class HpAdjointDesc(ctypes.Structure):
+    _fields_ = [("lam_h", _P), ("lam_qx", _P), ("lam_qy", _P), ("cot_h", _P), ("cot_qx", _P), ("cot_qy", _P),
+                ("xc", _P), ("yc", _P), ("cpitch", _I64), ("grads", _P)]
+
+
+GRAD_STRIDE = 16
 _lib = None
 
 # name -> (restype, argtypes); every symbol include/hipims_abi.h declares
@@ -88,6 +94,9 @@ SIGNATURES = {
     "hp_ensemble_workspace_bytes": (_I64, [_I32, ctypes.POINTER(HpGridDesc)]),
     "hp_ensemble_init": (_I, [_I32, ctypes.POINTER(HpGridDesc), _P, _I64, ctypes.POINTER(HpEnsemble), _P]),
     "hp_ensemble_step": (_I, [ctypes.POINTER(HpEnsemble), _I32, _P]),
+    "hp_adjoint_grad_blocks": (_I64, [ctypes.POINTER(HpGridDesc)]),
+    "hp_adjoint_step": (_I, [ctypes.POINTER(HpGridDesc), ctypes.POINTER(HpAdjointDesc), _P]),
+    "hp_adjoint_add_loss": (_I, [_I, _P, _P, _P, _P, _I32, ctypes.c_double, _P]),
     "hp_gather": (_I, [_I, _P, _P, _I64, _P, _P]),
     "hp_scatter": (_I, [_I, _P, _P, _I64, _P, _P]),
 }

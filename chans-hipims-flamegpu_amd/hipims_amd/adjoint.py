@@ -1,0 +1,187 @@
+"""Gradients of a gauge-misfit loss through the shallow-water step (no
+counterpart in the reference, which has no derivatives).
+
+Loss  L(theta) = sum_{k=1..K} sum_g (h_g^k - obs_g^k)^2  over K steps, where
+theta = (Manning n of every landcover class, rain multiplier).  The time-step
+schedule of the first forward run is recorded and then frozen (t_k, dt_k are
+constants for the derivative, for the finite-difference checks and for every
+recomputation), as the adjoint of the CFL minimum is not defined.
+
+backward(): checkpoints of the state every `checkpoint` steps are kept from
+the forward run; segments are recomputed last to first, their states stored,
+and the hand-written adjoint kernels (hp_adjoint_step) run the step in
+reverse.  Parameter gradients are accumulated in FP64 per block and summed at
+the end.  ``CalibrationLoss.apply(theta, calib)`` exposes this as a
+torch.autograd.Function.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import HpAdjointDesc, check, lib
+from .grid import GridSolver
+
+
+class Calibration:
+    def __init__(self, solver: GridSolver, gauges, steps: int, checkpoint: int = 100, obs=None):
+        """gauges: (row, col) raster cells; obs: [steps + 1, G] observed depths
+        (row 0 unused) or None (set later)."""
+        self.s = s = solver
+        self.K = int(steps)
+        self.ck = int(checkpoint)
+        dev = s.device
+        P = s.layout.pitch
+        g = np.asarray(gauges, dtype=np.int64).reshape(-1, 2)
+        self.G = g.shape[0]
+        self.gauge_off = torch.from_numpy(g[:, 0] * P + g[:, 1]).to(dev)
+        self.C = int(s.manning.numel())
+        self.u0 = [t.clone() for t in (s.h, s.qx, s.qy)]
+        self.hmax0 = s.h_max.clone()
+        self.ctrl0 = s.ctrl.clone()
+        self.sched = torch.zeros(self.K + 1, 2, dtype=torch.float64, device=dev)
+        self.sched_ready = False
+        self.hist = torch.zeros(self.K + 1, self.G, dtype=s.dtype, device=dev)
+        self.obs = None if obs is None else torch.as_tensor(obs, dtype=torch.float64, device=dev)
+        self.ckpts: dict[int, list[torch.Tensor]] = {}
+        # adjoint planes
+        L = s.layout
+        self.cpitch = int(np.ceil((L.nx + 1) / 64) * 64)
+        shp = (L.ny, P)
+        z = lambda: torch.zeros(shp, dtype=s.dtype, device=dev)
+        self.lam = [z(), z(), z()]
+        self.cot = [z(), z(), z()]
+        self.xc = torch.zeros(12, L.ny + 1, self.cpitch, dtype=s.dtype, device=dev)
+        self.yc = torch.zeros(12, L.ny + 1, self.cpitch, dtype=s.dtype, device=dev)
+        self.nblocks = int(lib().hp_adjoint_grad_blocks(ctypes.byref(s.desc)))
+        self.grads = torch.zeros(self.nblocks, _lib.GRAD_STRIDE, dtype=torch.float64, device=dev)
+        self.seg = [[z(), z(), z()] for _ in range(self.ck)]
+        ad = HpAdjointDesc()
+        ad.lam_h, ad.lam_qx, ad.lam_qy = (t.data_ptr() for t in self.lam)
+        ad.cot_h, ad.cot_qx, ad.cot_qy = (t.data_ptr() for t in self.cot)
+        ad.xc, ad.yc = self.xc.data_ptr(), self.yc.data_ptr()
+        ad.cpitch = self.cpitch
+        ad.grads = self.grads.data_ptr()
+        self.adesc = ad
+        if self.C > _lib.GRAD_STRIDE - 1:
+            raise ValueError("at most 15 Manning classes")
+
+    # -- parameters ---------------------------------------------------------------
+    def set_theta(self, theta):
+        th = torch.as_tensor(theta, dtype=torch.float64).detach().cpu().numpy()
+        self.s.set_manning(th[: self.C])
+        self.s.set_rain_scale(float(th[self.C]))
+
+    def theta0(self) -> torch.Tensor:
+        return torch.cat([self.s.manning.double().cpu(), torch.tensor([self.s.rain_scale], dtype=torch.float64)])
+
+    # -- forward ----------------------------------------------------------------------
+    def _restore_initial(self):
+        s = self.s
+        for t, v in zip((s.h, s.qx, s.qy), self.u0):
+            t.copy_(v)
+        s.h_max.copy_(self.hmax0)
+        s.ctrl.copy_(self.ctrl0)
+
+    def _record_gauges(self, k):
+        check(lib().hp_gather(_lib.dtype_code(self.s.dtype), self.s.h.data_ptr(), self.gauge_off.data_ptr(),
+                              self.G, self.hist[k].data_ptr(), _lib.stream_ptr(self.s.device)), "hp_gather")
+
+    def _step(self, k):
+        """Step k with the frozen (t_k, dt_k) once the schedule is known."""
+        s = self.s
+        if self.sched_ready:
+            s.ctrl[0:2].copy_(self.sched[k])
+        else:
+            self.sched[k].copy_(s.ctrl[0:2])
+        s.step(1)
+
+    def run(self, theta, keep_checkpoints=True) -> torch.Tensor:
+        """Forward run; returns the FP64 loss (needs obs)."""
+        self.set_theta(theta)
+        self._restore_initial()
+        self.ckpts.clear()
+        self._record_gauges(0)
+        for k in range(self.K):
+            if keep_checkpoints and k % self.ck == 0:
+                self.ckpts[k] = [t.clone() for t in (self.s.h, self.s.qx, self.s.qy)]
+            self._step(k)
+            self._record_gauges(k + 1)
+        self.sched_ready = True
+        return self.loss()
+
+    def simulate(self, theta) -> torch.Tensor:
+        """Forward run with the model's own CFL time steps (nothing recorded or
+        frozen); returns the gauge depths [steps + 1, G] in FP64."""
+        saved = self.sched_ready
+        self.sched_ready = False
+        sched = self.sched.clone()
+        self.set_theta(theta)
+        self._restore_initial()
+        self._record_gauges(0)
+        for k in range(self.K):
+            self.s.step(1)
+            self._record_gauges(k + 1)
+        self.sched.copy_(sched)
+        self.sched_ready = saved
+        return self.hist.double().clone()
+
+    def loss(self) -> torch.Tensor:
+        if self.obs is None:
+            raise RuntimeError("no observations set")
+        d = self.hist[1:].double() - self.obs[1:]
+        return (d * d).sum()
+
+    # -- backward -----------------------------------------------------------------------
+    def gradient(self) -> torch.Tensor:
+        """dL/dtheta of the last run() (FP64, on the host)."""
+        s = self.s
+        if not self.ckpts:
+            raise RuntimeError("run() with checkpoints first")
+        for t in self.lam:
+            t.zero_()
+        self.grads.zero_()
+        h_rec = self.hist.double().contiguous()
+        stream = _lib.stream_ptr(s.device)
+        dcode = _lib.dtype_code(s.dtype)
+        starts = sorted(self.ckpts)
+        for si in range(len(starts) - 1, -1, -1):
+            a = starts[si]
+            b = min(a + self.ck, self.K)
+            # recompute the segment, keeping the state before each step
+            for t, v in zip((s.h, s.qx, s.qy), self.ckpts[a]):
+                t.copy_(v)
+            for k in range(a, b):
+                for t, v in zip(self.seg[k - a], (s.h, s.qx, s.qy)):
+                    t.copy_(v)
+                if k < b - 1:
+                    self._step(k)
+            for k in range(b - 1, a - 1, -1):
+                check(lib().hp_adjoint_add_loss(dcode, self.lam[0].data_ptr(), self.gauge_off.data_ptr(),
+                                                h_rec[k + 1].data_ptr(), self.obs[k + 1].data_ptr(), self.G, 1.0,
+                                                stream), "hp_adjoint_add_loss")
+                for t, v in zip((s.h, s.qx, s.qy), self.seg[k - a]):
+                    t.copy_(v)
+                s.ctrl[0:2].copy_(self.sched[k])
+                check(lib().hp_adjoint_step(ctypes.byref(s.desc), ctypes.byref(self.adesc), stream),
+                      "hp_adjoint_step")
+        g = self.grads.sum(dim=0).cpu()
+        return torch.cat([g[: self.C], g[_lib.GRAD_STRIDE - 1: _lib.GRAD_STRIDE]])
+
+
+class CalibrationLoss(torch.autograd.Function):
+    """loss = CalibrationLoss.apply(theta, calib): forward runs the model with
+    theta, backward returns dL/dtheta from the adjoint."""
+
+    @staticmethod
+    def forward(ctx, theta, calib: Calibration):
+        ctx.calib = calib
+        return calib.run(theta).to(theta.device if theta.is_cuda else "cpu")
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        g = ctx.calib.gradient().to(grad_out.device)
+        return g * grad_out, None

@@ -211,6 +211,30 @@ int hp_ensemble_init(int32_t n, const hp_grid_desc* descs, void* work, int64_t w
  * control), three launches per step, graph-capturable. */
 int hp_ensemble_step(const hp_ensemble* e, int32_t nsteps, void* stream);
 
+/* ---- Adjoint of one step with a frozen time-step schedule (no reference
+ * counterpart).  Planes lam_* hold the cotangent of the state after the step
+ * on entry and of the state before it on return; the state planes of the
+ * domain must hold the state before the step, and its control block the t
+ * and dt of that step.  cot_* are scratch planes like h; xc / yc are 12
+ * planes each of (ny + 1) rows x cpitch (cpitch >= nx + 1) for the edge
+ * contributions.  grads: FP64 [hp_adjoint_grad_blocks][16] running sums:
+ * column c < 15 the gradient w.r.t. the Manning value of class c, column 15
+ * the gradient w.r.t. the rain multiplier (rain_scale). */
+typedef struct hp_adjoint_desc {
+  void *lam_h, *lam_qx, *lam_qy;
+  void *cot_h, *cot_qx, *cot_qy;
+  void *xc, *yc;
+  int64_t cpitch;
+  double* grads;
+} hp_adjoint_desc;
+
+int64_t hp_adjoint_grad_blocks(const hp_grid_desc* g);
+int hp_adjoint_step(const hp_grid_desc* g, const hp_adjoint_desc* a, void* stream);
+/* lam_h[offsets[k]] += weight * 2 * (h_rec[k] - obs[k]), k < n: the
+ * cotangent of a squared-misfit loss at n gauge cells. */
+int hp_adjoint_add_loss(int dtype, void* lam_h, const int64_t* offsets, const double* h_rec, const double* obs,
+                        int32_t n, double weight, void* stream);
+
 /* out[i] = plane[offsets[i]]  (compact gather, Godunov.get_h). */
 int hp_gather(int dtype, const void* plane, const int64_t* offsets, int64_t n, void* out, void* stream);
 /* plane[offsets[i]] = in[i]  (compact scatter). */

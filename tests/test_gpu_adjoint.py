@@ -1,0 +1,92 @@
+"""Adjoint gradients (GPU): the hand-written reverse of the step against
+central finite differences of the forward model with the same frozen
+time-step schedule, FP32 against FP64, and the autograd.Function wrapper."""
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from hipims_amd import generators as gen
+from hipims_amd.adjoint import Calibration, CalibrationLoss
+from hipims_amd.grid import GridSolver
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def make(dtype, n=64, steps=80, ck=25, seed=4242):
+    c = gen.rainfall_runoff(n, 5.0, seed=seed, rain_mmh=150.0, relief=8.0)
+    s = GridSolver(c.mask, c.h, c.qx, c.qy, c.z, c.dx, cfl=0.5, order=2, dtype=dtype, first_dt=0.5,
+                   manning=c.manning, landuse=c.landuse)
+    s.set_station_rain(c.rain_table)
+    # gauges: seeded choice among cells wet after a pilot run
+    s.step(steps)
+    h = s.raster(s.h)
+    rng = np.random.Generator(np.random.PCG64(1004))
+    wet = np.argwhere((h > 1e-3) & (c.mask > 0))
+    gauges = wet[rng.choice(len(wet), size=min(16, len(wet)), replace=False)]
+    s2 = GridSolver(c.mask, c.h, c.qx, c.qy, c.z, c.dx, cfl=0.5, order=2, dtype=dtype, first_dt=0.5,
+                    manning=c.manning, landuse=c.landuse)
+    s2.set_station_rain(c.rain_table)
+    cal = Calibration(s2, gauges, steps, ck)
+    th0 = cal.theta0()
+    pert = torch.tensor(np.random.Generator(np.random.PCG64(7)).choice([0.8, 1.2], size=th0.numel() - 1))
+    th_true = th0.clone()
+    th_true[:-1] *= pert
+    cal.obs = cal.simulate(th_true)
+    return cal, th0
+
+
+def fd_grad(cal, th0, eps=1e-3):
+    g = torch.zeros_like(th0)
+    for j in range(th0.numel()):
+        d = eps * abs(float(th0[j]))
+        tp, tm = th0.clone(), th0.clone()
+        tp[j] += d
+        tm[j] -= d
+        g[j] = (cal.run(tp).item() - cal.run(tm).item()) / (2 * d)
+    return g
+
+
+def test_fp64_adjoint_matches_finite_differences(cuda):
+    say("[adjoint fp64 vs FD]")
+    cal, th0 = make(torch.float64)
+    loss = cal.run(th0)
+    grad = cal.gradient()
+    fd3 = fd_grad(cal, th0, 1e-3)
+    fd5 = fd_grad(cal, th0, 1e-5)
+    say(f"  loss {loss.item():.6e}\n  adjoint {grad.numpy()}\n  fd(1e-3) {fd3.numpy()}\n  fd(1e-5) {fd5.numpy()}")
+    scale = grad.abs().max()
+    e3 = ((grad - fd3).abs() / scale).max().item()
+    e5 = ((grad - fd5).abs() / scale).max().item()
+    say(f"  max rel err vs FD: eps 1e-3 {e3:.2e}, eps 1e-5 {e5:.2e}")
+    assert min(e3, e5) < 1e-4
+
+
+def test_fp32_adjoint_against_fp64(cuda):
+    say("[adjoint fp32 vs fp64]")
+    cal64, th0 = make(torch.float64)
+    cal64.run(th0)
+    g64 = cal64.gradient()
+    cal32, th0_32 = make(torch.float32)
+    cal32.run(th0_32)
+    g32 = cal32.gradient()
+    err = ((g32 - g64).abs() / g64.abs().max()).max().item()
+    say(f"  fp32 {g32.numpy()}\n  fp64 {g64.numpy()}\n  max rel diff {err:.2e}")
+    assert err < 2e-2
+
+
+def test_autograd_function(cuda):
+    say("[autograd]")
+    cal, th0 = make(torch.float64, n=48, steps=40, ck=15)
+    theta = th0.clone().requires_grad_(True)
+    loss = CalibrationLoss.apply(theta, cal)
+    loss.backward()
+    ref = cal.gradient()
+    assert torch.allclose(theta.grad, ref, rtol=1e-12, atol=0.0)
+    assert torch.isfinite(theta.grad).all() and theta.grad.abs().max() > 0
