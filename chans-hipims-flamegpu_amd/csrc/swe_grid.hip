@@ -27,6 +27,7 @@
 #include <math.h>
 #include <stdint.h>
 #include <cstring>
+#include <utility>
 #include <string>
 #include <vector>
 
@@ -87,6 +88,22 @@ template <class T> __device__ inline void to_global(T fn, T ft, int d, T& gx, T&
     case 2: gx = -fn; gy = -ft; break;
     default: gx = ft; gy = -fn; break;
   }
+}
+
+// --- reductions ------------------------------------------------------------
+struct Partial {
+  double min_dt, vol, flux, rain;
+};
+
+template <class T> __device__ inline T wave_min(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
 }
 
 // LDS tile of one flux block.
@@ -164,12 +181,117 @@ __device__ inline void solve_tile_edge(const FluxTile<T, TY>& s, const hp_grid_d
   }
 }
 
-template <class T, int ORDER, int TY>
-__device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by) {
+// Station rain rates at time t_now into s_rain[0..n) (SWE_CUDA.py:472-495):
+// linear in time, 0 from the table's last row on; scaled by rain_scale.
+__device__ inline int station_rates(const hp_grid_desc& a, double t_now, double* s_rain, int cap) {
+  if (a.rain_mode != HP_RAIN_STATION || a.rain_table == nullptr || a.rain_rows <= 0) return 0;
+  const int rows = a.rain_rows, cols = a.rain_cols;
+  const int n_st = cols - 1;
+  const double* tab = a.rain_table;
+  const bool after = !(t_now < tab[(rows - 1) * cols]);
+  int i = 0;
+  while (i + 1 < rows - 1 && !(t_now < tab[(i + 1) * cols])) i++;
+  for (int k = threadIdx.x; k < n_st && k < cap; k += blockDim.x) {
+    double v = 0.0;
+    if (!after && rows > 1) {
+      const double per = (t_now - tab[i * cols]) / (tab[(i + 1) * cols] - tab[i * cols]);
+      v = tab[i * cols + 1 + k] + per * (tab[(i + 1) * cols + 1 + k] - tab[i * cols + 1 + k]);
+    }
+    s_rain[k] = v * a.rain_scale;
+  }
+  return n_st;
+}
+
+// Fold a block's CFL minimum and FP64 sums into one partial record.
+template <class T>
+__device__ inline void block_partial(T mn, double vol, double fsum, double rsum, Partial* part) {
+  __shared__ double s_red[4][kThreads / 64];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  mn = wave_min(mn);
+  vol = wave_sum(vol);
+  fsum = wave_sum(fsum);
+  rsum = wave_sum(rsum);
+  if (lane == 0) {
+    s_red[0][w] = (double)mn;
+    s_red[1][w] = vol;
+    s_red[2][w] = fsum;
+    s_red[3][w] = rsum;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    Partial p{s_red[0][0], s_red[1][0], s_red[2][0], s_red[3][0]};
+    for (int k = 1; k < kThreads / 64; ++k) {
+      p.min_dt = fmin(p.min_dt, s_red[0][k]);
+      p.vol += s_red[1][k];
+      p.flux += s_red[2][k];
+      p.rain += s_red[3][k];
+    }
+    *part = p;
+  }
+}
+
+// Rain + implicit friction + update of one cell (HP_UPDATE_STEP semantics,
+// SWE_CUDA.py:780-831), writing the new state to (ho, qxo, qyo) and h_max,
+// and folding the cell into the block's CFL minimum and FP64 sums.
+template <class T>
+__device__ inline void update_cell(const hp_grid_desc& a, int64_t i, T h, T qx0, T qy0, T dhf, T dqx, T dqy, T dt,
+                                   const double* s_rain, int n_st, T* ho, T* qxo, T* qyo, T& mn, double& vol,
+                                   double& fsum, double& rsum) {
+  const int cls = a.landuse[i];
+  if (cls == 255) {
+    ho[i] = h; qxo[i] = qx0; qyo[i] = qy0;
+    return;
+  }
+  T r = T(0);
+  if (a.rain_mode == HP_RAIN_STATION) {
+    const int k = a.station ? (int)a.station[i] : 0;
+    r = (k >= 0 && k < n_st) ? T(s_rain[k]) : T(0);
+  } else if (a.rain_mode == HP_RAIN_FIELD) {
+    r = T(a.rain_field[i]) * T(a.rain_scale);
+  }
+  const T rdt = r * dt;
+  const T dht = dhf + rdt;
+  if (!(fabs(dht) > T(0) || h >= T(0))) {
+    ho[i] = h; qxo[i] = qx0; qyo[i] = qy0;
+    vol += (double)h;
+    return;
+  }
+  h += dht;
+  T qx, qy;
+  if (h < T(kHs)) {
+    qx = T(0);
+    qy = T(0);
+  } else {
+    qx = qx0 + dqx;
+    qy = qy0 + dqy;
+    implicit_friction<T>(h, ((const T*)a.manning)[cls], dt, qx, qy);
+  }
+  ho[i] = h;
+  qxo[i] = qx;
+  qyo[i] = qy;
+  T* hmp = (T*)a.h_max;
+  hmp[i] = fmax(h, hmp[i]);
+  mn = fmin(mn, cfl_candidate<T>(h, qx, qy, T(a.cfl) * T(a.dx)));
+  vol += (double)h;
+  fsum += (double)dhf;
+  rsum += (double)rdt;
+}
+
+// FUSED = false: write the flux accumulators dh/dqx/dqy (two-pass step).
+// FUSED = true: finish the step in the same block - rain, implicit friction
+// and update of the tile's own cells into h_out/qx_out/qy_out (the input
+// planes stay untouched for the neighbouring tiles' halos), h_max, and one
+// partial record (CFL minimum, FP64 sums) per tile.
+template <class T, int ORDER, int TY, bool FUSED>
+__device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by, Partial* part) {
   __shared__ FluxTile<T, TY> s;
+  __shared__ double s_rain[FUSED ? kMaxStations : 1];
   using Tile = FluxTile<T, TY>;
   constexpr int LW = Tile::LW, LH = Tile::LH;
-  if (stopped(a.ctrl)) return;
+  if (stopped(a.ctrl)) {
+    if (FUSED && threadIdx.x == 0) *part = Partial{INFINITY, 0.0, 0.0, 0.0};
+    return;
+  }
   const int tid = threadIdx.x;
   const int x0 = bx * kTX, y0 = by * TY;
   const int nx = a.nx, ny = a.ny;
@@ -199,6 +321,7 @@ __device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by) {
   rows.d = a.given_depth ? series_row<T>((const T*)a.given_depth, a.gd_rows, a.gd_w, t) : 0;
   rows.w = a.given_wl ? series_row<T>((const T*)a.given_wl, a.gwl_rows, a.gwl_w, t) : 0;
   rows.q = a.given_q ? series_row<T>((const T*)a.given_q, a.gq_rows, a.gq_w, t) : 0;
+  const int n_st = FUSED ? station_rates(a, a.ctrl->t, s_rain, kMaxStations) : 0;
   __syncthreads();
 
   // every x-edge (row ly, between local columns lx-1 = A and lx = B) and
@@ -227,6 +350,8 @@ __device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by) {
   T* __restrict__ dh = (T*)a.dh;
   T* __restrict__ dqx = (T*)a.dqx;
   T* __restrict__ dqy = (T*)a.dqy;
+  T mn = T(INFINITY);
+  double vol = 0.0, fsum = 0.0, rsum = 0.0;
   for (int e = tid; e < TY * kTX; e += kThreads) {
     const int ly = e / kTX, lx = e - ly * kTX;
     const int gy = y0 + ly, gx = x0 + lx;
@@ -258,15 +383,21 @@ __device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by) {
       }
     }
     const int64_t gi = (int64_t)gy * pitch + gx;
-    dh[gi] = ah;
-    dqx[gi] = aqx;
-    dqy[gi] = aqy;
+    if (!FUSED) {
+      dh[gi] = ah;
+      dqx[gi] = aqx;
+      dqy[gi] = aqy;
+    } else {
+      update_cell<T>(a, gi, s.h[r][c], s.qx[r][c], s.qy[r][c], ah, aqx, aqy, dt, s_rain, n_st, (T*)a.h_out,
+                     (T*)a.qx_out, (T*)a.qy_out, mn, vol, fsum, rsum);
+    }
   }
+  if (FUSED) block_partial<T>(mn, vol, fsum, rsum, part);
 }
 
 template <class T, int ORDER, int TY>
 __global__ __launch_bounds__(kThreads) void k_flux(hp_grid_desc a) {
-  flux_tile<T, ORDER, TY>(a, blockIdx.x, blockIdx.y);
+  flux_tile<T, ORDER, TY, false>(a, blockIdx.x, blockIdx.y, nullptr);
 }
 
 template <class T, int ORDER, int TY>
@@ -275,23 +406,13 @@ __global__ __launch_bounds__(kThreads) void k_flux_multi(const hp_grid_desc* __r
   const int2 m = map[blockIdx.x];
   const hp_grid_desc& a = doms[m.x];
   const int ntx = (a.nx + kTX - 1) / kTX;
-  flux_tile<T, ORDER, TY>(a, m.y % ntx, m.y / ntx);
+  flux_tile<T, ORDER, TY, false>(a, m.y % ntx, m.y / ntx, nullptr);
 }
 
-// --- update pass ------------------------------------------------------------
-struct Partial {
-  double min_dt, vol, flux, rain;
-};
-
-template <class T> __device__ inline T wave_min(T v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
-  return v;
-}
-__device__ inline double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+// Single-pass step: reads (h, qx, qy), writes (h_out, qx_out, qy_out).
+template <class T, int ORDER, int TY>
+__global__ __launch_bounds__(kThreads) void k_step_fused(hp_grid_desc a) {
+  flux_tile<T, ORDER, TY, true>(a, blockIdx.x, blockIdx.y, (Partial*)a.work + blockIdx.x + blockIdx.y * gridDim.x);
 }
 
 // MODE selects which of the reference driver's update methods the pass is:
@@ -579,6 +700,23 @@ static int launch_flux(const hp_grid_desc* g, hipStream_t st) {
   return g->order == 2 ? launch_flux_t<double, 2>(g, st) : launch_flux_t<double, 1>(g, st);
 }
 
+template <class T, int ORDER, int TY>
+static int launch_fused_t(const hp_grid_desc* g, hipStream_t st) {
+  dim3 grid((g->nx + kTX - 1) / kTX, (g->ny + TY - 1) / TY);
+  hipLaunchKernelGGL((k_step_fused<T, ORDER, TY>), grid, dim3(kThreads), 0, st, *g);
+  return check_launch("k_step_fused");
+}
+
+static int launch_step_fused(const hp_grid_desc* g, hipStream_t st) {
+  const int ty = flux_rows(g);
+  if (g->dtype == HP_F32) {
+    if (g->order == 2) return ty == 4 ? launch_fused_t<float, 2, 4>(g, st) : launch_fused_t<float, 2, 8>(g, st);
+    return ty == 4 ? launch_fused_t<float, 1, 4>(g, st) : launch_fused_t<float, 1, 8>(g, st);
+  }
+  if (g->order == 2) return ty == 2 ? launch_fused_t<double, 2, 2>(g, st) : launch_fused_t<double, 2, 4>(g, st);
+  return ty == 2 ? launch_fused_t<double, 1, 2>(g, st) : launch_fused_t<double, 1, 4>(g, st);
+}
+
 template <int MODE>
 static void launch_update_mode(const hp_grid_desc* g, int64_t nb, hipStream_t st) {
   if (g->dtype == HP_F32) hipLaunchKernelGGL((k_update<float, MODE>), dim3((unsigned)nb), dim3(kThreads), 0, st, *g);
@@ -686,8 +824,39 @@ using namespace hp;
 extern "C" {
 
 int64_t hp_grid_workspace_bytes(const hp_grid_desc* g) {
-  if (!g || g->pitch <= 0 || g->ny <= 0) return 0;
-  return update_blocks(g) * (int64_t)sizeof(Partial);
+  if (!g || g->pitch <= 0 || g->ny <= 0 || g->nx <= 0) return 0;
+  const int64_t n = update_blocks(g) > flux_tiles(g) ? update_blocks(g) : flux_tiles(g);
+  return n * (int64_t)sizeof(Partial);
+}
+
+int hp_grid_step_fused(const hp_grid_desc* g, int32_t nsteps, void* stream) {
+  int rc = check_desc(g);
+  if (rc) return rc;
+  if (nsteps < 0) return fail(HP_EINVAL, "nsteps must be >= 0");
+  if (!g->h_out || !g->qx_out || !g->qy_out) return fail(HP_EINVAL, "the fused step needs h_out, qx_out, qy_out");
+  const int64_t nt = flux_tiles(g);
+  if (!g->work || g->work_bytes < nt * (int64_t)sizeof(Partial))
+    return fail(HP_ESPACE, "workspace smaller than hp_grid_workspace_bytes()");
+  hipStream_t st = (hipStream_t)stream;
+  hp_grid_desc d = *g;
+  for (int32_t k = 0; k < nsteps; ++k) {
+    rc = launch_step_fused(&d, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, st, d.ctrl, (const Partial*)d.work, (int)nt, d.dx);
+    rc = check_launch("k_finalize");
+    if (rc) return rc;
+    std::swap(d.h, d.h_out);
+    std::swap(d.qx, d.qx_out);
+    std::swap(d.qy, d.qy_out);
+  }
+  if (nsteps & 1) {  // bring the result back to the caller's state planes
+    const size_t bytes = (size_t)(g->pitch * (int64_t)g->ny) * (g->dtype == HP_F32 ? 4 : 8);
+    hipError_t e = hipMemcpyAsync(g->h, g->h_out, bytes, hipMemcpyDeviceToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(g->qx, g->qx_out, bytes, hipMemcpyDeviceToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(g->qy, g->qy_out, bytes, hipMemcpyDeviceToDevice, st);
+    if (e != hipSuccess) return fail(HP_ELAUNCH, std::string("state copy: ") + hipGetErrorString(e));
+  }
+  return HP_OK;
 }
 
 int hp_grid_flux(const hp_grid_desc* g, void* stream) {

@@ -51,6 +51,7 @@ class HpGridDesc(ctypes.Structure):
         ("tile_rows", _I32), ("reserved_i32", _I32),
         ("h", _P), ("qx", _P), ("qy", _P), ("z", _P), ("h_max", _P),
         ("dh", _P), ("dqx", _P), ("dqy", _P),
+        ("h_out", _P), ("qx_out", _P), ("qy_out", _P),
         ("type_slot", _P), ("landuse", _P), ("slot_code", _P), ("manning", _P),
         ("given_depth", _P), ("given_wl", _P), ("given_q", _P),
         ("gd_rows", _I32), ("gd_w", _I32), ("gwl_rows", _I32), ("gwl_w", _I32), ("gq_rows", _I32), ("gq_w", _I32),
@@ -91,6 +92,7 @@ SIGNATURES = {
     "hp_grid_update": (_I, [ctypes.POINTER(HpGridDesc), _P]),
     "hp_grid_update_mode": (_I, [ctypes.POINTER(HpGridDesc), _I, _P]),
     "hp_grid_step": (_I, [ctypes.POINTER(HpGridDesc), _I32, _P]),
+    "hp_grid_step_fused": (_I, [ctypes.POINTER(HpGridDesc), _I32, _P]),
     "hp_ensemble_workspace_bytes": (_I64, [_I32, ctypes.POINTER(HpGridDesc)]),
     "hp_ensemble_init": (_I, [_I32, ctypes.POINTER(HpGridDesc), _P, _I64, ctypes.POINTER(HpEnsemble), _P]),
     "hp_ensemble_step": (_I, [ctypes.POINTER(HpEnsemble), _I32, _P]),

@@ -119,6 +119,7 @@ class GridSolver:
         self.dh = torch.zeros_like(self.h)
         self.dqx = torch.zeros_like(self.h)
         self.dqy = torch.zeros_like(self.h)
+        self.h_out = self.qx_out = self.qy_out = None  # allocated by step_fused()
         self.type_slot = torch.from_numpy(L.type_slot).to(dev)
         self.slot_code = torch.from_numpy(L.slot_code).to(dev)
         self.offsets = torch.from_numpy(L.offsets).to(dev)
@@ -247,6 +248,8 @@ class GridSolver:
         for f in ("h", "qx", "qy", "z", "h_max", "dh", "dqx", "dqy", "type_slot", "landuse", "slot_code",
                   "manning"):
             setattr(d, f, getattr(self, f).data_ptr())
+        for f in ("h_out", "qx_out", "qy_out"):
+            setattr(d, f, _lib.ptr(getattr(self, f)))
         d.given_depth, d.gd_rows, d.gd_w = self.given_depth.data_ptr(), *self.given_depth.shape
         d.given_wl, d.gwl_rows, d.gwl_w = self.given_wl.data_ptr(), *self.given_wl.shape
         d.given_q, d.gq_rows, d.gq_w = self.given_q.data_ptr(), *self.given_q.shape
@@ -290,6 +293,15 @@ class GridSolver:
 
     def step(self, nsteps: int = 1):
         check(lib().hp_grid_step(ctypes.byref(self.desc), int(nsteps), self._stream()), "hp_grid_step")
+
+    def step_fused(self, nsteps: int = 1):
+        """nsteps single-pass steps (same results as step(); dh/dqx/dqy untouched)."""
+        if self.h_out is None:
+            self.h_out = torch.zeros_like(self.h)
+            self.qx_out = torch.zeros_like(self.h)
+            self.qy_out = torch.zeros_like(self.h)
+            self._desc = None
+        check(lib().hp_grid_step_fused(ctypes.byref(self.desc), int(nsteps), self._stream()), "hp_grid_step_fused")
 
     # -- views -----------------------------------------------------------------
     def compact(self, plane: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:

@@ -139,6 +139,8 @@ typedef struct hp_grid_desc {
   void *h, *qx, *qy, *z, *h_max;
   /* flux accumulators written by the flux pass (dtype) */
   void *dh, *dqx, *dqy;
+  /* second state buffers of the single-pass step (dtype; may be null otherwise) */
+  void *h_out, *qx_out, *qy_out;
   /* cell class planes */
   const uint8_t* type_slot; /* 0 nodata, else slot of slot_code */
   const uint8_t* landuse;   /* Manning class, 255 for nodata */
@@ -183,6 +185,13 @@ enum hp_update_mode {
 
 /* The update pass in the given mode (hp_grid_update == HP_UPDATE_STEP). */
 int hp_grid_update_mode(const hp_grid_desc* g, int mode, void* stream);
+
+/* nsteps single-pass steps: flux, rain, friction, update, h_max and the CFL
+ * minimum of each tile in one kernel reading (h, qx, qy) and writing
+ * (h_out, qx_out, qy_out), then the time-control block; the two buffers swap
+ * every step and the result is left in (h, qx, qy).  Same results as
+ * hp_grid_flux + hp_grid_update (HP_UPDATE_STEP); dh/dqx/dqy are not written. */
+int hp_grid_step_fused(const hp_grid_desc* g, int32_t nsteps, void* stream);
 
 /* nsteps x (flux, update). */
 int hp_grid_step(const hp_grid_desc* g, int32_t nsteps, void* stream);
