@@ -45,6 +45,7 @@ __global__ __launch_bounds__(kCThreads) void k_flux_compact(
   const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= M) return;
   const T dx = dxp[0], t = tp[0], dt = dtp[0];
+  const T dt_dx = dt / dx;
   const int32_t i = wet[w];
   const int32_t code = index[i];
   const T zref = Prec<T>::local_datum ? z[i] : T(0);
@@ -67,7 +68,7 @@ __global__ __launch_bounds__(kCThreads) void k_flux_compact(
       const FrameCell<T> g = bc_ghost<T>(type, c, d, bd, bw, bqx, bqy);
       const T etaC = ORDER == 1 ? c.h + c.z : c_wl;
       const T etaR = ORDER == 1 ? g.h + g.z : ghost_wl<T>(type, c_wl, g, bw);
-      o = solve_edge<T, ORDER>(c, c, g, g, true, etaC, etaR, dx, dt);
+      o = solve_edge<T, ORDER>(c, c, g, g, true, etaC, etaR, dx, dt, dt_dx);
     } else {
       const int32_t rrid = index[(int64_t)(d + 1) * N + rid];
       FrameCell<T> r, l, rr;
@@ -85,7 +86,7 @@ __global__ __launch_bounds__(kCThreads) void k_flux_compact(
       }
       const T etaC = ORDER == 1 ? c.h + c.z : c_wl;
       const T etaR = ORDER == 1 ? r.h + r.z : wl[rid] - zref;
-      o = solve_edge<T, ORDER>(c, l, r, rr, false, etaC, etaR, dx, dt);
+      o = solve_edge<T, ORDER>(c, l, r, rr, false, etaC, etaR, dx, dt, dt_dx);
     }
     if (!o.active) continue;
     T gx, gy;

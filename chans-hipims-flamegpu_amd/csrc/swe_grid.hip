@@ -137,7 +137,7 @@ __device__ inline FrameCell<T> frame_cell(const FluxTile<T, TY>& s, int r, int c
 template <class T, int ORDER, int TY>
 __device__ inline void solve_tile_edge(const FluxTile<T, TY>& s, const hp_grid_desc& a, const SeriesRows& rows,
                                        int ra, int ca, int rb, int cb, int rl, int cl, int rr2, int cr2, int dAB,
-                                       T dx, T dt, T* out) {
+                                       T dx, T dt, T dt_dx, T* out) {
   out[0] = out[1] = out[2] = out[3] = out[4] = T(0);
   const int tA = s.ty[ra][ca], tB = s.ty[rb][cb];
   if (tA == 0 && tB == 0) return;
@@ -149,7 +149,7 @@ __device__ inline void solve_tile_edge(const FluxTile<T, TY>& s, const hp_grid_d
     const FrameCell<T> r = frame_cell(s, rb, cb, dAB, zref);
     const FrameCell<T> l = s.ty[rl][cl] ? frame_cell(s, rl, cl, dAB, zref) : c;
     const FrameCell<T> rr = s.ty[rr2][cr2] ? frame_cell(s, rr2, cr2, dAB, zref) : r;
-    const EdgeOut<T> o = solve_edge<T, ORDER>(c, l, r, rr, false, c.h + c.z, r.h + r.z, dx, dt);
+    const EdgeOut<T> o = solve_edge<T, ORDER>(c, l, r, rr, false, c.h + c.z, r.h + r.z, dx, dt, dt_dx);
     if (!o.active) return;
     T gx, gy;
     to_global(o.fn, o.ft, dAB, gx, gy);
@@ -168,7 +168,7 @@ __device__ inline void solve_tile_edge(const FluxTile<T, TY>& s, const hp_grid_d
     const BcValues<T> bv = bc_values<T>(a, from_a ? tA : tB, rows);
     const FrameCell<T> g = bc_ghost<T>(bv.type, c, d, bv.gd, bv.gwl - zref, bv.gqx, bv.gqy);
     const T etaG = ORDER == 1 ? g.h + g.z : ghost_wl<T>(bv.type, c.h + c.z, g, bv.gwl - zref);
-    const EdgeOut<T> o = solve_edge<T, ORDER>(c, c, g, g, true, c.h + c.z, etaG, dx, dt);
+    const EdgeOut<T> o = solve_edge<T, ORDER>(c, c, g, g, true, c.h + c.z, etaG, dx, dt, dt_dx);
     if (!o.active) return;
     T gx, gy;
     to_global(o.fn, o.ft, d, gx, gy);
@@ -317,6 +317,7 @@ __device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by, Partial*
   const T t = T(a.ctrl->t);
   const T dt = T(a.ctrl->dt);
   const T dx = T(a.dx);
+  const T dt_dx = dt / dx;
   SeriesRows rows;
   rows.d = a.given_depth ? series_row<T>((const T*)a.given_depth, a.gd_rows, a.gd_w, t) : 0;
   rows.w = a.given_wl ? series_row<T>((const T*)a.given_wl, a.gwl_rows, a.gwl_w, t) : 0;
@@ -341,7 +342,7 @@ __device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by, Partial*
       ra = ly + 1; rb = ly + 2; rl = ly; rr2 = ly + 3;
       d = 3;
     }
-    solve_tile_edge<T, ORDER, TY>(s, a, rows, ra, ca, rb, cb, rl, cl, rr2, cr2, d, dx, dt, s.e[e]);
+    solve_tile_edge<T, ORDER, TY>(s, a, rows, ra, ca, rb, cb, rl, cl, rr2, cr2, d, dx, dt, dt_dx, s.e[e]);
   }
   __syncthreads();
 

@@ -45,12 +45,16 @@ template <> struct Prec<double> {
   __host__ __device__ static inline double div(double a, double b) { return a / b; }
   __host__ __device__ static inline double sqrt_(double x) { return __builtin_sqrt(x); }
   __host__ __device__ static inline double rcbrt_friction(double h) { return pow(h, -1.0 / 3.0); }
+  // (x / dx) * dt, the reference's order of operations
+  __host__ __device__ static inline double scale(double x, double dx, double dt, double) { return (x / dx) * dt; }
 };
 template <> struct Prec<float> {
   static constexpr bool local_datum = true;
   __device__ static inline float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
   __device__ static inline float sqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
   __device__ static inline float rcbrt_friction(float h) { return rcbrtf(h); }
+  // x * (dt / dx) with the quotient computed once per block
+  __device__ static inline float scale(float x, float, float, float dt_dx) { return x * dt_dx; }
 };
 
 template <class T> __host__ __device__ inline T minmod(T x, T y) {
@@ -229,10 +233,12 @@ __host__ __device__ inline void hllc_edge(FrameCell<T>& eL, FrameCell<T>& eR, T 
 // unreconstructed cells (the reference rebuilds them from (h, z)); order 2
 // uses the stored cell water level wl (a ghost's wl for a boundary edge).
 // Frame values carry z relative to c's datum (c.z == 0 in float).
-// The returned terms are scaled by dt/dx as  (raw / dx) * dt.
+// The returned terms are scaled by dt/dx: (raw / dx) * dt in double, raw *
+// dt_dx in float (dt_dx = dt / dx, computed once by the caller).
 template <class T, int ORDER>
 __host__ __device__ inline EdgeOut<T> solve_edge(const FrameCell<T>& c, const FrameCell<T>& l, const FrameCell<T>& r,
-                                        const FrameCell<T>& rr, bool r_is_ghost, T etaC, T etaR, T dx, T dt) {
+                                        const FrameCell<T>& rr, bool r_is_ghost, T etaC, T etaR, T dx, T dt,
+                                        T dt_dx) {
   const T hs = T(kHs), g = T(kG);
   EdgeOut<T> o;
   o.fh = T(0); o.fn = T(0); o.ft = T(0); o.bed_c = T(0); o.bed_r = T(0);
@@ -274,11 +280,12 @@ __host__ __device__ inline EdgeOut<T> solve_edge(const FrameCell<T>& c, const Fr
   }
   T fh, fn, ft;
   hllc_edge<T, ORDER>(eL, eR, etaC, etaR, fh, fn, ft);
-  o.fh = (fh / dx) * dt;
-  o.fn = (fn / dx) * dt;
-  o.ft = (ft / dx) * dt;
-  o.bed_c = ((T(0.5) * g * (eL.h + c.h) * (eL.z - c.z)) / dx) * dt;
-  o.bed_r = ((T(0.5) * g * (eR.h + r.h) * (eR.z - r.z)) / dx) * dt;
+  using P = Prec<T>;
+  o.fh = P::scale(fh, dx, dt, dt_dx);
+  o.fn = P::scale(fn, dx, dt, dt_dx);
+  o.ft = P::scale(ft, dx, dt, dt_dx);
+  o.bed_c = P::scale(T(0.5) * g * (eL.h + c.h) * (eL.z - c.z), dx, dt, dt_dx);
+  o.bed_r = P::scale(T(0.5) * g * (eR.h + r.h) * (eR.z - r.z), dx, dt, dt_dx);
   return o;
 }
 

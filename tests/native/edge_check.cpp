@@ -22,8 +22,8 @@ extern "C" {
 // Interior edge: water levels are the cells' h + z.  out: fh, fn, ft, bed_c, bed_r.
 int ec_edge(int order, const double* cells, double dx, double dt, double* out) {
   const FrameCell<double> l = cell(cells), c = cell(cells + 4), r = cell(cells + 8), rr = cell(cells + 12);
-  EdgeOut<double> o = order == 1 ? solve_edge<double, 1>(c, l, r, rr, false, c.h + c.z, r.h + r.z, dx, dt)
-                                  : solve_edge<double, 2>(c, l, r, rr, false, c.h + c.z, r.h + r.z, dx, dt);
+  EdgeOut<double> o = order == 1 ? solve_edge<double, 1>(c, l, r, rr, false, c.h + c.z, r.h + r.z, dx, dt, dt / dx)
+                                  : solve_edge<double, 2>(c, l, r, rr, false, c.h + c.z, r.h + r.z, dx, dt, dt / dx);
   out[0] = o.fh; out[1] = o.fn; out[2] = o.ft; out[3] = o.bed_c; out[4] = o.bed_r;
   return o.active ? 1 : 0;
 }
@@ -51,8 +51,8 @@ int ec_ghost_edge(int order, int type, int d, const double* c4, const double* gi
   const FrameCell<double> c = cell(c4);
   const FrameCell<double> g = bc_ghost<double>(type, c, d, given[0], given[1], given[2], given[3]);
   const double etaG = order == 1 ? g.h + g.z : ghost_wl<double>(type, c.h + c.z, g, given[1]);
-  EdgeOut<double> o = order == 1 ? solve_edge<double, 1>(c, c, g, g, true, c.h + c.z, etaG, dx, dt)
-                                  : solve_edge<double, 2>(c, c, g, g, true, c.h + c.z, etaG, dx, dt);
+  EdgeOut<double> o = order == 1 ? solve_edge<double, 1>(c, c, g, g, true, c.h + c.z, etaG, dx, dt, dt / dx)
+                                  : solve_edge<double, 2>(c, c, g, g, true, c.h + c.z, etaG, dx, dt, dt / dx);
   out[0] = o.fh; out[1] = o.fn; out[2] = o.ft; out[3] = o.bed_c; out[4] = o.bed_r;
   return o.active ? 1 : 0;
 }

@@ -287,14 +287,19 @@ def test_godunov_dropin_server_sequence(oracle_lib, cuda, tmp_path):
 
 
 def test_fp32_error_against_fp64_oracle(oracle_lib, cuda):
-    """FP32 structured path vs the FP64 oracle over a rainfall-runoff run:
-    max depth / discharge error and volume-balance error (stated, bounded)."""
+    """FP32 structured path vs the FP64 oracle over a rainfall-runoff run,
+    both driven with the oracle's time-step schedule so the states are
+    compared at the same simulated times: max depth / discharge error and
+    volume error are stated and bounded."""
     say("[fp32 vs fp64 oracle]")
     c = gen.rainfall_runoff(128, 5.0, rain_mmh=100.0)
     g, o = make_pair(oracle_lib, c, 2, dtype=torch.float32, rain=c.rain_table, station=np.zeros_like(c.mask))
     steps = 300
-    g.step(steps)
-    o.run(steps)
+    for _ in range(steps):
+        g.ctrl[0] = o.time
+        g.ctrl[1] = o.dt
+        g.step(1)
+        o.run(1)
     torch.cuda.synchronize()
     h32 = g.compact(g.h).double().cpu().numpy()
     q32 = np.hypot(g.compact(g.qx).double().cpu().numpy(), g.compact(g.qy).double().cpu().numpy())
@@ -303,15 +308,15 @@ def test_fp32_error_against_fp64_oracle(oracle_lib, cuda):
     dq = np.abs(q32 - q64).max()
     ctrl = g.ctrl_dict()
     area = c.dx * c.dx
-    v_gpu = ctrl["volume"]
-    balance = (v_gpu - (ctrl["rain_volume"] + ctrl["flux_volume"])) / ctrl["rain_volume"]
-    v_orc = o.h_c.sum() * area
-    say(f"  t32 {g.t:.1f} t64 {o.time:.1f}; max|dh| {dh:.3e} m (max h {o.h_c.max():.3f}), max|dq| {dq:.3e} m2/s, "
-        f"volume fp32 {v_gpu:.6e} fp64 {v_orc:.6e} (rel {abs(v_gpu - v_orc) / v_orc:.2e}), balance err {balance:.2e}")
-    assert abs(g.t - o.time) / o.time < 1e-3
-    assert dh < 5e-3 and dq < 5e-3
-    assert abs(balance) < 1e-4
-    assert abs(v_gpu - v_orc) / v_orc < 1e-3
+    v32 = h32.sum() * area
+    v64 = o.h_c.sum() * area
+    balance = (ctrl["volume"] - (ctrl["rain_volume"] + ctrl["flux_volume"])) / ctrl["rain_volume"]
+    say(f"  t {o.time:.1f} s; max|dh| {dh:.3e} m (max h {o.h_c.max():.3f} m), max|dq| {dq:.3e} m2/s "
+        f"(max q {q64.max():.3f}), volume fp32 {v32:.6e} fp64 {v64:.6e} (rel {abs(v32 - v64) / v64:.2e}), "
+        f"fp32 balance err {balance:.2e}")
+    assert dh < 2e-3 and dq < 2e-2
+    assert abs(v32 - v64) / v64 < 1e-4
+    assert abs(balance) < 1e-6
 
 
 def test_full_size_fp32_properties(cuda):
