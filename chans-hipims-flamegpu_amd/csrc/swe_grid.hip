@@ -131,38 +131,54 @@ __device__ inline FrameCell<T> frame_cell(const FluxTile<T, TY>& s, int r, int c
   return f;
 }
 
-// Solve the edge between LDS cells A=(ra,ca) and B=(rb,cb), where B lies in
-// direction dAB from A (0 E for x-edges, 3 S for y-edges).  (rl,cl) is A's
-// neighbour opposite to B, (rr2,cr2) is B's neighbour beyond B.
-template <class T, int ORDER, int TY>
+// Frame view for a compile-time direction D (0 E for x-edges, 3 S for
+// y-edges): the rotation is a fixed renaming, no selects.
+template <int D, class T, int TY>
+__device__ inline FrameCell<T> frame_cell_d(const FluxTile<T, TY>& s, int r, int c, T zref) {
+  FrameCell<T> f;
+  const T qx = s.qx[r][c], qy = s.qy[r][c];
+  f.h = s.h[r][c];
+  f.qn = D == 0 ? qx : -qy;
+  f.qt = D == 0 ? qy : qx;
+  f.z = s.z[r][c] - zref;
+  return f;
+}
+
+// Solve the edge between LDS cells A and B, where B lies in direction D from
+// A (D = 0: x-edge, B east of A; D = 3: y-edge, B south of A).  A is
+// (ra, ca); L is A's neighbour opposite to B, RR is B's neighbour beyond B.
+// Writes the edge record out[0..4].
+template <class T, int ORDER, int TY, int D>
 __device__ inline void solve_tile_edge(const FluxTile<T, TY>& s, const hp_grid_desc& a, const SeriesRows& rows,
-                                       int ra, int ca, int rb, int cb, int rl, int cl, int rr2, int cr2, int dAB,
-                                       T dx, T dt, T dt_dx, T* out) {
+                                       int ra, int ca, T dx, T dt, T dt_dx, T* out) {
+  constexpr int di = D == 0 ? 0 : 1, dj = D == 0 ? 1 : 0;  // B - A in (row, col)
+  const int rb = ra + di, cb = ca + dj;
+  const int rl = ra - di, cl = ca - dj;
+  const int rr2 = rb + di, cr2 = cb + dj;
   out[0] = out[1] = out[2] = out[3] = out[4] = T(0);
   const int tA = s.ty[ra][ca], tB = s.ty[rb][cb];
   if (tA == 0 && tB == 0) return;
-  const int dBA = (dAB + 2) & 3;
-  const bool axis_x = (dAB == 0 || dAB == 2);
+  constexpr T nA = T(D == 0 ? 1 : -1);  // A's outward normal component along the edge axis
   if (tA != 0 && tB != 0) {
     const T zref = Prec<T>::local_datum ? s.z[ra][ca] : T(0);
-    const FrameCell<T> c = frame_cell(s, ra, ca, dAB, zref);
-    const FrameCell<T> r = frame_cell(s, rb, cb, dAB, zref);
-    const FrameCell<T> l = s.ty[rl][cl] ? frame_cell(s, rl, cl, dAB, zref) : c;
-    const FrameCell<T> rr = s.ty[rr2][cr2] ? frame_cell(s, rr2, cr2, dAB, zref) : r;
+    const FrameCell<T> c = frame_cell_d<D>(s, ra, ca, zref);
+    const FrameCell<T> r = frame_cell_d<D>(s, rb, cb, zref);
+    const FrameCell<T> lc = frame_cell_d<D>(s, rl, cl, zref);
+    const FrameCell<T> rc = frame_cell_d<D>(s, rr2, cr2, zref);
+    const FrameCell<T> l = s.ty[rl][cl] ? lc : c;
+    const FrameCell<T> rr = s.ty[rr2][cr2] ? rc : r;
     const EdgeOut<T> o = solve_edge<T, ORDER>(c, l, r, rr, false, c.h + c.z, r.h + r.z, dx, dt, dt_dx);
     if (!o.active) return;
-    T gx, gy;
-    to_global(o.fn, o.ft, dAB, gx, gy);
     out[0] = o.fh;
-    out[1] = gx;
-    out[2] = gy;
-    const T nA = T(axis_x ? nrm_x(dAB) : nrm_y(dAB));
+    out[1] = D == 0 ? o.fn : o.ft;
+    out[2] = D == 0 ? o.ft : -o.fn;
     out[3] = o.bed_c * nA;
     out[4] = o.bed_r * (-nA);
   } else {
     // boundary edge, solved from the valid side with its outward normal
     const bool from_a = tA != 0;
-    const int rc = from_a ? ra : rb, cc = from_a ? ca : cb, d = from_a ? dAB : dBA;
+    const int rc = from_a ? ra : rb, cc = from_a ? ca : cb, d = from_a ? D : (D + 2) & 3;
+    const bool axis_x = D == 0;
     const T zref = Prec<T>::local_datum ? s.z[rc][cc] : T(0);
     const FrameCell<T> c = frame_cell(s, rc, cc, d, zref);
     const BcValues<T> bv = bc_values<T>(a, from_a ? tA : tB, rows);
@@ -325,24 +341,25 @@ __device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by, Partial*
   const int n_st = FUSED ? station_rates(a, a.ctrl->t, s_rain, kMaxStations) : 0;
   __syncthreads();
 
-  // every x-edge (row ly, between local columns lx-1 = A and lx = B) and
-  // y-edge (column lx, between local rows ly-1 = A and ly = B) in one flat
-  // loop so the 4 waves share the work evenly
-  for (int e = tid; e < Tile::NX + Tile::NY; e += kThreads) {
-    int ra, ca, rb, cb, rl, cl, rr2, cr2, d;
-    if (e < Tile::NX) {
-      const int ly = e / (kTX + 1), lx = e - ly * (kTX + 1);
-      ra = rb = rl = rr2 = ly + 2;
-      ca = lx + 1; cb = lx + 2; cl = lx; cr2 = lx + 3;
-      d = 0;
-    } else {
-      const int f = e - Tile::NX;
-      const int ly = f / kTX, lx = f - ly * kTX;
-      ca = cb = cl = cr2 = lx + 2;
-      ra = ly + 1; rb = ly + 2; rl = ly; rr2 = ly + 3;
-      d = 3;
-    }
-    solve_tile_edge<T, ORDER, TY>(s, a, rows, ra, ca, rb, cb, rl, cl, rr2, cr2, d, dx, dt, dt_dx, s.e[e]);
+  // Edges touching the tile: x-edges (row ly, between local columns lx-1 = A
+  // and lx = B, lx = 0..64) and y-edges (column lx, between local rows ly-1 =
+  // A and ly = B, ly = 0..TY).  The first TY x 64 of each kind fill whole
+  // passes of the block; the last x-edge column runs in wave 0 and the last
+  // y-edge row in wave 1, so every wave solves edges of one direction only.
+  for (int e = tid; e < TY * kTX; e += kThreads) {
+    const int ly = e >> 6, lx = e & (kTX - 1);
+    solve_tile_edge<T, ORDER, TY, 0>(s, a, rows, ly + 2, lx + 1, dx, dt, dt_dx, s.e[ly * (kTX + 1) + lx]);
+  }
+  for (int e = tid; e < TY * kTX; e += kThreads) {
+    const int ly = e >> 6, lx = e & (kTX - 1);
+    solve_tile_edge<T, ORDER, TY, 3>(s, a, rows, ly + 1, lx + 2, dx, dt, dt_dx, s.e[Tile::NX + ly * kTX + lx]);
+  }
+  static_assert(TY <= kTX && kThreads >= 2 * kTX, "edge tail layout");
+  if (tid < TY) {
+    solve_tile_edge<T, ORDER, TY, 0>(s, a, rows, tid + 2, kTX + 1, dx, dt, dt_dx, s.e[tid * (kTX + 1) + kTX]);
+  } else if (tid >= kTX && tid < 2 * kTX) {
+    const int lx = tid - kTX;
+    solve_tile_edge<T, ORDER, TY, 3>(s, a, rows, TY + 1, lx + 2, dx, dt, dt_dx, s.e[Tile::NX + TY * kTX + lx]);
   }
   __syncthreads();
 
@@ -396,15 +413,27 @@ __device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by, Partial*
   if (FUSED) block_partial<T>(mn, vol, fsum, rsum, part);
 }
 
+// Workgroups of a launch are handed to the 8 XCDs round-robin (block b runs
+// on XCD b % 8), and each XCD has its own L2.  Tile numbers are reassigned so
+// that every XCD walks one contiguous band of tiles: x- and y-neighbours then
+// share their halo lines in one L2 instead of fetching them from HBM twice.
+constexpr int kXcds = 8;
+__device__ inline int xcd_band(int b, int n) {
+  const int q = n / kXcds, rem = n - q * kXcds;
+  const int x = b % kXcds, local = b / kXcds;
+  return x * q + min(x, rem) + local;
+}
+
 template <class T, int ORDER, int TY>
-__global__ __launch_bounds__(kThreads) void k_flux(hp_grid_desc a) {
-  flux_tile<T, ORDER, TY, false>(a, blockIdx.x, blockIdx.y, nullptr);
+__global__ __launch_bounds__(kThreads) void k_flux(hp_grid_desc a, int ntx) {
+  const int tile = xcd_band(blockIdx.x, gridDim.x);
+  flux_tile<T, ORDER, TY, false>(a, tile % ntx, tile / ntx, nullptr);
 }
 
 template <class T, int ORDER, int TY>
 __global__ __launch_bounds__(kThreads) void k_flux_multi(const hp_grid_desc* __restrict__ doms,
                                                          const int2* __restrict__ map) {
-  const int2 m = map[blockIdx.x];
+  const int2 m = map[xcd_band(blockIdx.x, gridDim.x)];
   const hp_grid_desc& a = doms[m.x];
   const int ntx = (a.nx + kTX - 1) / kTX;
   flux_tile<T, ORDER, TY, false>(a, m.y % ntx, m.y / ntx, nullptr);
@@ -412,8 +441,9 @@ __global__ __launch_bounds__(kThreads) void k_flux_multi(const hp_grid_desc* __r
 
 // Single-pass step: reads (h, qx, qy), writes (h_out, qx_out, qy_out).
 template <class T, int ORDER, int TY>
-__global__ __launch_bounds__(kThreads) void k_step_fused(hp_grid_desc a) {
-  flux_tile<T, ORDER, TY, true>(a, blockIdx.x, blockIdx.y, (Partial*)a.work + blockIdx.x + blockIdx.y * gridDim.x);
+__global__ __launch_bounds__(kThreads) void k_step_fused(hp_grid_desc a, int ntx) {
+  const int tile = xcd_band(blockIdx.x, gridDim.x);
+  flux_tile<T, ORDER, TY, true>(a, tile % ntx, tile / ntx, (Partial*)a.work + tile);
 }
 
 // MODE selects which of the reference driver's update methods the pass is:
@@ -684,8 +714,8 @@ static int check_desc(const hp_grid_desc* g) {
 
 template <class T, int ORDER, int TY>
 static int launch_flux_single(const hp_grid_desc* g, hipStream_t st) {
-  dim3 grid((g->nx + kTX - 1) / kTX, (g->ny + TY - 1) / TY);
-  hipLaunchKernelGGL((k_flux<T, ORDER, TY>), grid, dim3(kThreads), 0, st, *g);
+  const int ntx = (g->nx + kTX - 1) / kTX, nty = (g->ny + TY - 1) / TY;
+  hipLaunchKernelGGL((k_flux<T, ORDER, TY>), dim3(ntx * nty), dim3(kThreads), 0, st, *g, ntx);
   return check_launch("k_flux");
 }
 
@@ -703,8 +733,8 @@ static int launch_flux(const hp_grid_desc* g, hipStream_t st) {
 
 template <class T, int ORDER, int TY>
 static int launch_fused_t(const hp_grid_desc* g, hipStream_t st) {
-  dim3 grid((g->nx + kTX - 1) / kTX, (g->ny + TY - 1) / TY);
-  hipLaunchKernelGGL((k_step_fused<T, ORDER, TY>), grid, dim3(kThreads), 0, st, *g);
+  const int ntx = (g->nx + kTX - 1) / kTX, nty = (g->ny + TY - 1) / TY;
+  hipLaunchKernelGGL((k_step_fused<T, ORDER, TY>), dim3(ntx * nty), dim3(kThreads), 0, st, *g, ntx);
   return check_launch("k_step_fused");
 }
 

@@ -58,8 +58,8 @@ template <> struct Prec<float> {
 };
 
 template <class T> __host__ __device__ inline T minmod(T x, T y) {
-  if (x * y <= T(0)) return T(0);
-  return fabs(x) < fabs(y) ? x : y;
+  const T m = fabs(x) < fabs(y) ? x : y;
+  return x * y <= T(0) ? T(0) : m;
 }
 
 // A cell seen from an edge frame.
@@ -163,6 +163,10 @@ template <class T> struct EdgeOut {
 // On return eL.h/eL.z and eR.h/eR.z hold the reconstructed depth and bed.
 template <class T, int ORDER>
 __host__ __device__ inline void hllc_edge(FrameCell<T>& eL, FrameCell<T>& eR, T etaC, T etaR, T& fh, T& fn, T& ft) {
+  // Written as straight-line code with selects: a wave whose edges fall in
+  // different wave-speed regions evaluates one path instead of all three
+  // under divergent branches.  The selected values are the same expressions
+  // the branches would produce.
   using P = Prec<T>;
   const T hs = T(kHs), g = T(kG);
   const T uL0 = eL.h < hs ? T(0) : P::div(eL.qn, eL.h);
@@ -177,23 +181,21 @@ __host__ __device__ inline void hllc_edge(FrameCell<T>& eL, FrameCell<T>& eR, T 
   wL = fmax(fmin(wL, emax), emin);
   wR = fmax(fmin(wR, emax), emin);
   if (ORDER == 1) {
-    if ((etaC - etaR) * (wL - wR) < T(0)) {
-      T m = (wL + wR) / T(2);
-      wL = m;
-      wR = m;
-    }
+    const T m = (wL + wR) / T(2);
+    const bool avg = (etaC - etaR) * (wL - wR) < T(0);
+    wL = avg ? m : wL;
+    wR = avg ? m : wR;
   } else {
-    if (etaC > etaR) wL = fmax(wL, wR);
-    else wR = fmax(wL, wR);
+    const T w = fmax(wL, wR);
+    const bool left_high = etaC > etaR;
+    wL = left_high ? w : wL;
+    wR = left_high ? wR : w;
   }
   const T zbm = fmax(wL - eL.h, wR - eR.h);
   eL.h = fmax(wL - zbm, T(0));
   eR.h = fmax(wR - zbm, T(0));
   eL.z = wL - eL.h;
   eR.z = wR - eR.h;
-
-  fh = T(0); fn = T(0); ft = T(0);
-  if (eL.h < hs && eR.h < hs) return;
 
   const T hL = eL.h, hR = eR.h;
   const T qnL = uL0 * hL, qtL = vL0 * hL;
@@ -207,21 +209,18 @@ __host__ __device__ inline void hllc_edge(FrameCell<T>& eL, FrameCell<T>& eR, T 
   const T sL = hL <= hs ? uR0 - T(2) * aR : fmin(uL0 - aL, u_star - a_star);
   const T sR = hR <= hs ? uL0 + T(2) * aL : fmax(uR0 + aR, u_star + a_star);
   const T sM = P::div(sL * hR * (uR0 - sR) - sR * hL * (uL0 - sL), hR * (uR0 - sR) - hL * (uL0 - sL));
-  if (T(0) <= sL) {
-    fh = qnL;
-    fn = uL0 * qnL + T(0.5) * g * hL * hL;
-    ft = uL0 * qtL;
-  } else if ((sL < T(0) && T(0) <= sM) || (sM < T(0) && T(0) <= sR)) {
-    const T qnfL = uL0 * qnL + T(0.5) * g * hL * hL;
-    const T qnfR = uR0 * qnR + T(0.5) * g * hR * hR;
-    fh = P::div(sR * qnL - sL * qnR + sL * sR * (hR - hL), sR - sL);
-    fn = P::div(sR * qnfL - sL * qnfR + sL * sR * (qnR - qnL), sR - sL);
-    ft = fh * ((sL < T(0) && T(0) <= sM) ? vL0 : vR0);
-  } else {
-    fh = qnR;
-    fn = uR0 * qnR + T(0.5) * g * hR * hR;
-    ft = uR0 * qtR;
-  }
+  const T qnfL = uL0 * qnL + T(0.5) * g * hL * hL;
+  const T qnfR = uR0 * qnR + T(0.5) * g * hR * hR;
+  const T fh_s = P::div(sR * qnL - sL * qnR + sL * sR * (hR - hL), sR - sL);
+  const T fn_s = P::div(sR * qnfL - sL * qnfR + sL * sR * (qnR - qnL), sR - sL);
+  const bool star_l = sL < T(0) && T(0) <= sM;
+  const T ft_s = fh_s * (star_l ? vL0 : vR0);
+  const bool left = T(0) <= sL;
+  const bool star = star_l || (sM < T(0) && T(0) <= sR);
+  const bool dry = hL < hs && hR < hs;
+  fh = dry ? T(0) : (left ? qnL : (star ? fh_s : qnR));
+  fn = dry ? T(0) : (left ? qnfL : (star ? fn_s : qnfR));
+  ft = dry ? T(0) : (left ? uL0 * qtL : (star ? ft_s : uR0 * qtR));
 }
 
 // Solve one edge from cell c towards r (direction d).
@@ -250,10 +249,10 @@ __host__ __device__ inline EdgeOut<T> solve_edge(const FrameCell<T>& c, const Fr
     eL = c;
     eR = r;
     if (!r_is_ghost) {
-      T rb = minmod<T>(c.z - l.z, r.z - c.z);
-      if (r.h > hs) eL.z += T(0.5) * rb;
-      rb = minmod<T>(r.z - rr.z, c.z - r.z);
-      if (c.h > hs) eR.z += T(0.5) * rb;
+      const T zl = eL.z + T(0.5) * minmod<T>(c.z - l.z, r.z - c.z);
+      const T zr = eR.z + T(0.5) * minmod<T>(r.z - rr.z, c.z - r.z);
+      eL.z = r.h > hs ? zl : eL.z;
+      eR.z = c.h > hs ? zr : eR.z;
     }
   } else {
     if (r_is_ghost) {
@@ -264,18 +263,14 @@ __host__ __device__ inline EdgeOut<T> solve_edge(const FrameCell<T>& c, const Fr
       eL.z = c.z + T(0.5) * minmod<T>(c.z - l.z, r.z - c.z);
       eR.h = r.h + T(0.5) * minmod<T>(r.h - rr.h, c.h - r.h);
       eR.z = r.z + T(0.5) * minmod<T>(r.z - rr.z, c.z - r.z);
-      if (c.h < hs) {
-        eL.qn = T(0); eL.qt = T(0);
-      } else {
-        eL.qn = c.qn + T(0.5) * minmod<T>(c.qn - l.qn, r.qn - c.qn);
-        eL.qt = c.qt + T(0.5) * minmod<T>(c.qt - l.qt, r.qt - c.qt);
-      }
-      if (r.h < hs) {
-        eR.qn = T(0); eR.qt = T(0);
-      } else {
-        eR.qn = r.qn + T(0.5) * minmod<T>(r.qn - rr.qn, c.qn - r.qn);
-        eR.qt = r.qt + T(0.5) * minmod<T>(r.qt - rr.qt, c.qt - r.qt);
-      }
+      const T qnL = c.qn + T(0.5) * minmod<T>(c.qn - l.qn, r.qn - c.qn);
+      const T qtL = c.qt + T(0.5) * minmod<T>(c.qt - l.qt, r.qt - c.qt);
+      const T qnR = r.qn + T(0.5) * minmod<T>(r.qn - rr.qn, c.qn - r.qn);
+      const T qtR = r.qt + T(0.5) * minmod<T>(r.qt - rr.qt, c.qt - r.qt);
+      eL.qn = c.h < hs ? T(0) : qnL;
+      eL.qt = c.h < hs ? T(0) : qtL;
+      eR.qn = r.h < hs ? T(0) : qnR;
+      eR.qt = r.h < hs ? T(0) : qtR;
     }
   }
   T fh, fn, ft;
