@@ -42,6 +42,19 @@ def alg_bytes(esize: int, rain_field: bool) -> dict:
     return {"k_flux": flux, "k_update": update, "k_step_fused": fused}
 
 
+def profiled_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` on `workload` as recorded from the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs (copy-calibrated) in profiles/, or
+    None when no such summary exists."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("| measured traffic per launch |"):
+            return float(line.split("|")[2].split()[0]) * 1e6
+    return None
+
+
 def time_two_pass(g, steps):
     """Two-pass step (flux kernel; update + finalize), HIP events between the
     kernels on the launch stream."""
@@ -98,9 +111,19 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     p.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, available cores)")
     p.add_argument("--no-cpu", action="store_true")
-    p.add_argument("--mode", choices=["step", "copy"], default="step",
-                   help="copy: a plain device copy of known size, the reference point for HBM byte counters")
+    p.add_argument("--mode", choices=["step", "copy", "ensemble", "adjoint"], default="step",
+                   help="copy: a plain device copy of known size, the reference point for HBM byte counters; "
+                        "ensemble: 64 uneven scenarios, graph-replayed ensemble vs the same run one by one "
+                        "through the Godunov driver; adjoint: gradient cost against a plain forward run")
     p.add_argument("--copy-mib", type=int, default=1024)
+    p.add_argument("--scenarios", type=int, default=64)
+    p.add_argument("--sim-seconds", type=float, default=900.0, help="ensemble: simulated time of every scenario")
+    p.add_argument("--replay-steps", type=int, default=50, help="ensemble: steps per graph replay")
+    p.add_argument("--adj-size", type=int, default=1024)
+    p.add_argument("--adj-steps", type=int, default=2000)
+    p.add_argument("--adj-checkpoint", type=int, default=100)
+    p.add_argument("--adj-gauges", type=int, default=32)
+    p.add_argument("--no-fd", action="store_true", help="adjoint: skip the finite-difference check")
     return p.parse_args()
 
 
@@ -126,6 +149,182 @@ def copy_calibration(a):
                       "ms_per_copy": ms, "GBps_read_plus_write": 2 * n * 4 / (ms / 1e3) / 1e9}), flush=True)
 
 
+def _ensemble_case(spec):
+    from hipims_amd import generators as gen
+    return gen.ensemble_case(*spec)
+
+
+def ensemble_mode(a):
+    """Config: 64 scenarios, sizes drawn from {512, 768, 1024, 1536, 2048}^2,
+    own DEM / landcover per scenario (seed 2000 + k), triangular hyetograph,
+    each run to the same simulated time with its own CFL dt; FP32, order 2.
+    Ensemble: every scenario advanced by one launch set per step, K steps per
+    graph replay.  Baseline: the same scenarios one after another through the
+    Godunov driver's per-step calls (addFlux, addStation_PrecipitationSource,
+    time_friction_euler_update_cuda, a host read of t each step)."""
+    import concurrent.futures as cf
+
+    import numpy as np
+    import torch
+
+    from hipims_amd import generators as gen
+    from hipims_amd.ensemble import Ensemble
+    from hipims_amd.grid import GridSolver
+    from hipims_amd.SWE_CUDA import Godunov
+
+    # generate on host worker processes before this process touches the GPU
+    specs = gen.ensemble_specs(a.scenarios, 2000)
+    t0 = time.perf_counter()
+    workers = max(1, min(16, len(os.sched_getaffinity(0))))
+    with cf.ProcessPoolExecutor(workers) as ex:
+        cases = list(ex.map(_ensemble_case, specs))
+    gen_s = time.perf_counter() - t0
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    print(f"[ensemble] generated {len(cases)} scenarios in {gen_s:.1f} s", file=sys.stderr, flush=True)
+
+    def solver(c):
+        s = GridSolver(c.mask, c.h, c.qx, c.qy, c.z, c.dx, cfl=c.cfl, order=2, dtype=torch.float32, device=dev,
+                       first_dt=c.first_dt, manning=c.manning, landuse=c.landuse)
+        s.set_station_rain(c.rain_table)
+        return s
+
+    ens = Ensemble([solver(c) for c in cases], t_stop=a.sim_seconds)
+    ens.capture(a.replay_steps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    last = [t0]
+
+    def progress(k, n_act):
+        now = time.perf_counter()
+        if now - last[0] > 10.0:
+            last[0] = now
+            print(f"[ensemble] {k} replays, {n_act} scenarios active, {now - t0:.1f} s", file=sys.stderr,
+                  flush=True)
+
+    replays = ens.run(a.replay_steps, progress=progress)
+    torch.cuda.synchronize()
+    ens_s = time.perf_counter() - t0
+    steps = [int(s.ctrl_dict()["steps"]) for s in ens.solvers]
+    cells = [s.layout.n_valid for s in ens.solvers]
+    work = float(sum(n * k for n, k in zip(cells, steps)))
+    print(f"[ensemble] {replays} replays, {ens_s:.2f} s; steps per scenario {min(steps)}..{max(steps)}",
+          file=sys.stderr, flush=True)
+    del ens
+    torch.cuda.empty_cache()
+
+    seq_s = 0.0
+    seq_steps = []
+    T = lambda x: torch.as_tensor(np.asarray(x))
+    for k, c in enumerate(cases):
+        num = Godunov(dev, c.dx, c.cfl, 1.0e30, t=0.0, export_n=0, firstTimeStep=c.first_dt, secondOrder=True,
+                      tensorType=torch.float32)
+        num.init__fluidField_tensor(T(c.mask), T(c.h), T(c.qx), T(c.qy), T(c.h + c.z), T(c.z), dev)
+        num.set__frictionField_tensor(c.manning, dev)
+        num.set_landuse(T(c.mask), T(c.landuse), dev)
+        num.set_distributed_rainfall_station_Mask(T(c.mask), T(np.zeros_like(c.mask)), dev)
+        num.set_boundary_tensor({}, dev)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        n = 0
+        while num.t.item() < a.sim_seconds:
+            num.addFlux()
+            num.addStation_PrecipitationSource(c.rain_table, dev)
+            num.time_friction_euler_update_cuda(dev)
+            n += 1
+        torch.cuda.synchronize()
+        seq_s += time.perf_counter() - t0
+        seq_steps.append(n)
+        del num
+        print(f"[ensemble] sequential baseline: scenario {k + 1} ({c.mask.shape[0] - 2}^2, {n} steps), "
+              f"{seq_s:.1f} s so far", file=sys.stderr, flush=True)
+    line = {"mode": "ensemble", "scenarios": len(cases), "sizes": [c.mask.shape[0] - 2 for c in cases],
+            "sim_seconds": a.sim_seconds, "dtype": "f32", "order": 2, "steps_per_replay": a.replay_steps,
+            "cell_updates": work, "ensemble_s": ens_s, "sequential_s": seq_s,
+            "ensemble_cell_updates_per_s": work / ens_s, "sequential_cell_updates_per_s": work / seq_s,
+            "speedup": seq_s / ens_s, "steps_min": min(steps), "steps_max": max(steps),
+            "same_step_counts": steps == seq_steps, "generation_s": gen_s}
+    print(json.dumps(line), flush=True)
+
+
+def adjoint_mode(a):
+    """Config: 1024^2, dx = 5 m, FP32 forward and adjoint, FP64 gradient
+    sums, 2000 steps, checkpoint every 100; 32 gauges among cells wet at the
+    end of a pilot run; observations from a run with every class n scaled by
+    0.8 or 1.2 (seeded); gradients w.r.t. the 6 class n and the rain
+    multiplier.  Cost ratio = (forward with checkpoints + reverse with
+    recomputation) / plain forward of the same steps; the gradient is checked
+    against central differences (eps = 1e-3 relative) on the frozen schedule."""
+    import numpy as np
+    import torch
+
+    from hipims_amd import generators as gen
+    from hipims_amd.adjoint import Calibration
+    from hipims_amd.grid import GridSolver
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    c = gen.rainfall_runoff(a.adj_size, 5.0)
+
+    def solver():
+        s = GridSolver(c.mask, c.h, c.qx, c.qy, c.z, c.dx, cfl=c.cfl, order=2, dtype=torch.float32, device=dev,
+                       first_dt=c.first_dt, manning=c.manning, landuse=c.landuse)
+        s.set_station_rain(c.rain_table)
+        return s
+
+    pilot = solver()
+    pilot.step(a.adj_steps)
+    h = pilot.raster(pilot.h)
+    rng = np.random.Generator(np.random.PCG64(1004))
+    wet = np.argwhere((h > 1e-3) & (c.mask > 0))
+    gauges = wet[rng.choice(len(wet), size=min(a.adj_gauges, len(wet)), replace=False)]
+    del pilot
+
+    # plain forward: the model's own CFL steps, one call
+    plain = solver()
+    plain.step(5)
+    torch.cuda.synchronize()
+    plain = solver()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plain.step(a.adj_steps)
+    torch.cuda.synchronize()
+    fwd_s = time.perf_counter() - t0
+
+    cal = Calibration(solver(), gauges, a.adj_steps, a.adj_checkpoint)
+    th0 = cal.theta0()
+    pert = torch.tensor(np.random.Generator(np.random.PCG64(7)).choice([0.8, 1.2], size=th0.numel() - 1))
+    th_true = th0.clone()
+    th_true[:-1] *= pert
+    cal.obs = cal.simulate(th_true)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    loss = cal.run(th0)
+    torch.cuda.synchronize()
+    run_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    grad = cal.gradient()
+    torch.cuda.synchronize()
+    rev_s = time.perf_counter() - t0
+    line = {"mode": "adjoint", "size": a.adj_size, "steps": a.adj_steps, "checkpoint": a.adj_checkpoint,
+            "gauges": int(len(gauges)), "dtype": "f32 state, f64 gradient sums", "loss": float(loss),
+            "grad": [float(v) for v in grad], "plain_forward_s": fwd_s, "recorded_forward_s": run_s,
+            "reverse_s": rev_s, "cost_ratio": (run_s + rev_s) / fwd_s}
+    if not a.no_fd:
+        fd = []
+        for j in range(th0.numel()):
+            d = 1e-3 * abs(float(th0[j]))
+            tp, tm = th0.clone(), th0.clone()
+            tp[j] += d
+            tm[j] -= d
+            fd.append((cal.run(tp).item() - cal.run(tm).item()) / (2 * d))
+        fd = np.array(fd)
+        g = grad.numpy()
+        line["fd"] = fd.tolist()
+        line["fd_max_rel_err"] = float(np.abs(g - fd).max() / np.abs(g).max())
+    print(json.dumps(line), flush=True)
+
+
 def main():
     a = parse()
     if a.mode == "copy":
@@ -134,6 +333,10 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
                                    "described, not implemented)"}))
         sys.exit(2)
+    if a.mode == "ensemble":
+        return ensemble_mode(a)
+    if a.mode == "adjoint":
+        return adjoint_mode(a)
     import numpy as np
     import torch
 
@@ -188,7 +391,8 @@ def main():
                    "cells": cells, "dx_m": 5.0, "order": a.order, "rain_mm_per_h": 50.0,
                    "sim_time_s": ctrl["t"]},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": profiled_traffic(dom, f"rainfall_runoff_{a.size}_{a.dtype}_order{a.order}"),
                      "alg_bytes_per_cell": B[dom], "kernel_ms": per_kernel[dom]},
         "kernels_ms": per_kernel,
         "path": best["path"],

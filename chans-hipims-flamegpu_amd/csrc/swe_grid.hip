@@ -39,8 +39,8 @@ namespace hp {
 
 constexpr int kTX = 64;
 constexpr int kThreads = 256;
-constexpr int kUpdateIters = 16;  // plane elements per thread per update block
-constexpr int kChunk = kThreads * kUpdateIters;
+constexpr int kVec = 4;       // consecutive plane elements per thread in the update pass
+constexpr int kChunk = 4096;  // plane elements per update block
 constexpr int kMaxStations = 1024;
 
 __device__ inline bool stopped(const hp_ctrl* c) { return c->t_stop > 0.0 && c->t >= c->t_stop; }
@@ -197,25 +197,50 @@ __device__ inline void solve_tile_edge(const FluxTile<T, TY>& s, const hp_grid_d
   }
 }
 
-// Station rain rates at time t_now into s_rain[0..n) (SWE_CUDA.py:472-495):
-// linear in time, 0 from the table's last row on; scaled by rain_scale.
-__device__ inline int station_rates(const hp_grid_desc& a, double t_now, double* s_rain, int cap) {
-  if (a.rain_mode != HP_RAIN_STATION || a.rain_table == nullptr || a.rain_rows <= 0) return 0;
+// Station rain at time t_now (SWE_CUDA.py:472-495): linear in time between
+// the table rows that bracket t_now, 0 from the table's last row on, scaled
+// by rain_scale.  The bracketing rows are found once per block; each cell
+// then evaluates its own station's rate from the two rows (a few cached
+// loads), so no per-block staging array is needed.
+struct RainRows {
+  const double* lo;  // station columns of the rows bracketing t_now
+  const double* hi;
+  double per;        // interpolation weight
+  int n_st;
+  bool zero;         // every station rate is 0 (after the table, or one row)
+};
+
+__device__ inline RainRows rain_rows(const hp_grid_desc& a, double t_now) {
+  RainRows r{nullptr, nullptr, 0.0, 0, true};
+  if (a.rain_mode != HP_RAIN_STATION || a.rain_table == nullptr || a.rain_rows <= 0) return r;
   const int rows = a.rain_rows, cols = a.rain_cols;
-  const int n_st = cols - 1;
   const double* tab = a.rain_table;
+  r.n_st = cols - 1;
   const bool after = !(t_now < tab[(rows - 1) * cols]);
   int i = 0;
   while (i + 1 < rows - 1 && !(t_now < tab[(i + 1) * cols])) i++;
-  for (int k = threadIdx.x; k < n_st && k < cap; k += blockDim.x) {
-    double v = 0.0;
-    if (!after && rows > 1) {
-      const double per = (t_now - tab[i * cols]) / (tab[(i + 1) * cols] - tab[i * cols]);
-      v = tab[i * cols + 1 + k] + per * (tab[(i + 1) * cols + 1 + k] - tab[i * cols + 1 + k]);
-    }
-    s_rain[k] = v * a.rain_scale;
+  r.zero = after || rows <= 1;
+  if (!r.zero) {
+    r.lo = tab + i * cols + 1;
+    r.hi = tab + (i + 1) * cols + 1;
+    r.per = (t_now - tab[i * cols]) / (tab[(i + 1) * cols] - tab[i * cols]);
   }
-  return n_st;
+  return r;
+}
+
+// Rain rate of a cell whose station index is k.
+template <class T> __device__ inline T station_rate(const RainRows& rr, int k, double scale) {
+  if (rr.zero || k < 0 || k >= rr.n_st) return T(0);
+  const double lo = rr.lo[k], hi = rr.hi[k];
+  return T((lo + rr.per * (hi - lo)) * scale);
+}
+
+// Rain rate of plane element i for the descriptor's rain mode.
+template <class T>
+__device__ inline T cell_rain(const hp_grid_desc& a, const RainRows& rr, int64_t i) {
+  if (a.rain_mode == HP_RAIN_STATION) return station_rate<T>(rr, a.station ? (int)a.station[i] : 0, a.rain_scale);
+  if (a.rain_mode == HP_RAIN_FIELD) return T(a.rain_field[i]) * T(a.rain_scale);
+  return T(0);
 }
 
 // Fold a block's CFL minimum and FP64 sums into one partial record.
@@ -246,51 +271,50 @@ __device__ inline void block_partial(T mn, double vol, double fsum, double rsum,
   }
 }
 
+// Per-cell inputs of the update that do not depend on the fluxes; loaded
+// before the flux work so their latency hides behind it.
+template <class T> struct CellIn {
+  T rain;   // rain rate (m/s)
+  T hmax;   // h_max before the step
+  T n;      // Manning n of the cell's class
+  bool on;  // landuse class != 255 (the reference's update covers the cell)
+};
+
 // Rain + implicit friction + update of one cell (HP_UPDATE_STEP semantics,
-// SWE_CUDA.py:780-831), writing the new state to (ho, qxo, qyo) and h_max,
-// and folding the cell into the block's CFL minimum and FP64 sums.
+// SWE_CUDA.py:780-831).  Returns the new state in (h, qx, qy) and the new
+// h_max, plus the cell's contributions to the block's reductions: its CFL
+// candidate (INFINITY if none), its depth for the volume sum (0 if the cell
+// is not covered), and its flux and rain depth increments (0 if not
+// updated).  Every output is assigned on every path, so the caller folds
+// them in unconditionally and the accumulators stay in registers.
 template <class T>
-__device__ inline void update_cell(const hp_grid_desc& a, int64_t i, T h, T qx0, T qy0, T dhf, T dqx, T dqy, T dt,
-                                   const double* s_rain, int n_st, T* ho, T* qxo, T* qyo, T& mn, double& vol,
-                                   double& fsum, double& rsum) {
-  const int cls = a.landuse[i];
-  if (cls == 255) {
-    ho[i] = h; qxo[i] = qx0; qyo[i] = qy0;
-    return;
-  }
-  T r = T(0);
-  if (a.rain_mode == HP_RAIN_STATION) {
-    const int k = a.station ? (int)a.station[i] : 0;
-    r = (k >= 0 && k < n_st) ? T(s_rain[k]) : T(0);
-  } else if (a.rain_mode == HP_RAIN_FIELD) {
-    r = T(a.rain_field[i]) * T(a.rain_scale);
-  }
-  const T rdt = r * dt;
+__device__ inline void update_state(const CellIn<T>& in, T& h, T& qx, T& qy, T dhf, T dqx, T dqy, T dt, T cfl_dx,
+                                    T& hmax, T& cand, T& vol_h, T& flux_h, T& rain_h) {
+  cand = T(INFINITY);
+  vol_h = T(0);
+  flux_h = T(0);
+  rain_h = T(0);
+  if (!in.on) return;
+  const T rdt = in.rain * dt;
   const T dht = dhf + rdt;
   if (!(fabs(dht) > T(0) || h >= T(0))) {
-    ho[i] = h; qxo[i] = qx0; qyo[i] = qy0;
-    vol += (double)h;
+    vol_h = h;
     return;
   }
   h += dht;
-  T qx, qy;
   if (h < T(kHs)) {
     qx = T(0);
     qy = T(0);
   } else {
-    qx = qx0 + dqx;
-    qy = qy0 + dqy;
-    implicit_friction<T>(h, ((const T*)a.manning)[cls], dt, qx, qy);
+    qx += dqx;
+    qy += dqy;
+    implicit_friction<T>(h, in.n, dt, qx, qy);
   }
-  ho[i] = h;
-  qxo[i] = qx;
-  qyo[i] = qy;
-  T* hmp = (T*)a.h_max;
-  hmp[i] = fmax(h, hmp[i]);
-  mn = fmin(mn, cfl_candidate<T>(h, qx, qy, T(a.cfl) * T(a.dx)));
-  vol += (double)h;
-  fsum += (double)dhf;
-  rsum += (double)rdt;
+  hmax = fmax(h, hmax);
+  cand = cfl_candidate<T>(h, qx, qy, cfl_dx);
+  vol_h = h;
+  flux_h = dhf;
+  rain_h = rdt;
 }
 
 // FUSED = false: write the flux accumulators dh/dqx/dqy (two-pass step).
@@ -301,14 +325,17 @@ __device__ inline void update_cell(const hp_grid_desc& a, int64_t i, T h, T qx0,
 template <class T, int ORDER, int TY, bool FUSED>
 __device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by, Partial* part) {
   __shared__ FluxTile<T, TY> s;
-  __shared__ double s_rain[FUSED ? kMaxStations : 1];
+  __shared__ T s_man[FUSED ? 256 : 1];
   using Tile = FluxTile<T, TY>;
-  constexpr int LW = Tile::LW, LH = Tile::LH;
+  constexpr int LH = Tile::LH;
+  constexpr int CPT = TY * kTX / kThreads;  // own cells per thread
+  static_assert(TY * kTX % kThreads == 0, "tile cells per thread");
   if (stopped(a.ctrl)) {
     if (FUSED && threadIdx.x == 0) *part = Partial{INFINITY, 0.0, 0.0, 0.0};
     return;
   }
   const int tid = threadIdx.x;
+  const int lane = tid & (kTX - 1), wv = tid / kTX;
   const int x0 = bx * kTX, y0 = by * TY;
   const int nx = a.nx, ny = a.ny;
   const int64_t pitch = a.pitch;
@@ -318,16 +345,57 @@ __device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by, Partial*
   const T* __restrict__ z = (const T*)a.z;
   const uint8_t* __restrict__ ty = a.type_slot;
 
-  for (int e = tid; e < LH * LW; e += kThreads) {
-    const int r = e / LW, c = e - r * LW;
-    const int gy = y0 - 2 + r, gx = x0 - 2 + c;
-    const bool in = gy >= 0 && gy < ny && gx >= 0 && gx < nx;
-    const int64_t gi = in ? (int64_t)gy * pitch + gx : 0;
-    s.ty[r][c] = in ? ty[gi] : (uint8_t)0;
-    s.h[r][c] = in ? h[gi] : T(0);
-    s.qx[r][c] = in ? qx[gi] : T(0);
-    s.qy[r][c] = in ? qy[gi] : T(0);
-    s.z[r][c] = in ? z[gi] : T(0);
+  // the thread's own cells: column x0 + lane, rows y0 + wv + 4k
+  CellIn<T> cin[FUSED ? CPT : 1];
+  if (FUSED) {
+    for (int k = tid; k < a.n_classes; k += kThreads) s_man[k] = ((const T*)a.manning)[k];
+    const RainRows rr = rain_rows(a, a.ctrl->t);
+    const int gx = x0 + lane;
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+      const int gy = y0 + wv + 4 * k;
+      cin[k] = CellIn<T>{T(0), T(0), T(0), false};
+      if (gx < nx && gy < ny) {
+        const int64_t gi = (int64_t)gy * pitch + gx;
+        const int cls = a.landuse[gi];
+        cin[k].on = cls != 255;
+        cin[k].n = T(cls);  // class id until the Manning table is in LDS
+        cin[k].rain = cell_rain<T>(a, rr, gi);
+        cin[k].hmax = ((const T*)a.h_max)[gi];
+      }
+    }
+  }
+
+  // Stage the tile and a 2-cell halo: the 64 interior columns row by row
+  // (one wave per row, coalesced), then the 4 halo columns.
+  {
+    const int gx = x0 + lane;
+    for (int r = wv; r < LH; r += kThreads / kTX) {
+      const int gy = y0 - 2 + r;
+      const bool in = gx < nx && gy >= 0 && gy < ny;
+      T vh = T(0), vqx = T(0), vqy = T(0), vz = T(0);
+      uint8_t vt = 0;
+      if (in) {
+        const int64_t gi = (int64_t)gy * pitch + gx;
+        vt = ty[gi]; vh = h[gi]; vqx = qx[gi]; vqy = qy[gi]; vz = z[gi];
+      }
+      s.ty[r][lane + 2] = vt; s.h[r][lane + 2] = vh; s.qx[r][lane + 2] = vqx; s.qy[r][lane + 2] = vqy;
+      s.z[r][lane + 2] = vz;
+    }
+    static_assert(4 * LH <= kThreads, "halo staging");
+    if (tid < 4 * LH) {
+      const int r = tid >> 2, k = tid & 3;
+      const int c = k < 2 ? k : kTX + k;  // local columns 0, 1, 66, 67
+      const int gy = y0 - 2 + r, gxh = x0 - 2 + c;
+      const bool in = gxh >= 0 && gxh < nx && gy >= 0 && gy < ny;
+      T vh = T(0), vqx = T(0), vqy = T(0), vz = T(0);
+      uint8_t vt = 0;
+      if (in) {
+        const int64_t gi = (int64_t)gy * pitch + gxh;
+        vt = ty[gi]; vh = h[gi]; vqx = qx[gi]; vqy = qy[gi]; vz = z[gi];
+      }
+      s.ty[r][c] = vt; s.h[r][c] = vh; s.qx[r][c] = vqx; s.qy[r][c] = vqy; s.z[r][c] = vz;
+    }
   }
 
   const T t = T(a.ctrl->t);
@@ -338,7 +406,6 @@ __device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by, Partial*
   rows.d = a.given_depth ? series_row<T>((const T*)a.given_depth, a.gd_rows, a.gd_w, t) : 0;
   rows.w = a.given_wl ? series_row<T>((const T*)a.given_wl, a.gwl_rows, a.gwl_w, t) : 0;
   rows.q = a.given_q ? series_row<T>((const T*)a.given_q, a.gq_rows, a.gq_w, t) : 0;
-  const int n_st = FUSED ? station_rates(a, a.ctrl->t, s_rain, kMaxStations) : 0;
   __syncthreads();
 
   // Edges touching the tile: x-edges (row ly, between local columns lx-1 = A
@@ -368,26 +435,25 @@ __device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by, Partial*
   T* __restrict__ dh = (T*)a.dh;
   T* __restrict__ dqx = (T*)a.dqx;
   T* __restrict__ dqy = (T*)a.dqy;
+  const T cfl_dx = T(a.cfl) * T(a.dx);
   T mn = T(INFINITY);
   double vol = 0.0, fsum = 0.0, rsum = 0.0;
-  for (int e = tid; e < TY * kTX; e += kThreads) {
-    const int ly = e / kTX, lx = e - ly * kTX;
+#pragma unroll
+  for (int k = 0; k < CPT; ++k) {
+    const int ly = wv + 4 * k, lx = lane;
     const int gy = y0 + ly, gx = x0 + lx;
     if (gy >= ny || gx >= nx) continue;
     const int r = ly + 2, c = lx + 2;
     T ah = T(0), aqx = T(0), aqy = T(0);
     if (s.ty[r][c] != 0) {
-      bool listed = t_zero || s.h[r][c] > hs;
-      // neighbours E, N, W, S; compact cell 0 never counts (fluxMaskGenerator_Kernel.cu:36)
-      const int nr[4] = {r, r - 1, r, r + 1};
-      const int nc[4] = {c + 1, c, c - 1, c};
-      const int ngx[4] = {gx + 1, gx, gx - 1, gx};
-      const int ngy[4] = {gy, gy - 1, gy, gy + 1};
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const bool is0 = (ngx[k] == a.cell0_x && ngy[k] == a.cell0_y);
-        listed = listed || (s.ty[nr[k]][nc[k]] != 0 && !is0 && s.h[nr[k]][nc[k]] > hs);
-      }
+      // fluxMask rule: t == 0, wet, or a wet valid neighbour E, N, W, S that
+      // is not compact cell 0 (fluxMaskGenerator_Kernel.cu:36); evaluated
+      // without short-circuit so the four LDS reads issue together
+      const bool w_e = (s.ty[r][c + 1] != 0) & (s.h[r][c + 1] > hs) & !(gx + 1 == a.cell0_x && gy == a.cell0_y);
+      const bool w_n = (s.ty[r - 1][c] != 0) & (s.h[r - 1][c] > hs) & !(gx == a.cell0_x && gy - 1 == a.cell0_y);
+      const bool w_w = (s.ty[r][c - 1] != 0) & (s.h[r][c - 1] > hs) & !(gx - 1 == a.cell0_x && gy == a.cell0_y);
+      const bool w_s = (s.ty[r + 1][c] != 0) & (s.h[r + 1][c] > hs) & !(gx == a.cell0_x && gy + 1 == a.cell0_y);
+      const bool listed = t_zero | (s.h[r][c] > hs) | w_e | w_n | w_w | w_s;
       if (listed) {
         const T* eE = s.e[ly * (kTX + 1) + lx + 1];         // this cell is A
         const T* eN = s.e[Tile::NX + ly * kTX + lx];        // this cell is B
@@ -406,8 +472,19 @@ __device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by, Partial*
       dqx[gi] = aqx;
       dqy[gi] = aqy;
     } else {
-      update_cell<T>(a, gi, s.h[r][c], s.qx[r][c], s.qy[r][c], ah, aqx, aqy, dt, s_rain, n_st, (T*)a.h_out,
-                     (T*)a.qx_out, (T*)a.qy_out, mn, vol, fsum, rsum);
+      CellIn<T> in = cin[k];
+      in.n = s_man[(int)in.n];
+      T hn = s.h[r][c], qxn = s.qx[r][c], qyn = s.qy[r][c], hm = in.hmax;
+      T cand, vh, fh, rh;
+      update_state<T>(in, hn, qxn, qyn, ah, aqx, aqy, dt, cfl_dx, hm, cand, vh, fh, rh);
+      mn = fmin(mn, cand);
+      vol += (double)vh;
+      fsum += (double)fh;
+      rsum += (double)rh;
+      ((T*)a.h_out)[gi] = hn;
+      ((T*)a.qx_out)[gi] = qxn;
+      ((T*)a.qy_out)[gi] = qyn;
+      if (hm != in.hmax) ((T*)a.h_max)[gi] = hm;
     }
   }
   if (FUSED) block_partial<T>(mn, vol, fsum, rsum, part);
@@ -456,39 +533,35 @@ __global__ __launch_bounds__(kThreads) void k_step_fused(hp_grid_desc a, int ntx
 //                       state change and no rain; h_max, CFL + dt/t
 // Processes plane elements [chunk*kChunk, (chunk+1)*kChunk) and writes one
 // partial record.
+template <class T> struct alignas(sizeof(T) * kVec) VecT {
+  T x[kVec];
+};
+struct alignas(kVec) VecU8 {
+  uint8_t x[kVec];
+};
+struct alignas(2 * kVec) VecI16 {
+  int16_t x[kVec];
+};
+struct alignas(4 * kVec) VecF32 {
+  float x[kVec];
+};
+template <class V, class T> __device__ inline V ldv(const T* p) { return *reinterpret_cast<const V*>(p); }
+template <class V, class T> __device__ inline void stv(T* p, const V& v) { *reinterpret_cast<V*>(p) = v; }
+
 template <class T, int MODE>
 __device__ inline void update_chunk(const hp_grid_desc& a, int64_t chunk, Partial* __restrict__ part) {
-  __shared__ double s_rain[kMaxStations];
-  __shared__ double s_red[4][kThreads / 64];
+  __shared__ T s_man[256];
   const int tid = threadIdx.x;
   if (stopped(a.ctrl)) {
     if (tid == 0) *part = Partial{INFINITY, 0.0, 0.0, 0.0};
     return;
   }
-  const double t_now = a.ctrl->t;
   const T dt = T(a.ctrl->dt);
   const T hs = T(kHs);
   const T cfl_dx = T(a.cfl) * T(a.dx);
-
-  // station rain values at the current time (SWE_CUDA.py:472-495)
-  int n_st = 0;
+  for (int k = tid; k < a.n_classes; k += kThreads) s_man[k] = ((const T*)a.manning)[k];
+  const RainRows rr = rain_rows(a, a.ctrl->t);
   const int mode = a.rain_mode;
-  if (mode == HP_RAIN_STATION && a.rain_table != nullptr && a.rain_rows > 0) {
-    const int rows = a.rain_rows, cols = a.rain_cols;
-    n_st = cols - 1;
-    const double* tab = a.rain_table;
-    const bool after = !(t_now < tab[(rows - 1) * cols]);
-    int i = 0;
-    while (i + 1 < rows - 1 && !(t_now < tab[(i + 1) * cols])) i++;
-    for (int k = tid; k < n_st && k < kMaxStations; k += kThreads) {
-      double v = 0.0;
-      if (!after && rows > 1) {
-        const double per = (t_now - tab[i * cols]) / (tab[(i + 1) * cols] - tab[i * cols]);
-        v = tab[i * cols + 1 + k] + per * (tab[(i + 1) * cols + 1 + k] - tab[i * cols + 1 + k]);
-      }
-      s_rain[k] = v * a.rain_scale;
-    }
-  }
   __syncthreads();
 
   const T* __restrict__ dhp = (const T*)a.dh;
@@ -499,92 +572,91 @@ __device__ inline void update_chunk(const hp_grid_desc& a, int64_t chunk, Partia
   T* __restrict__ qyp = (T*)a.qy;
   T* __restrict__ hmp = (T*)a.h_max;
   const uint8_t* __restrict__ lu = a.landuse;
-  const T* __restrict__ man = (const T*)a.manning;
   const int16_t* __restrict__ st = a.station;
   const float* __restrict__ rf = a.rain_field;
   const T rscale = T(a.rain_scale);
 
   T mn = T(INFINITY);
   double vol = 0.0, fsum = 0.0, rsum = 0.0;
-  const int64_t total = a.pitch * (int64_t)a.ny;
+  const int64_t total = a.pitch * (int64_t)a.ny;  // a multiple of 64
   const int64_t base = chunk * kChunk;
-#pragma unroll 4
-  for (int it = 0; it < kUpdateIters; ++it) {
-    const int64_t i = base + (int64_t)it * kThreads + tid;
-    if (i >= total) break;
-    const int cls = lu[i];
-    if (cls == 255) continue;
-    T h = hp_[i];
-    if (MODE == HP_UPDATE_CFL) {
-      if (h > hs) {
-        hmp[i] = fmax(h, hmp[i]);
-        mn = fmin(mn, cfl_candidate<T>(h, qxp[i], qyp[i], cfl_dx));
+  for (int it = 0; it < kChunk / (kThreads * kVec); ++it) {
+    const int64_t i0 = base + ((int64_t)it * kThreads + tid) * kVec;
+    if (i0 >= total) break;
+    const VecU8 vl = ldv<VecU8>(lu + i0);
+    VecT<T> vh = ldv<VecT<T>>(hp_ + i0);
+    VecT<T> vqx = ldv<VecT<T>>(qxp + i0);
+    VecT<T> vqy = ldv<VecT<T>>(qyp + i0);
+    VecT<T> vhm = ldv<VecT<T>>(hmp + i0);
+    VecT<T> vdh{}, vdqx{}, vdqy{}, vr{};
+    if (MODE != HP_UPDATE_CFL) {
+      vdh = ldv<VecT<T>>(dhp + i0);
+      vdqx = ldv<VecT<T>>(dqxp + i0);
+      vdqy = ldv<VecT<T>>(dqyp + i0);
+      if (mode == HP_RAIN_STATION) {
+        VecI16 vs{};
+        if (st) vs = ldv<VecI16>(st + i0);
+#pragma unroll
+        for (int j = 0; j < kVec; ++j) vr.x[j] = station_rate<T>(rr, vs.x[j], a.rain_scale);
+      } else if (mode == HP_RAIN_FIELD) {
+        const VecF32 vf = ldv<VecF32>(rf + i0);
+#pragma unroll
+        for (int j = 0; j < kVec; ++j) vr.x[j] = T(vf.x[j]) * rscale;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kVec; ++j) {
+      const int cls = vl.x[j];
+      if (cls == 255) continue;
+      T h = vh.x[j];
+      if (MODE == HP_UPDATE_CFL) {
+        if (h > hs) {
+          vhm.x[j] = fmax(h, vhm.x[j]);
+          mn = fmin(mn, cfl_candidate<T>(h, vqx.x[j], vqy.x[j], cfl_dx));
+        }
+        vol += (double)h;
+        continue;
+      }
+      const T dhf = vdh.x[j];
+      const T rdt = vr.x[j] * dt;
+      const T dht = dhf + rdt;
+      const bool listed = (MODE == HP_UPDATE_EULER) ? (fabs(dht) > T(0) || h > hs) : (fabs(dht) > T(0) || h >= T(0));
+      if (!listed) {
+        vol += (double)h;
+        continue;
+      }
+      h += dht;
+      T qx, qy;
+      if (MODE == HP_UPDATE_EULER) {
+        qx = vqx.x[j] + vdqx.x[j];
+        qy = vqy.x[j] + vdqy.x[j];
+      } else if (h < hs) {
+        qx = T(0);
+        qy = T(0);
+      } else {
+        qx = vqx.x[j] + vdqx.x[j];
+        qy = vqy.x[j] + vdqy.x[j];
+        implicit_friction<T>(h, s_man[cls], dt, qx, qy);
+      }
+      vh.x[j] = h;
+      vqx.x[j] = qx;
+      vqy.x[j] = qy;
+      if (MODE == HP_UPDATE_STEP) {
+        vhm.x[j] = fmax(h, vhm.x[j]);
+        mn = fmin(mn, cfl_candidate<T>(h, qx, qy, cfl_dx));
       }
       vol += (double)h;
-      continue;
+      fsum += (double)dhf;
+      rsum += (double)rdt;
     }
-    const T dhf = dhp[i];
-    T r = T(0);
-    if (mode == HP_RAIN_STATION) {
-      const int k = st ? (int)st[i] : 0;
-      r = (k >= 0 && k < n_st) ? T(s_rain[k]) : T(0);
-    } else if (mode == HP_RAIN_FIELD) {
-      r = T(rf[i]) * rscale;
+    if (MODE != HP_UPDATE_CFL) {
+      stv(hp_ + i0, vh);
+      stv(qxp + i0, vqx);
+      stv(qyp + i0, vqy);
     }
-    const T rdt = r * dt;
-    const T dht = dhf + rdt;
-    const bool listed = (MODE == HP_UPDATE_EULER) ? (fabs(dht) > T(0) || h > hs) : (fabs(dht) > T(0) || h >= T(0));
-    if (!listed) {
-      vol += (double)h;
-      continue;
-    }
-    h += dht;
-    T qx, qy;
-    if (MODE == HP_UPDATE_EULER) {
-      qx = qxp[i] + dqxp[i];
-      qy = qyp[i] + dqyp[i];
-    } else if (h < hs) {
-      qx = T(0);
-      qy = T(0);
-    } else {
-      qx = qxp[i] + dqxp[i];
-      qy = qyp[i] + dqyp[i];
-      implicit_friction<T>(h, man[cls], dt, qx, qy);
-    }
-    hp_[i] = h;
-    qxp[i] = qx;
-    qyp[i] = qy;
-    if (MODE == HP_UPDATE_STEP) {
-      hmp[i] = fmax(h, hmp[i]);
-      mn = fmin(mn, cfl_candidate<T>(h, qx, qy, cfl_dx));
-    }
-    vol += (double)h;
-    fsum += (double)dhf;
-    rsum += (double)rdt;
+    if (MODE == HP_UPDATE_STEP || MODE == HP_UPDATE_CFL) stv(hmp + i0, vhm);
   }
-
-  const int lane = tid & 63, w = tid >> 6;
-  mn = wave_min(mn);
-  vol = wave_sum(vol);
-  fsum = wave_sum(fsum);
-  rsum = wave_sum(rsum);
-  if (lane == 0) {
-    s_red[0][w] = (double)mn;
-    s_red[1][w] = vol;
-    s_red[2][w] = fsum;
-    s_red[3][w] = rsum;
-  }
-  __syncthreads();
-  if (tid == 0) {
-    Partial p{s_red[0][0], s_red[1][0], s_red[2][0], s_red[3][0]};
-    for (int k = 1; k < kThreads / 64; ++k) {
-      p.min_dt = fmin(p.min_dt, s_red[0][k]);
-      p.vol += s_red[1][k];
-      p.flux += s_red[2][k];
-      p.rain += s_red[3][k];
-    }
-    *part = p;
-  }
+  block_partial<T>(mn, vol, fsum, rsum, part);
 }
 
 template <class T, int MODE>
@@ -676,7 +748,7 @@ static int64_t update_blocks(const hp_grid_desc* g) {
 
 static int flux_rows(const hp_grid_desc* g) {
   if (g->dtype == HP_F32) return g->tile_rows == 4 ? 4 : 8;
-  return g->tile_rows == 2 ? 2 : 4;
+  return 4;
 }
 
 static int64_t flux_tiles(const hp_grid_desc* g) {
@@ -690,12 +762,22 @@ static int check_desc(const hp_grid_desc* g) {
   if (g->order != 1 && g->order != 2) return fail(HP_EINVAL, "order must be 1 or 2");
   if (g->nx <= 0 || g->ny <= 0) return fail(HP_EINVAL, "nx and ny must be positive");
   if (g->pitch < g->nx) return fail(HP_EINVAL, "pitch must be >= nx");
+  if (g->pitch % kVec != 0) return fail(HP_EINVAL, "pitch must be a multiple of 4");
   if (!(g->dx > 0.0)) return fail(HP_EINVAL, "dx must be positive");
   if (!g->h || !g->qx || !g->qy || !g->z || !g->h_max || !g->dh || !g->dqx || !g->dqy)
     return fail(HP_EINVAL, "a state plane pointer is null");
   if (!g->type_slot || !g->landuse || !g->slot_code || !g->manning || !g->ctrl)
     return fail(HP_EINVAL, "type_slot, landuse, slot_code, manning and ctrl are required");
   if (g->n_classes <= 0 || g->n_classes > 255) return fail(HP_EINVAL, "n_classes must be in 1..255");
+  {
+    // the update pass moves kVec consecutive elements per access
+    const uintptr_t va = (uintptr_t)kVec * (g->dtype == HP_F32 ? 4 : 8);
+    const void* planes[] = {g->h, g->qx, g->qy, g->h_max, g->dh, g->dqx, g->dqy};
+    for (const void* p : planes)
+      if ((uintptr_t)p % va) return fail(HP_EINVAL, "state planes must be aligned to 4 elements");
+    if ((uintptr_t)g->landuse % kVec || (uintptr_t)g->station % (2 * kVec) || (uintptr_t)g->rain_field % (4 * kVec))
+      return fail(HP_EINVAL, "landuse, station and rain_field planes must be aligned to 4 elements");
+  }
   if (g->rain_mode == HP_RAIN_STATION) {
     if (!g->rain_table || g->rain_rows < 1 || g->rain_cols < 2)
       return fail(HP_EINVAL, "station rain needs a table of >= 1 row and >= 2 columns");
@@ -723,7 +805,7 @@ template <class T, int ORDER>
 static int launch_flux_t(const hp_grid_desc* g, hipStream_t st) {
   const int ty = flux_rows(g);
   if (sizeof(T) == 4) return ty == 4 ? launch_flux_single<T, ORDER, 4>(g, st) : launch_flux_single<T, ORDER, 8>(g, st);
-  return ty == 2 ? launch_flux_single<T, ORDER, 2>(g, st) : launch_flux_single<T, ORDER, 4>(g, st);
+  return launch_flux_single<T, ORDER, 4>(g, st);
 }
 
 static int launch_flux(const hp_grid_desc* g, hipStream_t st) {
@@ -744,8 +826,8 @@ static int launch_step_fused(const hp_grid_desc* g, hipStream_t st) {
     if (g->order == 2) return ty == 4 ? launch_fused_t<float, 2, 4>(g, st) : launch_fused_t<float, 2, 8>(g, st);
     return ty == 4 ? launch_fused_t<float, 1, 4>(g, st) : launch_fused_t<float, 1, 8>(g, st);
   }
-  if (g->order == 2) return ty == 2 ? launch_fused_t<double, 2, 2>(g, st) : launch_fused_t<double, 2, 4>(g, st);
-  return ty == 2 ? launch_fused_t<double, 1, 2>(g, st) : launch_fused_t<double, 1, 4>(g, st);
+  if (g->order == 2) return launch_fused_t<double, 2, 4>(g, st);
+  return launch_fused_t<double, 1, 4>(g, st);
 }
 
 template <int MODE>
@@ -816,8 +898,8 @@ static int ensemble_step_once(const hp_ensemble* e, hipStream_t st) {
     if (e->order == 2) (ty == 4 ? launch_flux_multi<float, 2, 4> : launch_flux_multi<float, 2, 8>)(e, st);
     else (ty == 4 ? launch_flux_multi<float, 1, 4> : launch_flux_multi<float, 1, 8>)(e, st);
   } else {
-    if (e->order == 2) (ty == 2 ? launch_flux_multi<double, 2, 2> : launch_flux_multi<double, 2, 4>)(e, st);
-    else (ty == 2 ? launch_flux_multi<double, 1, 2> : launch_flux_multi<double, 1, 4>)(e, st);
+    if (e->order == 2) launch_flux_multi<double, 2, 4>(e, st);
+    else launch_flux_multi<double, 1, 4>(e, st);
   }
   int rc = check_launch("k_flux_multi");
   if (rc) return rc;

@@ -28,7 +28,8 @@
 //           datum) so thin films are not lost in the rounding of a 50 m
 //           water level; the implicit-friction quotient uses its
 //           cancellation-free form 2q / (1 + sqrt(1 + x)); divisions and
-//           square roots use the hardware reciprocal / square-root (<= 1 ulp).
+//           square roots use the hardware reciprocal / square-root (<= 1 ulp);
+//           the HLLC star celerity sqrt(g * base^2 / g) is taken as |base|.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -47,6 +48,11 @@ template <> struct Prec<double> {
   __host__ __device__ static inline double rcbrt_friction(double h) { return pow(h, -1.0 / 3.0); }
   // (x / dx) * dt, the reference's order of operations
   __host__ __device__ static inline double scale(double x, double dx, double dt, double) { return (x / dx) * dt; }
+  // star-region celerity sqrt(g h*) with h* = base^2 / g, as the reference forms it
+  __host__ __device__ static inline double star_celerity(double base, double g) {
+    const double h_star = base * base / g;
+    return __builtin_sqrt(g * h_star);
+  }
 };
 template <> struct Prec<float> {
   static constexpr bool local_datum = true;
@@ -55,6 +61,8 @@ template <> struct Prec<float> {
   __device__ static inline float rcbrt_friction(float h) { return rcbrtf(h); }
   // x * (dt / dx) with the quotient computed once per block
   __device__ static inline float scale(float x, float, float, float dt_dx) { return x * dt_dx; }
+  // sqrt(g * base^2 / g) is |base| exactly; no square root needed
+  __device__ static inline float star_celerity(float base, float) { return fabsf(base); }
 };
 
 template <class T> __host__ __device__ inline T minmod(T x, T y) {
@@ -203,9 +211,8 @@ __host__ __device__ inline void hllc_edge(FrameCell<T>& eL, FrameCell<T>& eR, T 
   const T aL = P::sqrt_(g * hL);
   const T aR = P::sqrt_(g * hR);
   const T base = (aL + aR) / T(2) + (uL0 - uR0) / T(4);
-  const T h_star = P::div(base * base, g);
   const T u_star = (uL0 + uR0) / T(2) + aL - aR;
-  const T a_star = P::sqrt_(g * h_star);
+  const T a_star = P::star_celerity(base, g);
   const T sL = hL <= hs ? uR0 - T(2) * aR : fmin(uL0 - aL, u_star - a_star);
   const T sR = hR <= hs ? uL0 + T(2) * aL : fmax(uR0 + aR, u_star + a_star);
   const T sM = P::div(sL * hR * (uR0 - sR) - sR * hL * (uL0 - sL), hR * (uR0 - sR) - hL * (uL0 - sL));
@@ -347,7 +354,7 @@ template <class T>
 __device__ inline T cfl_candidate(T h, T qx, T qy, T cfl_dx) {
   if (!(h > T(kHs))) return T(60.0);
   const float a = __builtin_sqrtf((float)(T(kG) * h));
-  return cfl_dx / (T(a) + Prec<T>::div(fmax(fabs(qx), fabs(qy)), h));
+  return Prec<T>::div(cfl_dx, T(a) + Prec<T>::div(fmax(fabs(qx), fabs(qy)), h));
 }
 
 // Mask-code decode (fluxCal_jh_modified_surface_Kernel.cu:528-535): leading

@@ -33,13 +33,18 @@ class Ensemble:
         if t_stop is not None:
             for s in solvers:
                 s.set_t_stop(t_stop)
-        n = len(solvers)
-        self._descs = (HpGridDesc * n)(*[s.desc for s in solvers])
+        self._build(list(range(len(solvers))))
+
+    def _build(self, members: list[int]):
+        """(Re)build the launch maps over the scenarios `members`."""
+        self.members = members
+        n = len(members)
+        self._descs = (HpGridDesc * n)(*[self.solvers[k].desc for k in members])
         need = int(lib().hp_ensemble_workspace_bytes(n, self._descs))
-        self.work = torch.empty(max(need, 256), dtype=torch.uint8, device=dev)
+        self.work = torch.empty(max(need, 256), dtype=torch.uint8, device=self.device)
         self.handle = HpEnsemble()
         check(lib().hp_ensemble_init(n, self._descs, self.work.data_ptr(), self.work.numel(),
-                                     ctypes.byref(self.handle), _lib.stream_ptr(dev)), "hp_ensemble_init")
+                                     ctypes.byref(self.handle), _lib.stream_ptr(self.device)), "hp_ensemble_init")
         self.graph = None
         self.graph_steps = 0
 
@@ -68,19 +73,34 @@ class Ensemble:
     def times(self) -> torch.Tensor:
         return torch.stack([s.ctrl[_lib.CTRL_T] for s in self.solvers]).cpu()
 
-    def active(self) -> int:
-        """Scenarios still below their stop time (one host synchronisation)."""
+    def active_mask(self) -> torch.Tensor:
+        """Per scenario: still below its stop time (one host synchronisation)."""
         c = torch.stack([s.ctrl for s in self.solvers]).cpu()
         t, stop = c[:, _lib.CTRL_T], c[:, _lib.CTRL_T_STOP]
-        return int(((stop <= 0) | (t < stop)).sum())
+        return (stop <= 0) | (t < stop)
 
-    def run(self, steps_per_replay: int = 50, max_replays: int = 100000) -> int:
+    def active(self) -> int:
+        return int(self.active_mask().sum())
+
+    def run(self, steps_per_replay: int = 50, max_replays: int = 10_000_000, compact: float = 0.75,
+            progress=None) -> int:
         """Replay the captured graph until every scenario has stopped; returns
-        the number of replays."""
+        the number of replays.  Stopped scenarios cost a launched-and-exiting
+        block per tile, so once the active count falls to `compact` of the
+        scenarios in the launch maps, the maps are rebuilt over the active
+        ones and the graph is captured again.  `progress(replays, active)` is
+        called after each host check."""
         if self.graph is None or self.graph_steps != steps_per_replay:
             self.capture(steps_per_replay)
         for k in range(max_replays):
-            if self.active() == 0:
+            act = self.active_mask()
+            n_act = int(act.sum())
+            if progress is not None:
+                progress(k, n_act)
+            if n_act == 0:
                 return k
+            if n_act <= compact * len(self.members):
+                self._build([m for m in self.members if bool(act[m])])
+                self.capture(steps_per_replay)
             self.replay()
         raise RuntimeError("ensemble did not finish within max_replays")
