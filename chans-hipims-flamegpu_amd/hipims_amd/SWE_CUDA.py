@@ -196,7 +196,9 @@ class Godunov:
     def _export_if_due(self):
         if self._ctrl[4].item() > 0:
             self._ctrl[4] = 0.0
-            self.exportField()
+            # the reference exports before t += dt, naming the files t + dt,
+            # which is the already advanced t here
+            self._export(self._time_tag(self.t.item()))
             print("give a output")
 
     def time_friction_euler_update_cuda(self, device):
@@ -290,12 +292,15 @@ class Godunov:
             row += list(self.get_h()[observe_index].cpu().numpy())
             dataStoreList.append(row)
 
-    def _time_tag(self) -> str:
-        # the reference names files with str(t + dt) of its state-dtype tensors
-        return str(torch.tensor([self.t.item()], dtype=self._tensorType, device=self._device))
+    def _time_tag(self, t: float) -> str:
+        # the reference names files with str() of a state-dtype tensor
+        return str(torch.tensor([t], dtype=self._tensorType, device=self._device))
 
     def exportField(self):
-        tag = self._time_tag()
+        """Snapshots named with str(t + dt), as SWE_CUDA.py:836-868."""
+        self._export(self._time_tag(self.t.item() + self.dt.item()))
+
+    def _export(self, tag: str):
         for name, val in (("h", self._h_internal), ("qx", self._qx_internal), ("qy", self._qy_internal),
                           ("wl", self._wl_internal), ("h_max", self._h_max)):
             torch.save(val, os.path.join(self._outpath, f"{name}_{tag}.pt"))
