@@ -108,6 +108,7 @@ def parse():
     p.add_argument("--size", type=int, default=2048)
     p.add_argument("--dtype", choices=["fp32", "fp64"], default="fp32")
     p.add_argument("--order", type=int, choices=[1, 2], default=2)
+    p.add_argument("--tile-rows", type=int, default=0, help="flux tile height (0: the library default)")
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     p.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, available cores)")
     p.add_argument("--no-cpu", action="store_true")
@@ -351,6 +352,8 @@ def main():
     g = GridSolver(case.mask, case.h, case.qx, case.qy, case.z, case.dx, cfl=case.cfl, order=a.order, dtype=dtype,
                    device=dev, first_dt=case.first_dt, manning=case.manning, landuse=case.landuse)
     g.set_station_rain(case.rain_table)
+    if a.tile_rows:
+        g.set_tile_rows(a.tile_rows)
     cells = case.n_valid
 
     g.step(a.warmup)
@@ -389,6 +392,7 @@ def main():
         "data": "synthetic (seeded fractal DEM, Voronoi landcover, uniform rain)",
         "config": {"workload": f"rainfall_runoff_{a.size}x{a.size}_dx5_{a.dtype}_order{a.order}",
                    "cells": cells, "dx_m": 5.0, "order": a.order, "rain_mm_per_h": 50.0,
+                   "tile_rows": a.tile_rows or (8 if esize == 4 else 4),
                    "sim_time_s": ctrl["t"]},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,

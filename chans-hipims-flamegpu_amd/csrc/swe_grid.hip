@@ -747,7 +747,7 @@ static int64_t update_blocks(const hp_grid_desc* g) {
 }
 
 static int flux_rows(const hp_grid_desc* g) {
-  if (g->dtype == HP_F32) return g->tile_rows == 4 ? 4 : 8;
+  if (g->dtype == HP_F32) return (g->tile_rows == 4 || g->tile_rows == 16) ? g->tile_rows : 8;
   return 4;
 }
 
@@ -804,7 +804,10 @@ static int launch_flux_single(const hp_grid_desc* g, hipStream_t st) {
 template <class T, int ORDER>
 static int launch_flux_t(const hp_grid_desc* g, hipStream_t st) {
   const int ty = flux_rows(g);
-  if (sizeof(T) == 4) return ty == 4 ? launch_flux_single<T, ORDER, 4>(g, st) : launch_flux_single<T, ORDER, 8>(g, st);
+  if (sizeof(T) == 4) {
+    if (ty == 4) return launch_flux_single<T, ORDER, 4>(g, st);
+    return ty == 16 ? launch_flux_single<T, ORDER, 16>(g, st) : launch_flux_single<T, ORDER, 8>(g, st);
+  }
   return launch_flux_single<T, ORDER, 4>(g, st);
 }
 
@@ -823,8 +826,12 @@ static int launch_fused_t(const hp_grid_desc* g, hipStream_t st) {
 static int launch_step_fused(const hp_grid_desc* g, hipStream_t st) {
   const int ty = flux_rows(g);
   if (g->dtype == HP_F32) {
-    if (g->order == 2) return ty == 4 ? launch_fused_t<float, 2, 4>(g, st) : launch_fused_t<float, 2, 8>(g, st);
-    return ty == 4 ? launch_fused_t<float, 1, 4>(g, st) : launch_fused_t<float, 1, 8>(g, st);
+    if (g->order == 2) {
+      if (ty == 4) return launch_fused_t<float, 2, 4>(g, st);
+      return ty == 16 ? launch_fused_t<float, 2, 16>(g, st) : launch_fused_t<float, 2, 8>(g, st);
+    }
+    if (ty == 4) return launch_fused_t<float, 1, 4>(g, st);
+    return ty == 16 ? launch_fused_t<float, 1, 16>(g, st) : launch_fused_t<float, 1, 8>(g, st);
   }
   if (g->order == 2) return launch_fused_t<double, 2, 4>(g, st);
   return launch_fused_t<double, 1, 4>(g, st);
@@ -895,8 +902,10 @@ static int ensemble_step_once(const hp_ensemble* e, hipStream_t st) {
   const bool f32 = e->dtype == HP_F32;
   const int ty = e->tile_rows;
   if (f32) {
-    if (e->order == 2) (ty == 4 ? launch_flux_multi<float, 2, 4> : launch_flux_multi<float, 2, 8>)(e, st);
-    else (ty == 4 ? launch_flux_multi<float, 1, 4> : launch_flux_multi<float, 1, 8>)(e, st);
+    if (e->order == 2)
+      (ty == 4 ? launch_flux_multi<float, 2, 4> : ty == 16 ? launch_flux_multi<float, 2, 16> : launch_flux_multi<float, 2, 8>)(e, st);
+    else
+      (ty == 4 ? launch_flux_multi<float, 1, 4> : ty == 16 ? launch_flux_multi<float, 1, 16> : launch_flux_multi<float, 1, 8>)(e, st);
   } else {
     if (e->order == 2) launch_flux_multi<double, 2, 4>(e, st);
     else launch_flux_multi<double, 1, 4>(e, st);

@@ -13,7 +13,7 @@ import torch
 
 from .grid import GridSolver
 
-CHOICES = {torch.float32: (4, 8), torch.float64: (4,)}
+CHOICES = {torch.float32: (4, 8, 16), torch.float64: (4,)}
 _cache: dict = {}
 
 

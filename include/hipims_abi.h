@@ -133,7 +133,7 @@ typedef struct hp_grid_desc {
   int32_t cell0_x, cell0_y; /* first valid cell in row-major order (-1: none) */
   int32_t n_classes;        /* entries of manning */
   int32_t rain_mode;        /* hp_rain_mode */
-  int32_t tile_rows;        /* flux tile height: float 4 or 8, double 2 or 4 (0: 8 / 4) */
+  int32_t tile_rows;        /* flux tile height: float 4, 8 or 16, double 4 (0: 8 / 4) */
   int32_t reserved_i32;
   /* state planes (dtype) */
   void *h, *qx, *qy, *z, *h_max;
