@@ -384,6 +384,7 @@ int hp_adjoint_step(const hp_grid_desc* g, const hp_adjoint_desc* ad, void* stre
     return fail(HP_EINVAL, "an adjoint plane pointer is null");
   if (ad->cpitch < (int64_t)g->nx + 1) return fail(HP_EINVAL, "cpitch must be >= nx + 1");
   if (g->n_classes > kGradStride - 1) return fail(HP_EINVAL, "the adjoint supports at most 15 Manning classes");
+  if (g->infil_ks) return fail(HP_EINVAL, "the adjoint does not cover infiltration");
   int rc = grid_flux_launch(g, (hipStream_t)stream);  // recompute the step's flux on U_k
   if (rc) return rc;
   if (g->dtype == HP_F32) return launch_adjoint<float>(g, ad, (hipStream_t)stream);

@@ -194,6 +194,21 @@ __global__ void k_euler_compact(int64_t K, const int32_t* __restrict__ wet, cons
   wl[i] = hi + zi;
 }
 
+// infiltrationCUDA_Kernel.cu:20-46, one thread per listed cell.
+template <class T>
+__global__ void k_infiltration_compact(int64_t M, const int32_t* __restrict__ wet, T* __restrict__ dh,
+                                       const uint8_t* __restrict__ landuse, const T* __restrict__ h,
+                                       const T* __restrict__ ks, const T* __restrict__ phi,
+                                       const T* __restrict__ dtheta, T* __restrict__ F, const T* __restrict__ dtp) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= M) return;
+  const int32_t i = wet[j];
+  const int c = landuse[i];
+  const T dF = green_ampt<T>(h[i], F[i], ks[c], phi[c], dtheta[c], dtp[0]);
+  F[i] += dF;
+  dh[i] -= dF;
+}
+
 static int check_dtype(int dtype) {
   if (dtype != HP_F32 && dtype != HP_F64) return fail(HP_EINVAL, "dtype must be HP_F32 or HP_F64");
   return HP_OK;
@@ -351,6 +366,26 @@ int hp_euler_update_compact(int dtype, int64_t K, const int32_t* wet, const void
                        (const double*)dqx, (const double*)dqy, (const double*)dz, (double*)h, (double*)wl, (double*)z,
                        (double*)qx, (double*)qy);
   return check_launch("k_euler_compact");
+}
+
+int hp_infiltration_compact(int dtype, int64_t M, const int32_t* wet, void* dh, const uint8_t* landuse,
+                            const void* h, const void* ks, const void* phi, const void* dtheta, void* F,
+                            const void* dt, void* stream) {
+  if (int rc = check_dtype(dtype)) return rc;
+  if (M < 0) return fail(HP_EINVAL, "M must be >= 0");
+  if (M == 0) return HP_OK;
+  if (!wet || !dh || !landuse || !h || !ks || !phi || !dtheta || !F || !dt) return fail(HP_EINVAL, "null pointer");
+  hipStream_t st = (hipStream_t)stream;
+  const unsigned nb = nblocks(M, 1024);
+  if (dtype == HP_F32)
+    hipLaunchKernelGGL(k_infiltration_compact<float>, dim3(nb), dim3(1024), 0, st, M, wet, (float*)dh, landuse,
+                       (const float*)h, (const float*)ks, (const float*)phi, (const float*)dtheta, (float*)F,
+                       (const float*)dt);
+  else
+    hipLaunchKernelGGL(k_infiltration_compact<double>, dim3(nb), dim3(1024), 0, st, M, wet, (double*)dh, landuse,
+                       (const double*)h, (const double*)ks, (const double*)phi, (const double*)dtheta, (double*)F,
+                       (const double*)dt);
+  return check_launch("k_infiltration_compact");
 }
 
 }  // extern "C"

@@ -277,6 +277,8 @@ template <class T> struct CellIn {
   T rain;   // rain rate (m/s)
   T hmax;   // h_max before the step
   T n;      // Manning n of the cell's class
+  T F;      // cumulative infiltrated depth (when infiltration is on)
+  int cls;  // landuse class
   bool on;  // landuse class != 255 (the reference's update covers the cell)
 };
 
@@ -288,15 +290,15 @@ template <class T> struct CellIn {
 // updated).  Every output is assigned on every path, so the caller folds
 // them in unconditionally and the accumulators stay in registers.
 template <class T>
-__device__ inline void update_state(const CellIn<T>& in, T& h, T& qx, T& qy, T dhf, T dqx, T dqy, T dt, T cfl_dx,
-                                    T& hmax, T& cand, T& vol_h, T& flux_h, T& rain_h) {
+__device__ inline void update_state(const CellIn<T>& in, T& h, T& qx, T& qy, T dhf, T dqx, T dqy, T dF, T dt,
+                                    T cfl_dx, T& hmax, T& cand, T& vol_h, T& flux_h, T& rain_h) {
   cand = T(INFINITY);
   vol_h = T(0);
   flux_h = T(0);
   rain_h = T(0);
   if (!in.on) return;
   const T rdt = in.rain * dt;
-  const T dht = dhf + rdt;
+  const T dht = (dhf + rdt) - dF;  // flux, + rain, - infiltration (the reference's order)
   if (!(fabs(dht) > T(0) || h >= T(0))) {
     vol_h = h;
     return;
@@ -354,14 +356,15 @@ __device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by, Partial*
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
       const int gy = y0 + wv + 4 * k;
-      cin[k] = CellIn<T>{T(0), T(0), T(0), false};
+      cin[k] = CellIn<T>{T(0), T(0), T(0), T(0), 0, false};
       if (gx < nx && gy < ny) {
         const int64_t gi = (int64_t)gy * pitch + gx;
         const int cls = a.landuse[gi];
         cin[k].on = cls != 255;
-        cin[k].n = T(cls);  // class id until the Manning table is in LDS
+        cin[k].cls = cls;
         cin[k].rain = cell_rain<T>(a, rr, gi);
         cin[k].hmax = ((const T*)a.h_max)[gi];
+        if (a.infil_ks) cin[k].F = ((const T*)a.infil_F)[gi];
       }
     }
   }
@@ -445,6 +448,7 @@ __device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by, Partial*
     if (gy >= ny || gx >= nx) continue;
     const int r = ly + 2, c = lx + 2;
     T ah = T(0), aqx = T(0), aqy = T(0);
+    bool listed = false;
     if (s.ty[r][c] != 0) {
       // fluxMask rule: t == 0, wet, or a wet valid neighbour E, N, W, S that
       // is not compact cell 0 (fluxMaskGenerator_Kernel.cu:36); evaluated
@@ -453,7 +457,7 @@ __device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by, Partial*
       const bool w_n = (s.ty[r - 1][c] != 0) & (s.h[r - 1][c] > hs) & !(gx == a.cell0_x && gy - 1 == a.cell0_y);
       const bool w_w = (s.ty[r][c - 1] != 0) & (s.h[r][c - 1] > hs) & !(gx - 1 == a.cell0_x && gy == a.cell0_y);
       const bool w_s = (s.ty[r + 1][c] != 0) & (s.h[r + 1][c] > hs) & !(gx == a.cell0_x && gy + 1 == a.cell0_y);
-      const bool listed = t_zero | (s.h[r][c] > hs) | w_e | w_n | w_w | w_s;
+      listed = t_zero | (s.h[r][c] > hs) | w_e | w_n | w_w | w_s;
       if (listed) {
         const T* eE = s.e[ly * (kTX + 1) + lx + 1];         // this cell is A
         const T* eN = s.e[Tile::NX + ly * kTX + lx];        // this cell is B
@@ -473,10 +477,17 @@ __device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by, Partial*
       dqy[gi] = aqy;
     } else {
       CellIn<T> in = cin[k];
-      in.n = s_man[(int)in.n];
+      in.n = s_man[in.cls];
       T hn = s.h[r][c], qxn = s.qx[r][c], qyn = s.qy[r][c], hm = in.hmax;
+      // infiltration over the flux list (SWE_CUDA.py:439-443)
+      T dF = T(0);
+      if (a.infil_ks && listed) {
+        dF = green_ampt<T>(hn, in.F, ((const T*)a.infil_ks)[in.cls], ((const T*)a.infil_phi)[in.cls],
+                           ((const T*)a.infil_dtheta)[in.cls], dt);
+        ((T*)a.infil_F)[gi] = in.F + dF;
+      }
       T cand, vh, fh, rh;
-      update_state<T>(in, hn, qxn, qyn, ah, aqx, aqy, dt, cfl_dx, hm, cand, vh, fh, rh);
+      update_state<T>(in, hn, qxn, qyn, ah, aqx, aqy, dF, dt, cfl_dx, hm, cand, vh, fh, rh);
       mn = fmin(mn, cand);
       vol += (double)vh;
       fsum += (double)fh;
@@ -521,6 +532,45 @@ template <class T, int ORDER, int TY>
 __global__ __launch_bounds__(kThreads) void k_step_fused(hp_grid_desc a, int ntx) {
   const int tile = xcd_band(blockIdx.x, gridDim.x);
   flux_tile<T, ORDER, TY, true>(a, tile % ntx, tile / ntx, (Partial*)a.work + tile);
+}
+
+// Infiltration pass over plane elements: a valid cell listed by the flux
+// rule of the current state (t == 0, wet, or a wet valid neighbour E, N, W,
+// S that is not compact cell 0: the wet list the reference's
+// addInfiltrationSource receives from addFlux) takes the Green-Ampt
+// increment; dF is written for every element (0 elsewhere) so the update
+// pass can subtract it after the rain.
+template <class T>
+__global__ __launch_bounds__(kThreads) void k_infiltration(hp_grid_desc a) {
+  if (stopped(a.ctrl)) return;
+  const int64_t pitch = a.pitch, total = pitch * (int64_t)a.ny;
+  const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= total) return;
+  const T* __restrict__ h = (const T*)a.h;
+  const uint8_t* __restrict__ ty = a.type_slot;
+  const int row = (int)(i / pitch), col = (int)(i - row * pitch);
+  const T hs = T(kHs);
+  T dF = T(0);
+  if (col < a.nx && ty[i] != 0) {
+    bool listed = T(a.ctrl->t) == T(0) || h[i] > hs;
+    const int nr[4] = {row, row - 1, row, row + 1};
+    const int nc[4] = {col + 1, col, col - 1, col};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (nr[k] < 0 || nr[k] >= a.ny || nc[k] < 0 || nc[k] >= a.nx) continue;
+      const int64_t j = (int64_t)nr[k] * pitch + nc[k];
+      const bool is0 = nc[k] == a.cell0_x && nr[k] == a.cell0_y;
+      listed = listed || (ty[j] != 0 && !is0 && h[j] > hs);
+    }
+    if (listed) {
+      const int cls = a.landuse[i];
+      T* F = (T*)a.infil_F;
+      dF = green_ampt<T>(h[i], F[i], ((const T*)a.infil_ks)[cls], ((const T*)a.infil_phi)[cls],
+                         ((const T*)a.infil_dtheta)[cls], T(a.ctrl->dt));
+      F[i] += dF;
+    }
+  }
+  ((T*)a.infil_dF)[i] = dF;
 }
 
 // MODE selects which of the reference driver's update methods the pass is:
@@ -574,6 +624,7 @@ __device__ inline void update_chunk(const hp_grid_desc& a, int64_t chunk, Partia
   const uint8_t* __restrict__ lu = a.landuse;
   const int16_t* __restrict__ st = a.station;
   const float* __restrict__ rf = a.rain_field;
+  const T* __restrict__ dfp = (const T*)a.infil_dF;
   const T rscale = T(a.rain_scale);
 
   T mn = T(INFINITY);
@@ -588,8 +639,9 @@ __device__ inline void update_chunk(const hp_grid_desc& a, int64_t chunk, Partia
     VecT<T> vqx = ldv<VecT<T>>(qxp + i0);
     VecT<T> vqy = ldv<VecT<T>>(qyp + i0);
     VecT<T> vhm = ldv<VecT<T>>(hmp + i0);
-    VecT<T> vdh{}, vdqx{}, vdqy{}, vr{};
+    VecT<T> vdh{}, vdqx{}, vdqy{}, vr{}, vdf{};
     if (MODE != HP_UPDATE_CFL) {
+      if (dfp) vdf = ldv<VecT<T>>(dfp + i0);
       vdh = ldv<VecT<T>>(dhp + i0);
       vdqx = ldv<VecT<T>>(dqxp + i0);
       vdqy = ldv<VecT<T>>(dqyp + i0);
@@ -619,7 +671,7 @@ __device__ inline void update_chunk(const hp_grid_desc& a, int64_t chunk, Partia
       }
       const T dhf = vdh.x[j];
       const T rdt = vr.x[j] * dt;
-      const T dht = dhf + rdt;
+      const T dht = (dhf + rdt) - vdf.x[j];  // flux, + rain, - infiltration (the reference's order)
       const bool listed = (MODE == HP_UPDATE_EULER) ? (fabs(dht) > T(0) || h > hs) : (fabs(dht) > T(0) || h >= T(0));
       if (!listed) {
         vol += (double)h;
@@ -790,6 +842,10 @@ static int check_desc(const hp_grid_desc* g) {
   if ((g->given_depth && (g->gd_rows < 1 || g->gd_w < 1)) || (g->given_wl && (g->gwl_rows < 1 || g->gwl_w < 1)) ||
       (g->given_q && (g->gq_rows < 1 || g->gq_w < 1)))
     return fail(HP_EINVAL, "boundary series tables need >= 1 row and column");
+  if (g->infil_ks && (!g->infil_phi || !g->infil_dtheta || !g->infil_F))
+    return fail(HP_EINVAL, "infiltration needs infil_phi, infil_dtheta and infil_F");
+  if ((uintptr_t)g->infil_dF % ((uintptr_t)kVec * (g->dtype == HP_F32 ? 4 : 8)))
+    return fail(HP_EINVAL, "infil_dF must be aligned to 4 elements");
   if (flux_tiles(g) > 0x7fffffffLL || update_blocks(g) > 0x7fffffffLL) return fail(HP_EINVAL, "domain too large");
   return HP_OK;
 }
@@ -835,6 +891,14 @@ static int launch_step_fused(const hp_grid_desc* g, hipStream_t st) {
   }
   if (g->order == 2) return launch_fused_t<double, 2, 4>(g, st);
   return launch_fused_t<double, 1, 4>(g, st);
+}
+
+static int launch_infiltration(const hp_grid_desc* g, hipStream_t st) {
+  const int64_t total = g->pitch * (int64_t)g->ny;
+  const unsigned nb = (unsigned)((total + kThreads - 1) / kThreads);
+  if (g->dtype == HP_F32) hipLaunchKernelGGL(k_infiltration<float>, dim3(nb), dim3(kThreads), 0, st, *g);
+  else hipLaunchKernelGGL(k_infiltration<double>, dim3(nb), dim3(kThreads), 0, st, *g);
+  return check_launch("k_infiltration");
 }
 
 template <int MODE>
@@ -995,14 +1059,28 @@ int hp_grid_update_mode(const hp_grid_desc* g, int mode, void* stream) {
   return launch_update(g, mode, (hipStream_t)stream);
 }
 
+int hp_grid_infiltration(const hp_grid_desc* g, void* stream) {
+  int rc = check_desc(g);
+  if (rc) return rc;
+  if (!g->infil_ks || !g->infil_dF) return fail(HP_EINVAL, "infiltration needs infil_ks and an infil_dF plane");
+  return launch_infiltration(g, (hipStream_t)stream);
+}
+
 int hp_grid_step(const hp_grid_desc* g, int32_t nsteps, void* stream) {
   int rc = check_desc(g);
   if (rc) return rc;
   if (nsteps < 0) return fail(HP_EINVAL, "nsteps must be >= 0");
+  if (g->infil_ks && !g->infil_dF) return fail(HP_EINVAL, "a step with infiltration needs an infil_dF plane");
+  hp_grid_desc d = *g;
+  if (!d.infil_ks) d.infil_dF = nullptr;  // nothing to subtract
   for (int32_t k = 0; k < nsteps; ++k) {
-    rc = launch_flux(g, (hipStream_t)stream);
+    rc = launch_flux(&d, (hipStream_t)stream);
     if (rc) return rc;
-    rc = launch_update(g, HP_UPDATE_STEP, (hipStream_t)stream);
+    if (d.infil_ks) {
+      rc = launch_infiltration(&d, (hipStream_t)stream);
+      if (rc) return rc;
+    }
+    rc = launch_update(&d, HP_UPDATE_STEP, (hipStream_t)stream);
     if (rc) return rc;
   }
   return HP_OK;
@@ -1017,6 +1095,7 @@ int hp_ensemble_init(int32_t n, const hp_grid_desc* descs, void* work, int64_t w
                      void* stream) {
   if (n <= 0 || !descs || !out) return fail(HP_EINVAL, "need n > 0 descriptors and an output handle");
   for (int k = 0; k < n; ++k) {
+    if (descs[k].infil_ks) return fail(HP_EINVAL, "infiltration is not supported in an ensemble");
     int rc = check_desc(&descs[k]);
     if (rc) return fail(rc, "domain " + std::to_string(k) + ": " + last_error_slot());
     if (descs[k].dtype != descs[0].dtype || descs[k].order != descs[0].order ||

@@ -347,6 +347,17 @@ __host__ __device__ inline void implicit_friction(T h, T n, T dt, T& qx, T& qy) 
   }
 }
 
+// Green-Ampt increment of one listed cell (infiltrationCUDA_Kernel.cu:30-44):
+// F1 = (A + sqrt(A^2 + 4 dt Ks (phi + h) dtheta)) / 2 with A = F0 + dt Ks, the
+// positive root of F^2 - A F - dt Ks (phi + h) dtheta = 0; the increment is
+// capped by the depth.
+template <class T>
+__host__ __device__ inline T green_ampt(T h, T F0, T ks, T phi, T dtheta, T dt) {
+  const T a = F0 + dt * ks;
+  const T F1 = T(0.5) * (a + Prec<T>::sqrt_(a * a + T(4) * dt * ks * (phi + h) * dtheta));
+  return fmin(h, F1 - F0);
+}
+
 // CFL time-step candidate of one updated cell (timeControl_Kernel.cu:186-189):
 // the square root is single precision in both instantiations, as in the
 // reference.

@@ -20,8 +20,9 @@ Differences, each documented where it matters:
     state dtype);
   * rasters must carry a ring of nodata (the reference's torch.roll neighbour
     table wraps around otherwise).
-Sediment, infiltration and the explicit-friction variants are outside this
-library and raise NotImplementedError.
+Green-Ampt infiltration (set__infiltrationField_tensor / addInfiltrationSource)
+is supported; sediment transport and the explicit-friction variants are
+outside this library and raise NotImplementedError.
 """
 from __future__ import annotations
 
@@ -305,10 +306,30 @@ class Godunov:
                           ("wl", self._wl_internal), ("h_max", self._h_max)):
             torch.save(val, os.path.join(self._outpath, f"{name}_{tag}.pt"))
 
+    # -- infiltration (SWE_CUDA.py:236-238, 311-337, 439-443) -----------------------
+    def add_cumulativeWaterDepth_Field(self, device):
+        self._cum_depth_requested = True
+
+    def set__infiltrationField_tensor(self, hydraulic_conductivity, capillary_head, water_content_diff, device):
+        """Per-landuse-class Green-Ampt parameters; the cumulative infiltrated
+        depth starts at 0."""
+        self.add_cumulativeWaterDepth_Field(device)
+        conv = lambda v: v.detach().cpu().numpy() if torch.is_tensor(v) else v
+        self.solver.set_infiltration(conv(hydraulic_conductivity), conv(capillary_head), conv(water_content_diff))
+
+    def addInfiltrationSource(self):
+        """Green-Ampt over this step's flux list, subtracted from the step's
+        depth change after the rain (the reference calls it after
+        addStation_PrecipitationSource, CatchFlood_main.py:164-166)."""
+        self.solver.infiltration()
+
+    @property
+    def _cumulativeWaterDepth(self):
+        return self.solver.compact(self.solver.F)
+
     # -- outside this library ------------------------------------------------------
     def _unsupported(self, *a, **k):
-        raise NotImplementedError("sediment, infiltration and explicit-friction paths are not part of this library")
+        raise NotImplementedError("sediment and explicit-friction paths are not part of this library")
 
-    add_concentrationField = add_cumulativeWaterDepth_Field = _unsupported
-    set__infiltrationField_tensor = addInfiltrationSource = addFriction = _unsupported
+    add_concentrationField = addFriction = _unsupported
     enableSediModule = add_sedi_source = sedi_euler_update = run_updating = _unsupported

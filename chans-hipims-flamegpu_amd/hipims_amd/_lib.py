@@ -22,7 +22,7 @@ HP_OK, HP_EINVAL, HP_ELAUNCH, HP_ESPACE = 0, -1, -2, -3
 HP_F32, HP_F64 = 0, 1
 HP_RAIN_NONE, HP_RAIN_STATION, HP_RAIN_FIELD = 0, 1, 2
 HP_UPDATE_STEP, HP_UPDATE_FRICTION, HP_UPDATE_EULER, HP_UPDATE_CFL = 0, 1, 2, 3
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 _P = ctypes.c_void_p
 _I64 = ctypes.c_int64
@@ -58,6 +58,7 @@ class HpGridDesc(ctypes.Structure):
         ("station", _P), ("rain_table", _P), ("rain_rows", _I32), ("rain_cols", _I32),
         ("rain_field", _P), ("rain_scale", ctypes.c_double),
         ("ctrl", _P), ("work", _P), ("work_bytes", _I64),
+        ("infil_ks", _P), ("infil_phi", _P), ("infil_dtheta", _P), ("infil_F", _P), ("infil_dF", _P),
     ]
 
 
@@ -87,7 +88,9 @@ SIGNATURES = {
     "hp_friction_update_compact": (_I, [_I, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "hp_time_control_compact": (_I, [_I, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "hp_euler_update_compact": (_I, [_I, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "hp_infiltration_compact": (_I, [_I, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "hp_grid_workspace_bytes": (_I64, [ctypes.POINTER(HpGridDesc)]),
+    "hp_grid_infiltration": (_I, [ctypes.POINTER(HpGridDesc), _P]),
     "hp_grid_flux": (_I, [ctypes.POINTER(HpGridDesc), _P]),
     "hp_grid_update": (_I, [ctypes.POINTER(HpGridDesc), _P]),
     "hp_grid_update_mode": (_I, [ctypes.POINTER(HpGridDesc), _I, _P]),

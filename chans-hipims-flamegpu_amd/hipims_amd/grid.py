@@ -146,6 +146,10 @@ class GridSolver:
         self.rain_field = None
         self.rain_mode = _lib.HP_RAIN_NONE
         self.rain_scale = 1.0
+        self.infil = None      # (ks, phi, dtheta) class tables when infiltration is on
+        self.F = None          # cumulative infiltrated depth plane
+        self.dF = None         # the step's infiltrated depth plane (two-pass path)
+        self._infil_pending = False
         self._desc = None
         self.work = None
         self._rebuild()
@@ -233,6 +237,30 @@ class GridSolver:
         self.rain_scale = float(s)
         self._desc = None
 
+    def set_infiltration(self, hydraulic_conductivity, capillary_head, water_content_diff):
+        """Green-Ampt infiltration with per-landuse-class tables (the
+        reference's set__infiltrationField_tensor, SWE_CUDA.py:311-337); the
+        cumulative depth F starts at 0.  step() and step_fused() then include
+        it; with flux()/update() call infiltration() between them."""
+        n = int(self.manning.numel())
+
+        def tab(v):
+            a = np.atleast_1d(np.asarray(_to_np(v), dtype=np.float64))
+            if a.size == 1:
+                a = np.full(n, a[0])
+            if a.size < n:
+                raise ValueError("infiltration tables need one entry per landuse class")
+            return torch.tensor(a, dtype=self.dtype, device=self.device)
+        self.infil = (tab(hydraulic_conductivity), tab(capillary_head), tab(water_content_diff))
+        if self.F is None:
+            self.F = torch.zeros_like(self.h)
+            self.dF = torch.zeros_like(self.h)
+        self._desc = None
+
+    def clear_infiltration(self):
+        self.infil = None
+        self._desc = None
+
     # -- descriptor ----------------------------------------------------------
     def _rebuild(self):
         L = self.layout
@@ -260,6 +288,9 @@ class GridSolver:
         d.rain_field = _lib.ptr(self.rain_field)
         d.rain_scale = self.rain_scale
         d.ctrl = self.ctrl.data_ptr()
+        if self.infil is not None:
+            d.infil_ks, d.infil_phi, d.infil_dtheta = (t.data_ptr() for t in self.infil)
+            d.infil_F = self.F.data_ptr()
         need = int(lib().hp_grid_workspace_bytes(ctypes.byref(d)))
         if self.work is None or self.work.numel() < need:
             self.work = torch.empty(max(need, 256), dtype=torch.uint8, device=self.device)
@@ -280,19 +311,37 @@ class GridSolver:
     def flux(self):
         check(lib().hp_grid_flux(ctypes.byref(self.desc), self._stream()), "hp_grid_flux")
 
+    def infiltration(self):
+        """Infiltration pass over the flux list of the current state; the next
+        update() subtracts it (Godunov.addInfiltrationSource)."""
+        if self.infil is None:
+            raise RuntimeError("set_infiltration() first")
+        d = self.desc
+        d.infil_dF = self.dF.data_ptr()
+        check(lib().hp_grid_infiltration(ctypes.byref(d), self._stream()), "hp_grid_infiltration")
+        self._infil_pending = True
+
     def update(self, mode: int = _lib.HP_UPDATE_STEP, rain: bool = True):
         """One update pass; `rain` False leaves the configured rain source out."""
         d = self.desc
         saved = d.rain_mode
         if not rain:
             d.rain_mode = _lib.HP_RAIN_NONE
+        d.infil_dF = self.dF.data_ptr() if self._infil_pending else None
         try:
             check(lib().hp_grid_update_mode(ctypes.byref(d), int(mode), self._stream()), "hp_grid_update_mode")
         finally:
             d.rain_mode = saved
+            d.infil_dF = None
+            self._infil_pending = False
 
     def step(self, nsteps: int = 1):
-        check(lib().hp_grid_step(ctypes.byref(self.desc), int(nsteps), self._stream()), "hp_grid_step")
+        d = self.desc
+        d.infil_dF = self.dF.data_ptr() if self.infil is not None else None
+        try:
+            check(lib().hp_grid_step(ctypes.byref(d), int(nsteps), self._stream()), "hp_grid_step")
+        finally:
+            d.infil_dF = None
 
     def step_fused(self, nsteps: int = 1):
         """nsteps single-pass steps (same results as step(); dh/dqx/dqy untouched)."""

@@ -13,6 +13,7 @@ reference's bindings do (TORCH_CHECK is_cuda / is_contiguous -> RuntimeError):
                                            friction_implicit_andUpdate_jh_interface.cpp:29-55
   timeControl.updateTimestep ............. timeControl.cpp:27-45
   euler_update.update .................... euler_update.cpp:27-47
+  infiltrationCalculation.addinfiltration  infiltration_interface.cpp:21-53
 
 (paths under CHANS-HiPIMS/hipims/cuda/).  All launch on torch's current
 stream of the tensors' device, asynchronously.
@@ -154,3 +155,22 @@ def euler_update(updateMask, h_update, qx_update, qy_update, z_update, h, wl, z,
             _lib.dtype_code(h.dtype), updateMask.numel(), updateMask.data_ptr(), h_update.data_ptr(),
             qx_update.data_ptr(), qy_update.data_ptr(), z_update.data_ptr(), h.data_ptr(), wl.data_ptr(),
             z.data_ptr(), qx.data_ptr(), qy.data_ptr(), _stream(h)), "euler_update.update")
+
+
+def add_infiltration(wetMask, h_update, landuse, h, hydraulic_conductivity, capillary_head, water_content_diff,
+                     cumulative_depth, dt):
+    """infiltrationCalculation.addinfiltration(wetMask, h_update, landuse, h, hydraulic_conductivity,
+    capillary_head, water_content_diff, cumulative_depth, dt): Green-Ampt over the listed cells,
+    h_update -= dF, cumulative_depth += dF."""
+    _check_input(wetMask=wetMask, landuse=landuse, h=h, h_update=h_update, dt=dt,
+                 hydraulic_conductivity=hydraulic_conductivity, capillary_head=capillary_head,
+                 water_content_diff=water_content_diff, cumulative_depth=cumulative_depth)
+    _idx("wetMask", wetMask)
+    _idx("landuse", landuse, torch.uint8)
+    _same_dtype(h, h_update=h_update, hydraulic_conductivity=hydraulic_conductivity, capillary_head=capillary_head,
+                water_content_diff=water_content_diff, cumulative_depth=cumulative_depth, dt=dt)
+    with torch.cuda.device(h.device):
+        check(lib().hp_infiltration_compact(
+            _lib.dtype_code(h.dtype), wetMask.numel(), wetMask.data_ptr(), h_update.data_ptr(), landuse.data_ptr(),
+            h.data_ptr(), hydraulic_conductivity.data_ptr(), capillary_head.data_ptr(), water_content_diff.data_ptr(),
+            cumulative_depth.data_ptr(), dt.data_ptr(), _stream(h)), "addinfiltration")

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define HP_ABI_VERSION 1
+#define HP_ABI_VERSION 2
 
 enum hp_status {
   HP_OK = 0,
@@ -89,6 +89,15 @@ int hp_time_control_compact(int dtype, int64_t K, const int32_t* wet, void* acce
 int hp_euler_update_compact(int dtype, int64_t K, const int32_t* wet, const void* dh, const void* dqx,
                             const void* dqy, const void* dz, void* h, void* wl, void* z, void* qx, void* qy,
                             void* stream);
+
+/* Replaces infiltrationCalculation.addinfiltration (infiltration_interface.cpp,
+ * kernel infiltrationCUDA_Kernel.cu:20-46): Green-Ampt over the M cells of
+ * wet; per landuse class saturated conductivity ks, capillary head phi and
+ * water-content deficit dtheta; F is the cumulative infiltrated depth;
+ * dh -= dF, F += dF. */
+int hp_infiltration_compact(int dtype, int64_t M, const int32_t* wet, void* dh, const uint8_t* landuse,
+                            const void* h, const void* ks, const void* phi, const void* dtheta, void* F,
+                            const void* dt, void* stream);
 
 /* ------------------------------------------------------------------------
  * 2. Structured-grid path.
@@ -161,6 +170,16 @@ typedef struct hp_grid_desc {
   hp_ctrl* ctrl;
   void* work;
   int64_t work_bytes;
+  /* Green-Ampt infiltration (Godunov.addInfiltrationSource, SWE_CUDA.py:439-443);
+   * infil_ks == NULL disables it.  ks, phi, dtheta: [n_classes] (dtype) per
+   * landuse class; infil_F: cumulative infiltrated depth plane (dtype);
+   * infil_dF: the step's infiltrated depth plane (dtype), written by
+   * hp_grid_infiltration; an update pass subtracts it when non-null. */
+  const void* infil_ks;
+  const void* infil_phi;
+  const void* infil_dtheta;
+  void* infil_F;
+  void* infil_dF;
 } hp_grid_desc;
 
 /* Bytes of device workspace `work` must provide for this domain. */
@@ -193,7 +212,13 @@ int hp_grid_update_mode(const hp_grid_desc* g, int mode, void* stream);
  * hp_grid_flux + hp_grid_update (HP_UPDATE_STEP); dh/dqx/dqy are not written. */
 int hp_grid_step_fused(const hp_grid_desc* g, int32_t nsteps, void* stream);
 
-/* nsteps x (flux, update). */
+/* Infiltration pass: cells listed by the flux rule of the current state
+ * (the wet list of the flux pass) take the Green-Ampt increment; infil_dF
+ * receives it (0 elsewhere) and infil_F accumulates it
+ * (infiltrationCUDA_Kernel.cu:20-46 on the structured grid). */
+int hp_grid_infiltration(const hp_grid_desc* g, void* stream);
+
+/* nsteps x (flux, [infiltration when infil_ks is set], update). */
 int hp_grid_step(const hp_grid_desc* g, int32_t nsteps, void* stream);
 
 /* ---- Ensemble: many domains advanced by the same launches, each with its own

@@ -55,8 +55,9 @@ def lib():
         L.orc_time_control.argtypes = [_I, _I64, _P, _P, _P, _P, _P, _P, _P, _P]
         L.orc_euler_update.argtypes = [_I, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]
         L.orc_run_steps.restype = _I
+        L.orc_infiltration.argtypes = [_I, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P]
         L.orc_run_steps.argtypes = ([_I, _I, _I64, _I, _P] + [_P] * 10 + [_P, _P, _P, _P, _I, _I, _P]
-                                    + [_P, _I, _I, _P, _I, _I, _P, _I, _I, _D, _D, _P, _P])
+                                    + [_P, _I, _I, _P, _I, _I, _P, _I, _I, _D, _D, _P, _P, _P, _P, _P, _P])
         _lib = L
     return _lib
 
@@ -152,6 +153,14 @@ def euler_update(wet, dh, dqx, dqy, dz, h, wl, z, qx, qy):
                            _p(dz), _p(h), _p(wl), _p(z), _p(qx), _p(qy))
 
 
+def infiltration(wet, dh, landuse, h, ks, phi, dth, F, dt):
+    """addInfiltrationSource over the wet list (infiltrationCUDA_Kernel.cu:20-46)."""
+    c = lambda a: np.ascontiguousarray(a, dtype=h.dtype)
+    ks, phi, dth = c(ks), c(phi), c(dth)
+    lib().orc_infiltration(int(h.dtype == np.float64), wet.shape[0], _p(wet), _p(dh), _p(landuse), _p(h), _p(ks),
+                           _p(phi), _p(dth), _p(F), _p(_scalar(dt, h.dtype)))
+
+
 # ---------------------------------------------------------------------------
 # Whole-step simulation in the reference layout
 # ---------------------------------------------------------------------------
@@ -180,6 +189,10 @@ class OracleSim:
     given_depth: np.ndarray | None = None
     given_wl: np.ndarray | None = None
     given_q: np.ndarray | None = None
+    # Green-Ampt per-class tables (set__infiltrationField_tensor); None: off
+    hydraulic_conductivity: np.ndarray | None = None
+    capillary_head: np.ndarray | None = None
+    water_content_diff: np.ndarray | None = None
 
     def __post_init__(self):
         valid = self.mask > 0
@@ -205,6 +218,11 @@ class OracleSim:
         self.gw = np.ascontiguousarray(self.given_wl if self.given_wl is not None else [[0.0, 0.0]], dtype=dt_)
         self.gq = np.ascontiguousarray(self.given_q if self.given_q is not None else [[0.0, 0.0, 0.0]], dtype=dt_)
         self.ctrl = np.array([self.t, self.first_dt, float(self.export_n), self.export_T], dtype=np.float64)
+        self.infil = None
+        if self.hydraulic_conductivity is not None:
+            self.infil = [np.ascontiguousarray(v, dtype=dt_) for v in
+                          (self.hydraulic_conductivity, self.capillary_head, self.water_content_diff)]
+        self.F = np.zeros(self.N, dt_)  # cumulative infiltrated depth
         self.rain_index = ctypes.c_int(0)
         self.last_seconds = 0.0
         self.exports = 0
@@ -228,6 +246,7 @@ class OracleSim:
             ctypes.byref(self.rain_index),
             _p(self.gd), self.gd.shape[0], self.gd.shape[1], _p(self.gw), self.gw.shape[0], self.gw.shape[1],
             _p(self.gq), self.gq.shape[0], self.gq.shape[1], float(self.dx), float(self.cfl),
+            *([_p(v) for v in self.infil] if self.infil else [None, None, None]), _p(self.F),
             _p(self.ctrl), ctypes.byref(secs))
         self.last_seconds = secs.value
         self.exports += n_exp

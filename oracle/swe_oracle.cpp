@@ -15,7 +15,8 @@
 //   friction+update .. cuda/friction_implicit_andUpdate_jh_Kernel.cu:22-146
 //   CFL accelerator .. cuda/timeControl_Kernel.cu:164-198
 //   euler update ..... cuda/euler_update_Kernel.cu:18-36
-//   driver step ...... pythonHipims/SWE_CUDA.py:339-424, 472-502, 780-831
+//   infiltration ..... cuda/infiltrationCUDA_Kernel.cu:20-46
+//   driver step ...... pythonHipims/SWE_CUDA.py:339-424, 436-446, 472-502, 780-831
 //
 // Every per-cell loop writes only the cell it owns, as the reference kernels
 // do, so the OpenMP loops are race-free.  Floating-point contraction is off
@@ -25,7 +26,8 @@
 // The FP32 instantiation computes in pure float; the reference's FP32 kernels
 // mix in double through `auto` and double literals (e.g.
 // fluxCal_jh_modified_surface_Kernel.cu:181-185).  That mix is NOT restated:
-// FP32 oracle results are parity unpinned and are used only for timing.
+// FP32 oracle results are parity unpinned; they serve as CPU timing and as
+// the accuracy bar of float arithmetic run through the reference's algorithm.
 
 #include <algorithm>
 #include <cmath>
@@ -438,6 +440,26 @@ void euler_update(int64_t K, const int32_t* wet, const T* dh, const T* dqx, cons
   }
 }
 
+// Green-Ampt infiltration over the wet list, infiltrationCUDA_Kernel.cu:20-46:
+// F1 = (A + sqrt(A^2 + 4 dt Ks (phi + h) dtheta)) / 2 with A = F0 + dt Ks,
+// dF = min(h, F1 - F0); F += dF; dh -= dF.  Parameters are per landuse class.
+template <class T>
+void infiltration(int64_t M, const int32_t* wet, T* dh, const uint8_t* landuse, const T* h, const T* ks,
+                  const T* phi, const T* dth, T* F, const T* dtp) {
+  const T dt = *dtp;
+#pragma omp parallel for schedule(static)
+  for (int64_t j = 0; j < M; ++j) {
+    const int32_t i = wet[j];
+    const int c = landuse[i];
+    const T K = ks[c], head = phi[c] + h[i], d = dth[c], F0 = F[i];
+    const T a = F0 + dt * K;
+    const T F1 = T(0.5) * (a + std::sqrt(a * a + T(4) * dt * K * head * d));
+    const T dF = std::fmin(h[i], F1 - F0);
+    F[i] += dF;
+    dh[i] -= dF;
+  }
+}
+
 }  // namespace
 
 // The normal table used when a caller passes none: SWE_CUDA.py:185-188.
@@ -464,6 +486,8 @@ template <class T> struct Sim {
   const uint8_t* landuse;
   const T* manning;
   const int16_t* station;      // may be null (no rain)
+  const T *ks, *phi, *dth;     // Green-Ampt class tables; null: no infiltration
+  T* F;                        // cumulative infiltrated depth
   const double* rain_table;    // [rows, 1+S] in m/s (SWE_CUDA.py:472-495)
   int rain_rows, rain_cols;
   int order;
@@ -526,6 +550,10 @@ template <class T> void sim_step(Sim<T>& s) {
     rain_interp<T>(s);
     station_rain<T>(N, s.dh, s.station, s.rain_now.data(), 1, &s.dt);
   }
+  // addInfiltrationSource over the flux wet list (SWE_CUDA.py:439-443,
+  // called after the rain in CatchFlood_main.py:164-166)
+  if (s.ks && !s.list.empty())
+    infiltration<T>((int64_t)s.list.size(), s.list.data(), s.dh, s.landuse, s.h, s.ks, s.phi, s.dth, s.F, &s.dt);
   // time_friction_euler_update_cuda
   const T dt_old = s.dt;
   s.list.clear();
@@ -648,6 +676,16 @@ void orc_euler_update(int is_f64, int64_t K, const int32_t* wet, const void* dh,
                         (float*)h, (float*)wl, (float*)z, (float*)qx, (float*)qy);
 }
 
+void orc_infiltration(int is_f64, int64_t M, const int32_t* wet, void* dh, const uint8_t* landuse, const void* h,
+                      const void* ks, const void* phi, const void* dth, void* F, const void* dt) {
+  if (is_f64)
+    infiltration<double>(M, wet, (double*)dh, landuse, (const double*)h, (const double*)ks, (const double*)phi,
+                         (const double*)dth, (double*)F, (const double*)dt);
+  else
+    infiltration<float>(M, wet, (float*)dh, landuse, (const float*)h, (const float*)ks, (const float*)phi,
+                        (const float*)dth, (float*)F, (const float*)dt);
+}
+
 // Run `nsteps` whole steps of the server sequence on caller-owned compact
 // arrays.  ctrl (in/out, double[4]) = {t, dt, export_n, export_T}.
 // rain_table may be null (no rain).  Returns the number of export clamps hit,
@@ -657,7 +695,8 @@ int orc_run_steps(int is_f64, int order, int64_t N, int nsteps, const int32_t* i
                   const uint8_t* landuse, const void* manning, const int16_t* station, const double* rain_table,
                   int rain_rows, int rain_cols, int* rain_index, const void* gd, int gd_rows, int gd_w,
                   const void* gwl, int gwl_rows, int gwl_w, const void* gq, int gq_rows, int gq_w, double dx,
-                  double cfl, double* ctrl, double* seconds) {
+                  double cfl, const void* ks, const void* phi, const void* dth, void* F, double* ctrl,
+                  double* seconds) {
   auto run = [&](auto tag) -> int {
     using T = decltype(tag);
     Sim<T> s;
@@ -665,6 +704,7 @@ int orc_run_steps(int is_f64, int order, int64_t N, int nsteps, const int32_t* i
     s.h = (T*)h; s.wl = (T*)wl; s.qx = (T*)qx; s.qy = (T*)qy; s.z = (T*)z; s.h_max = (T*)h_max;
     s.dh = (T*)dh; s.dqx = (T*)dqx; s.dqy = (T*)dqy; s.dz = (T*)dz;
     s.landuse = landuse; s.manning = (const T*)manning; s.station = station;
+    s.ks = (const T*)ks; s.phi = (const T*)phi; s.dth = (const T*)dth; s.F = (T*)F;
     s.rain_table = rain_table; s.rain_rows = rain_rows; s.rain_cols = rain_cols;
     s.order = order;
     s.tb = Tables{gd, gd_rows, gd_w, gwl, gwl_rows, gwl_w, gq, gq_rows, gq_w};
