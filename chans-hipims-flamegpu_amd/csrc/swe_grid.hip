@@ -113,11 +113,16 @@ template <class T, int TY> struct FluxTile {
   static constexpr int NY = (TY + 1) * kTX;  // y-edges
   T h[LH][LW], qx[LH][LW], qy[LH][LW], z[LH][LW];
   uint8_t ty[LH][LW];
-  // per edge: 0 mass, 1 qx, 2 qy (from side A's view, to subtract at A and
-  // add at B), 3 bed term of A, 4 bed term of B (each to subtract along the
-  // edge axis).  A = west / north cell, B = east / south cell.
-  // x-edge (ly, lx) is record ly*(TX+1)+lx; y-edge (ly, lx) is NX + ly*TX+lx.
-  T e[NX + NY][5];
+  // Edge records, A = west / north cell, B = east / south cell, fluxes from
+  // A's view (subtract at A, add at B); x-edge (ly, lx) is record
+  // ly*(TX+1)+lx, y-edge (ly, lx) is NX + ly*TX+lx.
+  //   double (kRec 5): 0 mass, 1 qx, 2 qy, 3 bed term of A, 4 bed term of B
+  //     (the reference's accumulation order: flux terms, then the bed term);
+  //   float (kRec 4): 0 mass, 1 momentum across the edge axis, 2 momentum
+  //     along the axis + A's bed term, 3 the same - B's bed term (one fewer
+  //     LDS word per edge, so one more block fits a CU).
+  static constexpr int kRec = sizeof(T) == 4 ? 4 : 5;
+  T e[NX + NY][kRec];
 };
 
 template <class T, int TY>
@@ -144,6 +149,24 @@ __device__ inline FrameCell<T> frame_cell_d(const FluxTile<T, TY>& s, int r, int
   return f;
 }
 
+// Write one edge record (layouts in FluxTile).  gx, gy: global momentum flux
+// from A's view; bedA / bedB: the two sides' bed terms along the edge axis.
+template <class T, int D, int kRec>
+__device__ inline void store_record(T* out, T fh, T gx, T gy, T bedA, T bedB) {
+  out[0] = fh;
+  if (kRec == 5) {
+    out[1] = gx;
+    out[2] = gy;
+    out[3] = bedA;
+    out[4] = bedB;
+  } else {
+    const T axis = D == 0 ? gx : gy;
+    out[1] = D == 0 ? gy : gx;
+    out[2] = axis + bedA;
+    out[3] = axis - bedB;
+  }
+}
+
 // Solve the edge between LDS cells A and B, where B lies in direction D from
 // A (D = 0: x-edge, B east of A; D = 3: y-edge, B south of A).  A is
 // (ra, ca); L is A's neighbour opposite to B, RR is B's neighbour beyond B.
@@ -155,7 +178,9 @@ __device__ inline void solve_tile_edge(const FluxTile<T, TY>& s, const hp_grid_d
   const int rb = ra + di, cb = ca + dj;
   const int rl = ra - di, cl = ca - dj;
   const int rr2 = rb + di, cr2 = cb + dj;
-  out[0] = out[1] = out[2] = out[3] = out[4] = T(0);
+  constexpr int kRec = FluxTile<T, TY>::kRec;
+#pragma unroll
+  for (int k = 0; k < kRec; ++k) out[k] = T(0);
   const int tA = s.ty[ra][ca], tB = s.ty[rb][cb];
   if (tA == 0 && tB == 0) return;
   constexpr T nA = T(D == 0 ? 1 : -1);  // A's outward normal component along the edge axis
@@ -169,11 +194,7 @@ __device__ inline void solve_tile_edge(const FluxTile<T, TY>& s, const hp_grid_d
     const FrameCell<T> rr = s.ty[rr2][cr2] ? rc : r;
     const EdgeOut<T> o = solve_edge<T, ORDER>(c, l, r, rr, false, c.h + c.z, r.h + r.z, dx, dt, dt_dx);
     if (!o.active) return;
-    out[0] = o.fh;
-    out[1] = D == 0 ? o.fn : o.ft;
-    out[2] = D == 0 ? o.ft : -o.fn;
-    out[3] = o.bed_c * nA;
-    out[4] = o.bed_r * (-nA);
+    store_record<T, D, kRec>(out, o.fh, D == 0 ? o.fn : o.ft, D == 0 ? o.ft : -o.fn, o.bed_c * nA, o.bed_r * (-nA));
   } else {
     // boundary edge, solved from the valid side with its outward normal
     const bool from_a = tA != 0;
@@ -189,11 +210,8 @@ __device__ inline void solve_tile_edge(const FluxTile<T, TY>& s, const hp_grid_d
     T gx, gy;
     to_global(o.fn, o.ft, d, gx, gy);
     const T bed = o.bed_c * T(axis_x ? nrm_x(d) : nrm_y(d));
-    if (from_a) {
-      out[0] = o.fh; out[1] = gx; out[2] = gy; out[3] = bed;
-    } else {
-      out[0] = -o.fh; out[1] = -gx; out[2] = -gy; out[4] = bed;
-    }
+    if (from_a) store_record<T, D, kRec>(out, o.fh, gx, gy, bed, T(0));
+    else store_record<T, D, kRec>(out, -o.fh, -gx, -gy, T(0), bed);
   }
 }
 
@@ -464,10 +482,17 @@ __device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by, Partial*
         const T* eW = s.e[ly * (kTX + 1) + lx];             // this cell is B
         const T* eS = s.e[Tile::NX + (ly + 1) * kTX + lx];  // this cell is A
         // the reference's order: E, N, W, S; per edge HLLC terms then bed term
-        ah -= eE[0]; aqx -= eE[1]; aqx -= eE[3]; aqy -= eE[2];
-        ah -= -eN[0]; aqx -= -eN[1]; aqy -= -eN[2]; aqy -= eN[4];
-        ah -= -eW[0]; aqx -= -eW[1]; aqx -= eW[4]; aqy -= -eW[2];
-        ah -= eS[0]; aqx -= eS[1]; aqy -= eS[2]; aqy -= eS[3];
+        if (Tile::kRec == 5) {
+          ah -= eE[0]; aqx -= eE[1]; aqx -= eE[3]; aqy -= eE[2];
+          ah -= -eN[0]; aqx -= -eN[1]; aqy -= -eN[2]; aqy -= eN[4];
+          ah -= -eW[0]; aqx -= -eW[1]; aqx -= eW[4]; aqy -= -eW[2];
+          ah -= eS[0]; aqx -= eS[1]; aqy -= eS[2]; aqy -= eS[3];
+        } else {
+          ah -= eE[0]; aqx -= eE[2]; aqy -= eE[1];
+          ah += eN[0]; aqx += eN[1]; aqy += eN[3];
+          ah += eW[0]; aqx += eW[3]; aqy += eW[1];
+          ah -= eS[0]; aqx -= eS[1]; aqy -= eS[2];
+        }
       }
     }
     const int64_t gi = (int64_t)gy * pitch + gx;
