@@ -351,11 +351,24 @@ __host__ __device__ inline void implicit_friction(T h, T n, T dt, T& qx, T& qy) 
 // F1 = (A + sqrt(A^2 + 4 dt Ks (phi + h) dtheta)) / 2 with A = F0 + dt Ks, the
 // positive root of F^2 - A F - dt Ks (phi + h) dtheta = 0; the increment is
 // capped by the depth.
+// In float, F1 - F0 loses the increment to cancellation once F0 >> dt Ks;
+// the increment is then taken from the rationalised form
+// dF = 2 (F0 dt Ks + c) / (s + F0 - dt Ks), s = sqrt(A^2 + 4c), which has no
+// cancellation for F0 >= dt Ks (in double the reference's form is kept).
 template <class T>
 __host__ __device__ inline T green_ampt(T h, T F0, T ks, T phi, T dtheta, T dt) {
-  const T a = F0 + dt * ks;
-  const T F1 = T(0.5) * (a + Prec<T>::sqrt_(a * a + T(4) * dt * ks * (phi + h) * dtheta));
-  return fmin(h, F1 - F0);
+  const T dks = dt * ks;
+  const T a = F0 + dks;
+  const T c4 = T(4) * dt * ks * (phi + h) * dtheta;
+  const T s = Prec<T>::sqrt_(a * a + c4);
+  T dF;
+  if (Prec<T>::local_datum && F0 >= dks) {
+    const T den = s + (F0 - dks);
+    dF = den > T(0) ? Prec<T>::div(T(2) * F0 * dks + T(0.5) * c4, den) : T(0);
+  } else {
+    dF = T(0.5) * (a + s) - F0;
+  }
+  return fmin(h, dF);
 }
 
 // CFL time-step candidate of one updated cell (timeControl_Kernel.cu:186-189):

@@ -52,8 +52,10 @@ def test_compact_op_matches_oracle(oracle_lib, cuda):
         cls = rng.integers(0, 6, n).astype(np.uint8)
         wet = np.sort(rng.choice(n, 3000, replace=False)).astype(np.int32)
         dh0 = rng.standard_normal(n).astype(dt_) * 1e-3
-        dh_o, F_o = dh0.copy(), F0.copy()
-        oracle_lib.infiltration(wet, dh_o, cls, h, KS, PHI, DTH, F_o, 3.0)
+        # the oracle in FP64 on the same (float-representable) inputs: the
+        # FP32 path is held to its exact result
+        dh_o, F_o = dh0.astype(np.float64), F0.astype(np.float64)
+        oracle_lib.infiltration(wet, dh_o, cls, h.astype(np.float64), KS, PHI, DTH, F_o, 3.0)
         T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(cuda)
         dh_g, F_g = T(dh0), T(F0)
         tt = torch.float64 if dt_ == np.float64 else torch.float32
