@@ -42,6 +42,7 @@ template <class T> struct AdjPlanes {
   T* xc;  // 12 planes: role(4) x var(3), each (ny+1) x cpitch
   T* yc;
   int64_t cpitch, cplane;
+  int frozen_slopes;
 };
 
 template <class T>
@@ -249,7 +250,7 @@ __global__ __launch_bounds__(kAThreads) void k_adj_edges(hp_grid_desc a, AdjPlan
       const T v_br = (axis_x ? cBx : cBy) * nA;
       T a_ec = T(0), a_er = T(0);
       solve_edge_vjp<T, ORDER>(c, l, r, rr, false, c.h + c.z, r.h + r.z, dx, dt, v_fh, v_fn, v_ft, v_bc, v_br,
-                               adj[1], adj[0], adj[2], adj[3], a_ec, a_er);
+                               adj[1], adj[0], adj[2], adj[3], a_ec, a_er, p.frozen_slopes != 0);
       adj[1].h += a_ec;
       adj[2].h += a_er;
       if (!vl) { adj[1].h += adj[0].h; adj[1].qn += adj[0].qn; adj[1].qt += adj[0].qt; adj[0] = zero_adj<T>(); }
@@ -349,6 +350,7 @@ static int launch_adjoint(const hp_grid_desc* g, const hp_adjoint_desc* ad, hipS
   p.xc = (T*)ad->xc; p.yc = (T*)ad->yc;
   p.cpitch = ad->cpitch;
   p.cplane = ad->cpitch * (int64_t)(g->ny + 1);
+  p.frozen_slopes = ad->slope_mode == 1;
   const int64_t total = g->pitch * (int64_t)g->ny;
   const unsigned nbu = (unsigned)((total + kAChunk - 1) / kAChunk);
   hipLaunchKernelGGL((k_adj_update<T>), dim3(nbu), dim3(kAThreads), 0, st, *g, p, ad->grads);
@@ -385,6 +387,7 @@ int hp_adjoint_step(const hp_grid_desc* g, const hp_adjoint_desc* ad, void* stre
   if (ad->cpitch < (int64_t)g->nx + 1) return fail(HP_EINVAL, "cpitch must be >= nx + 1");
   if (g->n_classes > kGradStride - 1) return fail(HP_EINVAL, "the adjoint supports at most 15 Manning classes");
   if (g->infil_ks) return fail(HP_EINVAL, "the adjoint does not cover infiltration");
+  if (ad->slope_mode != 0 && ad->slope_mode != 1) return fail(HP_EINVAL, "slope_mode must be 0 or 1");
   int rc = grid_flux_launch(g, (hipStream_t)stream);  // recompute the step's flux on U_k
   if (rc) return rc;
   if (g->dtype == HP_F32) return launch_adjoint<float>(g, ad, (hipStream_t)stream);

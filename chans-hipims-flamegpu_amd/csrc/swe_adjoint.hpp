@@ -46,7 +46,7 @@ __host__ __device__ inline bool solve_edge_vjp(const FrameCell<T>& c, const Fram
                                                const FrameCell<T>& rr, bool r_is_ghost, T etaC, T etaR, T dx, T dt,
                                                T v_fh, T v_fn, T v_ft, T v_bc, T v_br, FrameAdj<T>& ac,
                                                FrameAdj<T>& al, FrameAdj<T>& ar, FrameAdj<T>& arr, T& a_etaC,
-                                               T& a_etaR) {
+                                               T& a_etaR, bool frozen_slopes = false) {
   const T hs = T(kHs), g = T(kG);
   if (c.h < hs && r.h < hs) return false;
 
@@ -249,7 +249,12 @@ __host__ __device__ inline bool solve_edge_vjp(const FrameCell<T>& c, const Fram
   }
 
   // ---- reverse: edge states ----
-  if (ORDER == 1 || r_is_ghost) {
+  if (ORDER == 1 || r_is_ghost || frozen_slopes) {
+    // (with frozen slopes the limited slope terms are constants)
+    if (ORDER == 2 && !r_is_ghost) {
+      if (zeroL) a_eLqn = a_eLqt = T(0);
+      if (zeroR) a_eRqn = a_eRqt = T(0);
+    }
     ac.h += a_eLh; ac.qn += a_eLqn; ac.qt += a_eLqt;
     ar.h += a_eRh; ar.qn += a_eRqn; ar.qt += a_eRqt;
   } else {

@@ -27,9 +27,16 @@ from .grid import GridSolver
 
 
 class Calibration:
-    def __init__(self, solver: GridSolver, gauges, steps: int, checkpoint: int = 100, obs=None):
+    def __init__(self, solver: GridSolver, gauges, steps: int, checkpoint: int = 100, obs=None,
+                 slopes: str = "exact"):
         """gauges: (row, col) raster cells; obs: [steps + 1, G] observed depths
-        (row 0 unused) or None (set later)."""
+        (row 0 unused) or None (set later).  slopes: "exact" differentiates
+        the limited MUSCL slopes (the exact derivative of the discrete step;
+        its linearisation grows over long horizons, as that of MUSCL with
+        forward Euler does), "frozen" holds them fixed (stable over long
+        horizons; an approximation of the derivative)."""
+        if slopes not in ("exact", "frozen"):
+            raise ValueError("slopes must be 'exact' or 'frozen'")
         self.s = s = solver
         self.K = int(steps)
         self.ck = int(checkpoint)
@@ -65,6 +72,7 @@ class Calibration:
         ad.xc, ad.yc = self.xc.data_ptr(), self.yc.data_ptr()
         ad.cpitch = self.cpitch
         ad.grads = self.grads.data_ptr()
+        ad.slope_mode = 1 if slopes == "frozen" else 0
         self.adesc = ad
         if self.C > _lib.GRAD_STRIDE - 1:
             raise ValueError("at most 15 Manning classes")

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define HP_ABI_VERSION 2
+#define HP_ABI_VERSION 3
 
 enum hp_status {
   HP_OK = 0,
@@ -253,13 +253,20 @@ int hp_ensemble_step(const hp_ensemble* e, int32_t nsteps, void* stream);
  * planes each of (ny + 1) rows x cpitch (cpitch >= nx + 1) for the edge
  * contributions.  grads: FP64 [hp_adjoint_grad_blocks][16] running sums:
  * column c < 15 the gradient w.r.t. the Manning value of class c, column 15
- * the gradient w.r.t. the rain multiplier (rain_scale). */
+ * the gradient w.r.t. the rain multiplier (rain_scale).
+ * slope_mode 0 differentiates the minmod-limited MUSCL slopes exactly (the
+ * exact derivative of the step; over long horizons its frozen-branch
+ * linearisation grows, as that of MUSCL with forward Euler does); 1 holds
+ * the limited slopes fixed (the derivative of the first-order flux at the
+ * reconstructed states, stable over long horizons, an approximation). */
 typedef struct hp_adjoint_desc {
   void *lam_h, *lam_qx, *lam_qy;
   void *cot_h, *cot_qx, *cot_qy;
   void *xc, *yc;
   int64_t cpitch;
   double* grads;
+  int32_t slope_mode;
+  int32_t reserved;
 } hp_adjoint_desc;
 
 int64_t hp_adjoint_grad_blocks(const hp_grid_desc* g);
