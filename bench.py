@@ -124,6 +124,10 @@ def parse():
     p.add_argument("--adj-steps", type=int, default=2000)
     p.add_argument("--adj-checkpoint", type=int, default=100)
     p.add_argument("--adj-gauges", type=int, default=32)
+    p.add_argument("--adj-order", type=int, choices=[1, 2], default=1,
+                   help="adjoint: scheme order (the server's default, order 1, has a stable adjoint over long "
+                        "horizons; the frozen-branch linearisation of MUSCL with forward Euler grows)")
+    p.add_argument("--adj-slopes", choices=["exact", "frozen"], default="exact")
     p.add_argument("--no-fd", action="store_true", help="adjoint: skip the finite-difference check")
     return p.parse_args()
 
@@ -160,9 +164,12 @@ def ensemble_mode(a):
     own DEM / landcover per scenario (seed 2000 + k), triangular hyetograph,
     each run to the same simulated time with its own CFL dt; FP32, order 2.
     Ensemble: every scenario advanced by one launch set per step, K steps per
-    graph replay.  Baseline: the same scenarios one after another through the
-    Godunov driver's per-step calls (addFlux, addStation_PrecipitationSource,
-    time_friction_euler_update_cuda, a host read of t each step)."""
+    graph replay.  Baselines: the same scenarios one after another through
+    the per-step calls (addFlux, addStation_PrecipitationSource,
+    time_friction_euler_update_cuda, a host read of t each step) of (a) the
+    structured Godunov drop-in and (b) CompactGodunov, the reference's own
+    compact-layout sequence (wet lists by nonzero, host dt logic) over the
+    HIP compact ops."""
     import concurrent.futures as cf
 
     import numpy as np
@@ -171,6 +178,7 @@ def ensemble_mode(a):
     from hipims_amd import generators as gen
     from hipims_amd.ensemble import Ensemble
     from hipims_amd.grid import GridSolver
+    from hipims_amd.compact_driver import CompactGodunov
     from hipims_amd.SWE_CUDA import Godunov
 
     # generate on host worker processes before this process touches the GPU
@@ -214,12 +222,31 @@ def ensemble_mode(a):
     del ens
     torch.cuda.empty_cache()
 
+    seq = {}
+    for name, cls in (("godunov", Godunov), ("compact", CompactGodunov)):
+        seq[name] = sequential(cls, cases, a.sim_seconds, dev, name)
+    seq_s, seq_steps = seq["godunov"]
+    line = {"mode": "ensemble", "scenarios": len(cases), "sizes": [c.mask.shape[0] - 2 for c in cases],
+            "sim_seconds": a.sim_seconds, "dtype": "f32", "order": 2, "steps_per_replay": a.replay_steps,
+            "cell_updates": work, "ensemble_s": ens_s, "ensemble_cell_updates_per_s": work / ens_s,
+            "steps_min": min(steps), "steps_max": max(steps), "generation_s": gen_s,
+            "baselines": {name: {"seconds": v[0], "cell_updates_per_s": work / v[0], "speedup": v[0] / ens_s,
+                                 "same_step_counts": v[1] == steps} for name, v in seq.items()}}
+    print(json.dumps(line), flush=True)
+
+
+def sequential(cls, cases, t_stop, dev, name):
+    """The scenarios one after another through a driver's per-step calls:
+    addFlux, addStation_PrecipitationSource, time_friction_euler_update_cuda
+    and a host read of t each step (CHANS-HiPIMS_server.py:246-251)."""
+    import numpy as np
+    import torch
     seq_s = 0.0
     seq_steps = []
     T = lambda x: torch.as_tensor(np.asarray(x))
     for k, c in enumerate(cases):
-        num = Godunov(dev, c.dx, c.cfl, 1.0e30, t=0.0, export_n=0, firstTimeStep=c.first_dt, secondOrder=True,
-                      tensorType=torch.float32)
+        num = cls(dev, c.dx, c.cfl, 1.0e30, t=0.0, export_n=0, firstTimeStep=c.first_dt, secondOrder=True,
+                  tensorType=torch.float32)
         num.init__fluidField_tensor(T(c.mask), T(c.h), T(c.qx), T(c.qy), T(c.h + c.z), T(c.z), dev)
         num.set__frictionField_tensor(c.manning, dev)
         num.set_landuse(T(c.mask), T(c.landuse), dev)
@@ -228,7 +255,7 @@ def ensemble_mode(a):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         n = 0
-        while num.t.item() < a.sim_seconds:
+        while num.t.item() < t_stop:
             num.addFlux()
             num.addStation_PrecipitationSource(c.rain_table, dev)
             num.time_friction_euler_update_cuda(dev)
@@ -237,15 +264,9 @@ def ensemble_mode(a):
         seq_s += time.perf_counter() - t0
         seq_steps.append(n)
         del num
-        print(f"[ensemble] sequential baseline: scenario {k + 1} ({c.mask.shape[0] - 2}^2, {n} steps), "
+        print(f"[ensemble] sequential {name}: scenario {k + 1} ({c.mask.shape[0] - 2}^2, {n} steps), "
               f"{seq_s:.1f} s so far", file=sys.stderr, flush=True)
-    line = {"mode": "ensemble", "scenarios": len(cases), "sizes": [c.mask.shape[0] - 2 for c in cases],
-            "sim_seconds": a.sim_seconds, "dtype": "f32", "order": 2, "steps_per_replay": a.replay_steps,
-            "cell_updates": work, "ensemble_s": ens_s, "sequential_s": seq_s,
-            "ensemble_cell_updates_per_s": work / ens_s, "sequential_cell_updates_per_s": work / seq_s,
-            "speedup": seq_s / ens_s, "steps_min": min(steps), "steps_max": max(steps),
-            "same_step_counts": steps == seq_steps, "generation_s": gen_s}
-    print(json.dumps(line), flush=True)
+    return seq_s, seq_steps
 
 
 def adjoint_mode(a):
@@ -268,8 +289,8 @@ def adjoint_mode(a):
     c = gen.rainfall_runoff(a.adj_size, 5.0)
 
     def solver():
-        s = GridSolver(c.mask, c.h, c.qx, c.qy, c.z, c.dx, cfl=c.cfl, order=2, dtype=torch.float32, device=dev,
-                       first_dt=c.first_dt, manning=c.manning, landuse=c.landuse)
+        s = GridSolver(c.mask, c.h, c.qx, c.qy, c.z, c.dx, cfl=c.cfl, order=a.adj_order, dtype=torch.float32,
+                       device=dev, first_dt=c.first_dt, manning=c.manning, landuse=c.landuse)
         s.set_station_rain(c.rain_table)
         return s
 
@@ -292,7 +313,7 @@ def adjoint_mode(a):
     torch.cuda.synchronize()
     fwd_s = time.perf_counter() - t0
 
-    cal = Calibration(solver(), gauges, a.adj_steps, a.adj_checkpoint)
+    cal = Calibration(solver(), gauges, a.adj_steps, a.adj_checkpoint, slopes=a.adj_slopes)
     th0 = cal.theta0()
     pert = torch.tensor(np.random.Generator(np.random.PCG64(7)).choice([0.8, 1.2], size=th0.numel() - 1))
     th_true = th0.clone()
@@ -307,7 +328,8 @@ def adjoint_mode(a):
     grad = cal.gradient()
     torch.cuda.synchronize()
     rev_s = time.perf_counter() - t0
-    line = {"mode": "adjoint", "size": a.adj_size, "steps": a.adj_steps, "checkpoint": a.adj_checkpoint,
+    line = {"mode": "adjoint", "size": a.adj_size, "order": a.adj_order, "slopes": a.adj_slopes,
+            "steps": a.adj_steps, "checkpoint": a.adj_checkpoint,
             "gauges": int(len(gauges)), "dtype": "f32 state, f64 gradient sums", "loss": float(loss),
             "grad": [float(v) for v in grad], "plain_forward_s": fwd_s, "recorded_forward_s": run_s,
             "reverse_s": rev_s, "cost_ratio": (run_s + rev_s) / fwd_s}
