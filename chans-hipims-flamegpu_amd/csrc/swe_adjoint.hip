@@ -26,12 +26,12 @@
 #include "../../include/hipims_abi.h"
 #include "hp_common.hpp"
 #include "swe_adjoint.hpp"
+#include "swe_source.hpp"
 
 namespace hp {
 
 constexpr int kAThreads = 256;
-constexpr int kAIters = 16;  // must match the update pass's chunking
-constexpr int kAChunk = kAThreads * kAIters;
+constexpr int kAChunk = 4096;  // plane elements per block of the pointwise passes
 constexpr int kGradStride = 16;  // doubles per block: classes 0..14, rain at 15
 
 int grid_flux_launch(const hp_grid_desc* g, hipStream_t st);  // swe_grid.hip
@@ -64,33 +64,23 @@ __device__ inline bool flux_listed(const hp_grid_desc& a, const T* h, int x, int
   return false;
 }
 
+// Reverse of rain + friction + update, kVec consecutive elements per thread
+// per access.  The derivative w.r.t. the Manning value of a cell's class goes
+// into the cell's FP64 running sum gn (folded per class once, by
+// hp_adjoint_finish); the rain-multiplier derivative is summed per block.
 template <class T>
-__global__ __launch_bounds__(kAThreads) void k_adj_update(hp_grid_desc a, AdjPlanes<T> p, double* grads) {
-  __shared__ double s_rain[1024];
-  __shared__ double s_g[kAThreads / 64][kGradStride];
+__global__ __launch_bounds__(kAThreads) void k_adj_update(hp_grid_desc a, AdjPlanes<T> p, double* grads,
+                                                          double* gn) {
+  __shared__ T s_man[256];
+  __shared__ double s_r[kAThreads / 64];
   const int tid = threadIdx.x;
   const double t_now = a.ctrl->t;
   const T dt = T(a.ctrl->dt);
   const T t_T = T(t_now);
   const T hs = T(kHs);
   const int mode = a.rain_mode;
-  int n_st = 0;
-  if (mode == HP_RAIN_STATION && a.rain_table != nullptr && a.rain_rows > 0) {
-    const int rows = a.rain_rows, cols = a.rain_cols;
-    n_st = cols - 1;
-    const double* tab = a.rain_table;
-    const bool after = !(t_now < tab[(rows - 1) * cols]);
-    int i = 0;
-    while (i + 1 < rows - 1 && !(t_now < tab[(i + 1) * cols])) i++;
-    for (int k = tid; k < n_st && k < 1024; k += kAThreads) {
-      double v = 0.0;
-      if (!after && rows > 1) {
-        const double per = (t_now - tab[i * cols]) / (tab[(i + 1) * cols] - tab[i * cols]);
-        v = tab[i * cols + 1 + k] + per * (tab[(i + 1) * cols + 1 + k] - tab[i * cols + 1 + k]);
-      }
-      s_rain[k] = v;  // unscaled rate
-    }
-  }
+  for (int k = tid; k < a.n_classes; k += kAThreads) s_man[k] = ((const T*)a.manning)[k];
+  const RainRows rr = rain_rows(a, t_now);
   __syncthreads();
   const T* __restrict__ h = (const T*)a.h;
   const T* __restrict__ qx = (const T*)a.qx;
@@ -98,67 +88,117 @@ __global__ __launch_bounds__(kAThreads) void k_adj_update(hp_grid_desc a, AdjPla
   const T* __restrict__ dh = (const T*)a.dh;
   const T* __restrict__ dqx = (const T*)a.dqx;
   const T* __restrict__ dqy = (const T*)a.dqy;
-  const T* __restrict__ man = (const T*)a.manning;
   const T scale = T(a.rain_scale);
-  double g_loc[kGradStride];
+  double rsum = 0.0;
+  const int64_t total = a.pitch * (int64_t)a.ny;  // a multiple of 64
+  const int64_t base = (int64_t)blockIdx.x * kAChunk;
+  for (int it = 0; it < kAChunk / (kAThreads * kVec); ++it) {
+    const int64_t i0 = base + ((int64_t)it * kAThreads + tid) * kVec;
+    if (i0 >= total) break;
+    const VecU8 vl = ldv<VecU8>(a.landuse + i0);
+    const VecT<T> vh = ldv<VecT<T>>(h + i0), vqx = ldv<VecT<T>>(qx + i0), vqy = ldv<VecT<T>>(qy + i0);
+    const VecT<T> vdh = ldv<VecT<T>>(dh + i0), vdqx = ldv<VecT<T>>(dqx + i0), vdqy = ldv<VecT<T>>(dqy + i0);
+    VecT<T> lh = ldv<VecT<T>>(p.lh + i0), lqx = ldv<VecT<T>>(p.lqx + i0), lqy = ldv<VecT<T>>(p.lqy + i0);
+    VecD vg = ldv<VecD>(gn + i0);
+    VecT<T> ch{}, cqx{}, cqy{}, rbv{}, rsc{};
+    if (mode == HP_RAIN_STATION) {
+      VecI16 vs{};
+      if (a.station) vs = ldv<VecI16>(a.station + i0);
 #pragma unroll
-  for (int k = 0; k < kGradStride; ++k) g_loc[k] = 0.0;
+      for (int j = 0; j < kVec; ++j) {
+        rbv.x[j] = station_rate<T>(rr, vs.x[j], 1.0);
+        rsc.x[j] = station_rate<T>(rr, vs.x[j], a.rain_scale);
+      }
+    } else if (mode == HP_RAIN_FIELD) {
+      const VecF32 vf = ldv<VecF32>(a.rain_field + i0);
+#pragma unroll
+      for (int j = 0; j < kVec; ++j) {
+        rbv.x[j] = T(vf.x[j]);
+        rsc.x[j] = T(vf.x[j]) * scale;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kVec; ++j) {
+      const int cls = vl.x[j];
+      if (cls == 255) {
+        lh.x[j] = lqx.x[j] = lqy.x[j] = T(0);
+        continue;
+      }
+      const T h0 = vh.x[j];
+      const T rdt = rsc.x[j] * dt;
+      const T dht = vdh.x[j] + rdt;
+      if (!(fabs(dht) > T(0) || h0 >= T(0))) continue;  // not updated: lambda passes through
+      const T h1 = h0 + dht;
+      T a_h1 = lh.x[j], a_qsx = T(0), a_qsy = T(0);
+      if (!(h1 < hs)) {
+        T a_hf = T(0), a_n = T(0);
+        friction_vjp<T>(h1, s_man[cls], dt, vqx.x[j] + vdqx.x[j], vqy.x[j] + vdqy.x[j], lqx.x[j], lqy.x[j], a_qsx,
+                        a_qsy, a_hf, a_n);
+        a_h1 += a_hf;
+        vg.x[j] += (double)a_n;
+      }
+      rsum += (double)a_h1 * (double)rbv.x[j] * (double)dt;
+      lh.x[j] = a_h1;
+      lqx.x[j] = a_qsx;
+      lqy.x[j] = a_qsy;
+      const int64_t i = i0 + j;
+      const int y = (int)(i / a.pitch), x = (int)(i - (int64_t)y * a.pitch);
+      if (flux_listed<T>(a, h, x, y, t_T)) {
+        ch.x[j] = a_h1;
+        cqx.x[j] = a_qsx;
+        cqy.x[j] = a_qsy;
+      }
+    }
+    stv(p.lh + i0, lh);
+    stv(p.lqx + i0, lqx);
+    stv(p.lqy + i0, lqy);
+    stv(p.ch + i0, ch);
+    stv(p.cqx + i0, cqx);
+    stv(p.cqy + i0, cqy);
+    stv(gn + i0, vg);
+  }
+  const int lane = tid & 63, w = tid >> 6;
+  for (int off = 32; off > 0; off >>= 1) rsum += __shfl_xor(rsum, off, 64);
+  if (lane == 0) s_r[w] = rsum;
+  __syncthreads();
+  if (tid == 0) {
+    double v = 0.0;
+    for (int k = 0; k < kAThreads / 64; ++k) v += s_r[k];
+    grads[(int64_t)blockIdx.x * kGradStride + kGradStride - 1] += v;
+  }
+}
+
+// grads[b][c] (c < 15) = sum of gn over block b's cells of class c: the
+// per-class Manning gradients, folded once after the reverse sweep.
+__global__ __launch_bounds__(kAThreads) void k_adj_finish(hp_grid_desc a, const double* __restrict__ gn,
+                                                          double* grads) {
+  __shared__ double s_g[kAThreads / 64][kGradStride];
+  const int tid = threadIdx.x;
+  double acc[kGradStride - 1];
+#pragma unroll
+  for (int c = 0; c < kGradStride - 1; ++c) acc[c] = 0.0;
   const int64_t total = a.pitch * (int64_t)a.ny;
   const int64_t base = (int64_t)blockIdx.x * kAChunk;
-  for (int it = 0; it < kAIters; ++it) {
+  for (int it = 0; it < kAChunk / kAThreads; ++it) {
     const int64_t i = base + (int64_t)it * kAThreads + tid;
     if (i >= total) break;
     const int cls = a.landuse[i];
-    if (cls == 255) {
-      p.lh[i] = T(0); p.lqx[i] = T(0); p.lqy[i] = T(0);
-      p.ch[i] = T(0); p.cqx[i] = T(0); p.cqy[i] = T(0);
-      continue;
-    }
-    const T h0 = h[i], dhf = dh[i];
-    T rb = T(0);
-    if (mode == HP_RAIN_STATION) {
-      const int k = a.station ? (int)a.station[i] : 0;
-      rb = (k >= 0 && k < n_st) ? T(s_rain[k]) : T(0);
-    } else if (mode == HP_RAIN_FIELD) {
-      rb = T(a.rain_field[i]);
-    }
-    const T rdt = (rb * scale) * dt;
-    const T dht = dhf + rdt;
-    const T lh1 = p.lh[i], lqx1 = p.lqx[i], lqy1 = p.lqy[i];
-    T lh0, lqx0, lqy0, ch = T(0), cqx = T(0), cqy = T(0);
-    if (!(fabs(dht) > T(0) || h0 >= T(0))) {
-      lh0 = lh1; lqx0 = lqx1; lqy0 = lqy1;
-    } else {
-      const T h1 = h0 + dht;
-      T a_h1 = lh1, a_qsx = T(0), a_qsy = T(0);
-      if (!(h1 < hs)) {
-        T a_hf = T(0), a_n = T(0);
-        friction_vjp<T>(h1, man[cls], dt, qx[i] + dqx[i], qy[i] + dqy[i], lqx1, lqy1, a_qsx, a_qsy, a_hf, a_n);
-        a_h1 += a_hf;
-        if (cls < kGradStride - 1) g_loc[cls] += (double)a_n;
-      }
-      g_loc[kGradStride - 1] += (double)a_h1 * (double)rb * (double)dt;
-      lh0 = a_h1; lqx0 = a_qsx; lqy0 = a_qsy;
-      const int y = (int)(i / a.pitch), x = (int)(i - (int64_t)y * a.pitch);
-      if (flux_listed<T>(a, h, x, y, t_T)) {
-        ch = a_h1; cqx = a_qsx; cqy = a_qsy;
-      }
-    }
-    p.lh[i] = lh0; p.lqx[i] = lqx0; p.lqy[i] = lqy0;
-    p.ch[i] = ch; p.cqx[i] = cqx; p.cqy[i] = cqy;
+    const double v = gn[i];
+#pragma unroll
+    for (int c = 0; c < kGradStride - 1; ++c) acc[c] += (cls == c) ? v : 0.0;
   }
   const int lane = tid & 63, w = tid >> 6;
 #pragma unroll
-  for (int k = 0; k < kGradStride; ++k) {
-    double v = g_loc[k];
+  for (int c = 0; c < kGradStride - 1; ++c) {
+    double v = acc[c];
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    if (lane == 0) s_g[w][k] = v;
+    if (lane == 0) s_g[w][c] = v;
   }
   __syncthreads();
-  if (tid < kGradStride) {
+  if (tid < kGradStride - 1) {
     double v = 0.0;
     for (int k = 0; k < kAThreads / 64; ++k) v += s_g[k][tid];
-    grads[(int64_t)blockIdx.x * kGradStride + tid] += v;
+    grads[(int64_t)blockIdx.x * kGradStride + tid] = v;
   }
 }
 
@@ -298,7 +338,10 @@ __global__ __launch_bounds__(kAThreads) void k_adj_edges(hp_grid_desc a, AdjPlan
       dir[from_a ? 1 : 2] = d;
     }
   }
-  for (int role = 0; role < 4; ++role) {
+  // order 1 reads no cell beyond the edge's two (the bed reconstruction's
+  // neighbours enter through the static bed only): roles l and rr carry
+  // nothing and are neither written nor gathered
+  for (int role = ORDER == 1 ? 1 : 0; role < (ORDER == 1 ? 3 : 4); ++role) {
     T gx, gy;
     to_global_adj<T>(adj[role], dir[role], gx, gy);
     T* base = out + (int64_t)(role * 3) * p.cplane + slot;
@@ -308,7 +351,7 @@ __global__ __launch_bounds__(kAThreads) void k_adj_edges(hp_grid_desc a, AdjPlan
   }
 }
 
-template <class T>
+template <class T, int ORDER>
 __global__ __launch_bounds__(kAThreads) void k_adj_gather(hp_grid_desc a, AdjPlanes<T> p) {
   const int64_t i = (int64_t)blockIdx.x * kAThreads + threadIdx.x;
   const int64_t total = a.pitch * (int64_t)a.ny;
@@ -323,7 +366,7 @@ __global__ __launch_bounds__(kAThreads) void k_adj_gather(hp_grid_desc a, AdjPla
                          (int64_t)(y - 1) * C + x};
   T sh = T(0), sx = T(0), sy = T(0);
 #pragma unroll
-  for (int role = 0; role < 4; ++role) {
+  for (int role = ORDER == 1 ? 1 : 0; role < (ORDER == 1 ? 3 : 4); ++role) {
     const T* bx = p.xc + (int64_t)(role * 3) * p.cplane;
     const T* by = p.yc + (int64_t)(role * 3) * p.cplane;
     sh += bx[xs[role]] + by[ys[role]];
@@ -353,7 +396,7 @@ static int launch_adjoint(const hp_grid_desc* g, const hp_adjoint_desc* ad, hipS
   p.frozen_slopes = ad->slope_mode == 1;
   const int64_t total = g->pitch * (int64_t)g->ny;
   const unsigned nbu = (unsigned)((total + kAChunk - 1) / kAChunk);
-  hipLaunchKernelGGL((k_adj_update<T>), dim3(nbu), dim3(kAThreads), 0, st, *g, p, ad->grads);
+  hipLaunchKernelGGL((k_adj_update<T>), dim3(nbu), dim3(kAThreads), 0, st, *g, p, ad->grads, ad->gn);
   int rc = check_launch("k_adj_update");
   if (rc) return rc;
   const int64_t nedges = (int64_t)(g->nx + 1) * g->ny + (int64_t)g->nx * (g->ny + 1);
@@ -363,7 +406,8 @@ static int launch_adjoint(const hp_grid_desc* g, const hp_adjoint_desc* ad, hipS
   rc = check_launch("k_adj_edges");
   if (rc) return rc;
   const unsigned nbg = (unsigned)((total + kAThreads - 1) / kAThreads);
-  hipLaunchKernelGGL((k_adj_gather<T>), dim3(nbg), dim3(kAThreads), 0, st, *g, p);
+  if (g->order == 2) hipLaunchKernelGGL((k_adj_gather<T, 2>), dim3(nbg), dim3(kAThreads), 0, st, *g, p);
+  else hipLaunchKernelGGL((k_adj_gather<T, 1>), dim3(nbg), dim3(kAThreads), 0, st, *g, p);
   return check_launch("k_adj_gather");
 }
 
@@ -382,16 +426,27 @@ int hp_adjoint_step(const hp_grid_desc* g, const hp_adjoint_desc* ad, void* stre
   if (!ad) return fail(HP_EINVAL, "null adjoint descriptor");
   if (!g) return fail(HP_EINVAL, "null descriptor");
   if (!ad->lam_h || !ad->lam_qx || !ad->lam_qy || !ad->cot_h || !ad->cot_qx || !ad->cot_qy || !ad->xc || !ad->yc ||
-      !ad->grads)
+      !ad->grads || !ad->gn)
     return fail(HP_EINVAL, "an adjoint plane pointer is null");
+  if (g->pitch % kVec) return fail(HP_EINVAL, "pitch must be a multiple of 4");
   if (ad->cpitch < (int64_t)g->nx + 1) return fail(HP_EINVAL, "cpitch must be >= nx + 1");
   if (g->n_classes > kGradStride - 1) return fail(HP_EINVAL, "the adjoint supports at most 15 Manning classes");
   if (g->infil_ks) return fail(HP_EINVAL, "the adjoint does not cover infiltration");
   if (ad->slope_mode != 0 && ad->slope_mode != 1) return fail(HP_EINVAL, "slope_mode must be 0 or 1");
-  int rc = grid_flux_launch(g, (hipStream_t)stream);  // recompute the step's flux on U_k
-  if (rc) return rc;
+  if (!ad->flux_ready) {
+    int rc = grid_flux_launch(g, (hipStream_t)stream);  // recompute the step's flux on U_k
+    if (rc) return rc;
+  }
   if (g->dtype == HP_F32) return launch_adjoint<float>(g, ad, (hipStream_t)stream);
   return launch_adjoint<double>(g, ad, (hipStream_t)stream);
+}
+
+int hp_adjoint_finish(const hp_grid_desc* g, const hp_adjoint_desc* ad, void* stream) {
+  if (!g || !ad || !ad->gn || !ad->grads || !g->landuse) return fail(HP_EINVAL, "null descriptor or plane");
+  const int64_t nb = hp_adjoint_grad_blocks(g);
+  hipLaunchKernelGGL(k_adj_finish, dim3((unsigned)nb), dim3(kAThreads), 0, (hipStream_t)stream, *g,
+                     (const double*)ad->gn, ad->grads);
+  return check_launch("k_adj_finish");
 }
 
 int hp_adjoint_add_loss(int dtype, void* lam_h, const int64_t* offsets, const double* h_rec, const double* obs,

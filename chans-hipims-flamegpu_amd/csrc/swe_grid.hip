@@ -34,12 +34,12 @@
 #include "../../include/hipims_abi.h"
 #include "hp_common.hpp"
 #include "swe_physics.hpp"
+#include "swe_source.hpp"
 
 namespace hp {
 
 constexpr int kTX = 64;
 constexpr int kThreads = 256;
-constexpr int kVec = 4;       // consecutive plane elements per thread in the update pass
 constexpr int kChunk = 4096;  // plane elements per update block
 constexpr int kMaxStations = 1024;
 
@@ -213,52 +213,6 @@ __device__ inline void solve_tile_edge(const FluxTile<T, TY>& s, const hp_grid_d
     if (from_a) store_record<T, D, kRec>(out, o.fh, gx, gy, bed, T(0));
     else store_record<T, D, kRec>(out, -o.fh, -gx, -gy, T(0), bed);
   }
-}
-
-// Station rain at time t_now (SWE_CUDA.py:472-495): linear in time between
-// the table rows that bracket t_now, 0 from the table's last row on, scaled
-// by rain_scale.  The bracketing rows are found once per block; each cell
-// then evaluates its own station's rate from the two rows (a few cached
-// loads), so no per-block staging array is needed.
-struct RainRows {
-  const double* lo;  // station columns of the rows bracketing t_now
-  const double* hi;
-  double per;        // interpolation weight
-  int n_st;
-  bool zero;         // every station rate is 0 (after the table, or one row)
-};
-
-__device__ inline RainRows rain_rows(const hp_grid_desc& a, double t_now) {
-  RainRows r{nullptr, nullptr, 0.0, 0, true};
-  if (a.rain_mode != HP_RAIN_STATION || a.rain_table == nullptr || a.rain_rows <= 0) return r;
-  const int rows = a.rain_rows, cols = a.rain_cols;
-  const double* tab = a.rain_table;
-  r.n_st = cols - 1;
-  const bool after = !(t_now < tab[(rows - 1) * cols]);
-  int i = 0;
-  while (i + 1 < rows - 1 && !(t_now < tab[(i + 1) * cols])) i++;
-  r.zero = after || rows <= 1;
-  if (!r.zero) {
-    r.lo = tab + i * cols + 1;
-    r.hi = tab + (i + 1) * cols + 1;
-    r.per = (t_now - tab[i * cols]) / (tab[(i + 1) * cols] - tab[i * cols]);
-  }
-  return r;
-}
-
-// Rain rate of a cell whose station index is k.
-template <class T> __device__ inline T station_rate(const RainRows& rr, int k, double scale) {
-  if (rr.zero || k < 0 || k >= rr.n_st) return T(0);
-  const double lo = rr.lo[k], hi = rr.hi[k];
-  return T((lo + rr.per * (hi - lo)) * scale);
-}
-
-// Rain rate of plane element i for the descriptor's rain mode.
-template <class T>
-__device__ inline T cell_rain(const hp_grid_desc& a, const RainRows& rr, int64_t i) {
-  if (a.rain_mode == HP_RAIN_STATION) return station_rate<T>(rr, a.station ? (int)a.station[i] : 0, a.rain_scale);
-  if (a.rain_mode == HP_RAIN_FIELD) return T(a.rain_field[i]) * T(a.rain_scale);
-  return T(0);
 }
 
 // Fold a block's CFL minimum and FP64 sums into one partial record.
@@ -608,21 +562,6 @@ __global__ __launch_bounds__(kThreads) void k_infiltration(hp_grid_desc a) {
 //                       state change and no rain; h_max, CFL + dt/t
 // Processes plane elements [chunk*kChunk, (chunk+1)*kChunk) and writes one
 // partial record.
-template <class T> struct alignas(sizeof(T) * kVec) VecT {
-  T x[kVec];
-};
-struct alignas(kVec) VecU8 {
-  uint8_t x[kVec];
-};
-struct alignas(2 * kVec) VecI16 {
-  int16_t x[kVec];
-};
-struct alignas(4 * kVec) VecF32 {
-  float x[kVec];
-};
-template <class V, class T> __device__ inline V ldv(const T* p) { return *reinterpret_cast<const V*>(p); }
-template <class V, class T> __device__ inline void stv(T* p, const V& v) { *reinterpret_cast<V*>(p) = v; }
-
 template <class T, int MODE>
 __device__ inline void update_chunk(const hp_grid_desc& a, int64_t chunk, Partial* __restrict__ part) {
   __shared__ T s_man[256];

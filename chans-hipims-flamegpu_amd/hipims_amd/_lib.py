@@ -22,7 +22,7 @@ HP_OK, HP_EINVAL, HP_ELAUNCH, HP_ESPACE = 0, -1, -2, -3
 HP_F32, HP_F64 = 0, 1
 HP_RAIN_NONE, HP_RAIN_STATION, HP_RAIN_FIELD = 0, 1, 2
 HP_UPDATE_STEP, HP_UPDATE_FRICTION, HP_UPDATE_EULER, HP_UPDATE_CFL = 0, 1, 2, 3
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 _P = ctypes.c_void_p
 _I64 = ctypes.c_int64
@@ -70,7 +70,8 @@ class HpEnsemble(ctypes.Structure):
 
 class HpAdjointDesc(ctypes.Structure):
     _fields_ = [("lam_h", _P), ("lam_qx", _P), ("lam_qy", _P), ("cot_h", _P), ("cot_qx", _P), ("cot_qy", _P),
-                ("xc", _P), ("yc", _P), ("cpitch", _I64), ("grads", _P), ("slope_mode", _I32), ("reserved", _I32)]
+                ("xc", _P), ("yc", _P), ("cpitch", _I64), ("grads", _P), ("gn", _P), ("slope_mode", _I32),
+                ("flux_ready", _I32)]
 
 
 GRAD_STRIDE = 16
@@ -101,6 +102,7 @@ SIGNATURES = {
     "hp_ensemble_step": (_I, [ctypes.POINTER(HpEnsemble), _I32, _P]),
     "hp_adjoint_grad_blocks": (_I64, [ctypes.POINTER(HpGridDesc)]),
     "hp_adjoint_step": (_I, [ctypes.POINTER(HpGridDesc), ctypes.POINTER(HpAdjointDesc), _P]),
+    "hp_adjoint_finish": (_I, [ctypes.POINTER(HpGridDesc), ctypes.POINTER(HpAdjointDesc), _P]),
     "hp_adjoint_add_loss": (_I, [_I, _P, _P, _P, _P, _I32, ctypes.c_double, _P]),
     "hp_gather": (_I, [_I, _P, _P, _I64, _P, _P]),
     "hp_scatter": (_I, [_I, _P, _P, _I64, _P, _P]),

@@ -7,12 +7,20 @@ schedule of the first forward run is recorded and then frozen (t_k, dt_k are
 constants for the derivative, for the finite-difference checks and for every
 recomputation), as the adjoint of the CFL minimum is not defined.
 
-backward(): checkpoints of the state every `checkpoint` steps are kept from
-the forward run; segments are recomputed last to first, their states stored,
-and the hand-written adjoint kernels (hp_adjoint_step) run the step in
-reverse.  Parameter gradients are accumulated in FP64 per block and summed at
-the end.  ``CalibrationLoss.apply(theta, calib)`` exposes this as a
-torch.autograd.Function.
+backward(): the state before every `checkpoint`-th step is kept from the
+forward run; segments are recomputed last to first, storing the state before
+each step and the step's flux, and the hand-written adjoint kernels
+(hp_adjoint_step, with the saved flux) run the steps in reverse directly on
+the stored planes.  checkpoint=1 keeps every state and flux from the forward
+run and recomputes nothing (K x 6 planes of device memory).  Parameter
+gradients are accumulated in FP64 (per cell for Manning, per block for the
+rain multiplier) and folded at the end.  ``CalibrationLoss.apply(theta,
+calib)`` exposes this as a torch.autograd.Function.
+
+The exact adjoint of the order-1 step stays bounded over long horizons; the
+frozen-branch linearisation of the order-2 (MUSCL, forward Euler) step grows
+by about 1.2x per step, so long-horizon calibration uses order 1 (the
+reference server's default) or ``slopes="frozen"``.
 """
 from __future__ import annotations
 
@@ -22,7 +30,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import HpAdjointDesc, check, lib
+from ._lib import HpAdjointDesc, HpGridDesc, check, lib
 from .grid import GridSolver
 
 
@@ -65,14 +73,20 @@ class Calibration:
         self.yc = torch.zeros(12, L.ny + 1, self.cpitch, dtype=s.dtype, device=dev)
         self.nblocks = int(lib().hp_adjoint_grad_blocks(ctypes.byref(s.desc)))
         self.grads = torch.zeros(self.nblocks, _lib.GRAD_STRIDE, dtype=torch.float64, device=dev)
-        self.seg = [[z(), z(), z()] for _ in range(self.ck)]
+        self.gn = torch.zeros(shp, dtype=torch.float64, device=dev)
+        # states before each step of a segment and the steps' fluxes
+        self.segU = None if self.ck == 1 else torch.zeros((self.ck,) + tuple(s.U.shape), dtype=s.dtype, device=dev)
+        self.segD = None if self.ck == 1 else torch.zeros_like(self.segU)
+        self.fluxes: dict[int, torch.Tensor] = {}
         ad = HpAdjointDesc()
         ad.lam_h, ad.lam_qx, ad.lam_qy = (t.data_ptr() for t in self.lam)
         ad.cot_h, ad.cot_qx, ad.cot_qy = (t.data_ptr() for t in self.cot)
         ad.xc, ad.yc = self.xc.data_ptr(), self.yc.data_ptr()
         ad.cpitch = self.cpitch
         ad.grads = self.grads.data_ptr()
+        ad.gn = self.gn.data_ptr()
         ad.slope_mode = 1 if slopes == "frozen" else 0
+        ad.flux_ready = 1
         self.adesc = ad
         if self.C > _lib.GRAD_STRIDE - 1:
             raise ValueError("at most 15 Manning classes")
@@ -112,11 +126,14 @@ class Calibration:
         self.set_theta(theta)
         self._restore_initial()
         self.ckpts.clear()
+        self.fluxes.clear()
         self._record_gauges(0)
         for k in range(self.K):
             if keep_checkpoints and k % self.ck == 0:
-                self.ckpts[k] = [t.clone() for t in (self.s.h, self.s.qx, self.s.qy)]
+                self.ckpts[k] = self.s.U.clone()
             self._step(k)
+            if keep_checkpoints and self.ck == 1:
+                self.fluxes[k] = self.s.D.clone()
             self._record_gauges(k + 1)
         self.sched_ready = True
         return self.loss()
@@ -152,30 +169,42 @@ class Calibration:
         for t in self.lam:
             t.zero_()
         self.grads.zero_()
+        self.gn.zero_()
         h_rec = self.hist.double().contiguous()
         stream = _lib.stream_ptr(s.device)
         dcode = _lib.dtype_code(s.dtype)
+        base = s.desc
+        d = HpGridDesc.from_buffer_copy(base)
+        plane = s.h.numel() * s.h.element_size()
         starts = sorted(self.ckpts)
         for si in range(len(starts) - 1, -1, -1):
             a = starts[si]
             b = min(a + self.ck, self.K)
-            # recompute the segment, keeping the state before each step
-            for t, v in zip((s.h, s.qx, s.qy), self.ckpts[a]):
-                t.copy_(v)
-            for k in range(a, b):
-                for t, v in zip(self.seg[k - a], (s.h, s.qx, s.qy)):
-                    t.copy_(v)
-                if k < b - 1:
-                    self._step(k)
+            if self.ck == 1:
+                states = {a: (self.ckpts[a], self.fluxes[a])}
+            else:
+                # recompute the segment: state before each step, then its flux
+                s.U.copy_(self.ckpts[a])
+                states = {}
+                for k in range(a, b):
+                    j = k - a
+                    self.segU[j].copy_(s.U)
+                    s.ctrl[0:2].copy_(self.sched[k])
+                    s.flux()
+                    self.segD[j].copy_(s.D)
+                    if k < b - 1:
+                        s.update()
+                    states[k] = (self.segU[j], self.segD[j])
             for k in range(b - 1, a - 1, -1):
                 check(lib().hp_adjoint_add_loss(dcode, self.lam[0].data_ptr(), self.gauge_off.data_ptr(),
                                                 h_rec[k + 1].data_ptr(), self.obs[k + 1].data_ptr(), self.G, 1.0,
                                                 stream), "hp_adjoint_add_loss")
-                for t, v in zip((s.h, s.qx, s.qy), self.seg[k - a]):
-                    t.copy_(v)
+                U, D = states[k]
+                d.h, d.qx, d.qy = U.data_ptr(), U.data_ptr() + plane, U.data_ptr() + 2 * plane
+                d.dh, d.dqx, d.dqy = D.data_ptr(), D.data_ptr() + plane, D.data_ptr() + 2 * plane
                 s.ctrl[0:2].copy_(self.sched[k])
-                check(lib().hp_adjoint_step(ctypes.byref(s.desc), ctypes.byref(self.adesc), stream),
-                      "hp_adjoint_step")
+                check(lib().hp_adjoint_step(ctypes.byref(d), ctypes.byref(self.adesc), stream), "hp_adjoint_step")
+        check(lib().hp_adjoint_finish(ctypes.byref(base), ctypes.byref(self.adesc), stream), "hp_adjoint_finish")
         g = self.grads.sum(dim=0).cpu()
         return torch.cat([g[: self.C], g[_lib.GRAD_STRIDE - 1: _lib.GRAD_STRIDE]])
 

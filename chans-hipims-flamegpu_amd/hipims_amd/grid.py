@@ -111,14 +111,14 @@ class GridSolver:
             p[:, : L.nx] = np.where(L.valid, r, fill)
             return torch.from_numpy(p).to(dev)
 
-        self.h = plane(h)
-        self.qx = plane(qx)
-        self.qy = plane(qy)
+        # state and flux accumulators each in one allocation (planes are views),
+        # so a whole state or flux can be saved or restored with one copy
+        self.U = torch.stack([plane(h), plane(qx), plane(qy)])
+        self.h, self.qx, self.qy = self.U[0], self.U[1], self.U[2]
         self.z = plane(z)
         self.h_max = self.h.clone()
-        self.dh = torch.zeros_like(self.h)
-        self.dqx = torch.zeros_like(self.h)
-        self.dqy = torch.zeros_like(self.h)
+        self.D = torch.zeros_like(self.U)
+        self.dh, self.dqx, self.dqy = self.D[0], self.D[1], self.D[2]
         self.h_out = self.qx_out = self.qy_out = None  # allocated by step_fused()
         self.type_slot = torch.from_numpy(L.type_slot).to(dev)
         self.slot_code = torch.from_numpy(L.slot_code).to(dev)

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define HP_ABI_VERSION 3
+#define HP_ABI_VERSION 4
 
 enum hp_status {
   HP_OK = 0,
@@ -251,9 +251,13 @@ int hp_ensemble_step(const hp_ensemble* e, int32_t nsteps, void* stream);
  * domain must hold the state before the step, and its control block the t
  * and dt of that step.  cot_* are scratch planes like h; xc / yc are 12
  * planes each of (ny + 1) rows x cpitch (cpitch >= nx + 1) for the edge
- * contributions.  grads: FP64 [hp_adjoint_grad_blocks][16] running sums:
- * column c < 15 the gradient w.r.t. the Manning value of class c, column 15
- * the gradient w.r.t. the rain multiplier (rain_scale).
+ * contributions.  grads: FP64 [hp_adjoint_grad_blocks][16]: column 15 the
+ * running sums of the gradient w.r.t. the rain multiplier (rain_scale),
+ * columns c < 15 the gradient w.r.t. the Manning value of class c, written
+ * by hp_adjoint_finish from gn, the per-cell FP64 running sum (a plane like
+ * h) of the derivative w.r.t. the Manning value of the cell's class.
+ * flux_ready 1: dh/dqx/dqy already hold the step's flux (saved when the
+ * step was recomputed), so hp_adjoint_step does not recompute it.
  * slope_mode 0 differentiates the minmod-limited MUSCL slopes exactly (the
  * exact derivative of the step; over long horizons its frozen-branch
  * linearisation grows, as that of MUSCL with forward Euler does); 1 holds
@@ -265,12 +269,15 @@ typedef struct hp_adjoint_desc {
   void *xc, *yc;
   int64_t cpitch;
   double* grads;
+  double* gn;
   int32_t slope_mode;
-  int32_t reserved;
+  int32_t flux_ready;
 } hp_adjoint_desc;
 
 int64_t hp_adjoint_grad_blocks(const hp_grid_desc* g);
 int hp_adjoint_step(const hp_grid_desc* g, const hp_adjoint_desc* a, void* stream);
+/* Fold gn into the per-class columns of grads (once, after the reverse sweep). */
+int hp_adjoint_finish(const hp_grid_desc* g, const hp_adjoint_desc* a, void* stream);
 /* lam_h[offsets[k]] += weight * 2 * (h_rec[k] - obs[k]), k < n: the
  * cotangent of a squared-misfit loss at n gauge cells. */
 int hp_adjoint_add_loss(int dtype, void* lam_h, const int64_t* offsets, const double* h_rec, const double* obs,
