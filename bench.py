@@ -328,11 +328,29 @@ def adjoint_mode(a):
     grad = cal.gradient()
     torch.cuda.synchronize()
     rev_s = time.perf_counter() - t0
+    # the same gradient keeping every state and flux of the forward run
+    # (no recomputation; K x 6 planes of HBM)
+    cal_all = Calibration(solver(), gauges, a.adj_steps, 1, obs=cal.obs, slopes=a.adj_slopes)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    cal_all.run(th0)
+    torch.cuda.synchronize()
+    run_all_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    grad_all = cal_all.gradient()
+    torch.cuda.synchronize()
+    rev_all_s = time.perf_counter() - t0
+    store_all = {"recorded_forward_s": run_all_s, "reverse_s": rev_all_s,
+                 "cost_ratio": (run_all_s + rev_all_s) / fwd_s,
+                 "stored_bytes": sum(t.numel() * t.element_size() for t in cal_all.ckpts.values()) * 2,
+                 "grad_equal": bool(torch.equal(grad_all, grad))}
+    del cal_all
+    torch.cuda.empty_cache()
     line = {"mode": "adjoint", "size": a.adj_size, "order": a.adj_order, "slopes": a.adj_slopes,
             "steps": a.adj_steps, "checkpoint": a.adj_checkpoint,
             "gauges": int(len(gauges)), "dtype": "f32 state, f64 gradient sums", "loss": float(loss),
             "grad": [float(v) for v in grad], "plain_forward_s": fwd_s, "recorded_forward_s": run_s,
-            "reverse_s": rev_s, "cost_ratio": (run_s + rev_s) / fwd_s}
+            "reverse_s": rev_s, "cost_ratio": (run_s + rev_s) / fwd_s, "store_all": store_all}
     if not a.no_fd:
         fd = []
         for j in range(th0.numel()):
