@@ -55,6 +55,19 @@ def profiled_traffic(kernel: str, workload: str):
     return None
 
 
+def workload_name(a) -> str:
+    if a.case == "runoff":
+        return f"rainfall_runoff_{a.size}x{a.size}_dx5_{a.dtype}_order{a.order}"
+    return f"roofline_{a.size}x{a.size}_dx2_rainfield_{a.dtype}_order{a.order}"
+
+
+def workload_tag(a) -> str:
+    """Workload part of the profiles/ file names."""
+    if a.case == "runoff":
+        return f"rainfall_runoff_{a.size}_{a.dtype}_order{a.order}"
+    return f"roofline_{a.size}_{a.dtype}_order{a.order}"
+
+
 def time_two_pass(g, steps):
     """Two-pass step (flux kernel; update + finalize), HIP events between the
     kernels on the launch stream."""
@@ -106,6 +119,9 @@ def parse():
     p.add_argument("--steps", type=int, default=1000)
     p.add_argument("--warmup", type=int, default=50)
     p.add_argument("--size", type=int, default=2048)
+    p.add_argument("--case", choices=["runoff", "roofline"], default="runoff",
+                   help="runoff: dx 5 m, uniform 50 mm/h station rain; roofline: dx 2 m, static Gaussian "
+                        "random rain field (mean 30, std 10 mm/h)")
     p.add_argument("--dtype", choices=["fp32", "fp64"], default="fp32")
     p.add_argument("--order", type=int, choices=[1, 2], default=2)
     p.add_argument("--tile-rows", type=int, default=0, help="flux tile height (0: the library default)")
@@ -342,7 +358,7 @@ def adjoint_mode(a):
     rev_all_s = time.perf_counter() - t0
     store_all = {"recorded_forward_s": run_all_s, "reverse_s": rev_all_s,
                  "cost_ratio": (run_all_s + rev_all_s) / fwd_s,
-                 "stored_bytes": sum(t.numel() * t.element_size() for t in cal_all.ckpts.values()) * 2,
+                 "stored_bytes": 2 * cal_all.segU.numel() * cal_all.segU.element_size(),
                  "grad_equal": bool(torch.equal(grad_all, grad))}
     del cal_all
     torch.cuda.empty_cache()
@@ -388,10 +404,14 @@ def main():
     torch.cuda.set_device(dev)
     dtype = torch.float32 if a.dtype == "fp32" else torch.float64
     esize = 4 if dtype == torch.float32 else 8
-    case = gen.rainfall_runoff(a.size, 5.0)
+    case = gen.rainfall_runoff(a.size, 5.0) if a.case == "runoff" else gen.roofline_case(a.size, 2.0)
     g = GridSolver(case.mask, case.h, case.qx, case.qy, case.z, case.dx, cfl=case.cfl, order=a.order, dtype=dtype,
                    device=dev, first_dt=case.first_dt, manning=case.manning, landuse=case.landuse)
-    g.set_station_rain(case.rain_table)
+    if case.rain_table is not None:
+        g.set_station_rain(case.rain_table)
+    else:
+        g.set_rain_field(case.rain_field)
+    field = case.rain_table is None
     if a.tile_rows:
         g.set_tile_rows(a.tile_rows)
     cells = case.n_valid
@@ -403,7 +423,7 @@ def main():
     ctrl = g.ctrl_dict()
     assert np.isfinite(ctrl["t"]) and ctrl["dt"] > 0
 
-    B = alg_bytes(esize, rain_field=False)
+    B = alg_bytes(esize, rain_field=field)
     best = fused if fused["total_ms"] < two["total_ms"] else two
     ms_step = best["total_ms"] / best["steps"]
     value = cells * best["steps"] / (best["total_ms"] / 1e3)
@@ -429,14 +449,15 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32" if esize == 4 else "f64",
-        "data": "synthetic (seeded fractal DEM, Voronoi landcover, uniform rain)",
-        "config": {"workload": f"rainfall_runoff_{a.size}x{a.size}_dx5_{a.dtype}_order{a.order}",
-                   "cells": cells, "dx_m": 5.0, "order": a.order, "rain_mm_per_h": 50.0,
+        "data": "synthetic (seeded fractal DEM, Voronoi landcover, " + ("uniform rain)" if not field else
+                                                                          "Gaussian random rain field)"),
+        "config": {"workload": workload_name(a), "cells": cells, "dx_m": case.dx, "order": a.order,
+                   "rain": "station 50 mm/h" if not field else "field 30 +- 10 mm/h",
                    "tile_rows": a.tile_rows or (8 if esize == 4 else 4),
                    "sim_time_s": ctrl["t"]},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": profiled_traffic(dom, f"rainfall_runoff_{a.size}_{a.dtype}_order{a.order}"),
+                     "traffic": profiled_traffic(dom, workload_tag(a)),
                      "alg_bytes_per_cell": B[dom], "kernel_ms": per_kernel[dom]},
         "kernels_ms": per_kernel,
         "path": best["path"],
@@ -476,7 +497,9 @@ def cpu_baseline(a, g, case, dtype):
     secs = sim.last_seconds
     return {"value": sim.N * n / secs, "unit": "cell-updates/s", "cores": threads, "kind": "port",
             "sample": f"{n} steps of the same {a.size}x{a.size} workload from the GPU's warmed state "
-                      f"({secs:.1f} s, oracle C++/OpenMP, {np_dt.__name__})"}
+                      f"({secs:.1f} s, oracle C++/OpenMP, {np_dt.__name__}"
+                      + (")" if case.rain_table is not None else "; the oracle driver has no rain-field source, "
+                         "so its steps run without rain)")}
 
 
 if __name__ == "__main__":

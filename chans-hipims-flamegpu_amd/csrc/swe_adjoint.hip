@@ -229,30 +229,25 @@ __device__ inline void to_global_adj(const FrameAdj<T>& f, int d, T& ax, T& ay) 
 // One edge per thread.  Edges [0, nxe) are x-edges (row y, between columns
 // j-1 = A and j = B, j in [0, nx]); the rest are y-edges (column x, between
 // rows i-1 = A and i = B, i in [0, ny]).  Writes 12 cotangents per edge.
-template <class T, int ORDER>
+template <class T, int ORDER, int DIR>
 __global__ __launch_bounds__(kAThreads) void k_adj_edges(hp_grid_desc a, AdjPlanes<T> p) {
-  const int64_t e = (int64_t)blockIdx.x * kAThreads + threadIdx.x;
+  // DIR 0: x-edges, grid (ceil((nx+1)/256), ny); DIR 3: y-edges, grid
+  // (ceil(nx/256), ny+1).  The direction is a compile-time constant, so the
+  // frame rotations of the interior edges fold away.
   const int nx = a.nx, ny = a.ny;
-  const int64_t nxe = (int64_t)(nx + 1) * ny;
-  const int64_t nye = (int64_t)nx * (ny + 1);
-  if (e >= nxe + nye) return;
-  const bool xe = e < nxe;
-  int yA, xA, yB, xB, yl, xl, yr2, xr2, dAB;
-  int64_t slot;
+  const int col = blockIdx.x * kAThreads + threadIdx.x, row = blockIdx.y;
+  constexpr bool xe = DIR == 0;
+  if (col >= (xe ? nx + 1 : nx) || row >= (xe ? ny : ny + 1)) return;
+  int yA, xA, yB, xB, yl, xl, yr2, xr2;
+  constexpr int dAB = DIR;
   if (xe) {
-    const int y = (int)(e / (nx + 1)), j = (int)(e - (int64_t)y * (nx + 1));
-    yA = yB = yl = yr2 = y;
-    xA = j - 1; xB = j; xl = j - 2; xr2 = j + 1;
-    dAB = 0;
-    slot = (int64_t)y * p.cpitch + j;
+    yA = yB = yl = yr2 = row;
+    xA = col - 1; xB = col; xl = col - 2; xr2 = col + 1;
   } else {
-    const int64_t f = e - nxe;
-    const int i = (int)(f / nx), x = (int)(f - (int64_t)i * nx);
-    xA = xB = xl = xr2 = x;
-    yA = i - 1; yB = i; yl = i - 2; yr2 = i + 1;
-    dAB = 3;
-    slot = (int64_t)i * p.cpitch + x;
+    xA = xB = xl = xr2 = col;
+    yA = row - 1; yB = row; yl = row - 2; yr2 = row + 1;
   }
+  const int64_t slot = (int64_t)row * p.cpitch + col;
   T* out = xe ? p.xc : p.yc;
   FrameAdj<T> adj[4] = {zero_adj<T>(), zero_adj<T>(), zero_adj<T>(), zero_adj<T>()};  // l, A, B, rr
   int dir[4] = {dAB, dAB, dAB, dAB};
@@ -399,10 +394,15 @@ static int launch_adjoint(const hp_grid_desc* g, const hp_adjoint_desc* ad, hipS
   hipLaunchKernelGGL((k_adj_update<T>), dim3(nbu), dim3(kAThreads), 0, st, *g, p, ad->grads, ad->gn);
   int rc = check_launch("k_adj_update");
   if (rc) return rc;
-  const int64_t nedges = (int64_t)(g->nx + 1) * g->ny + (int64_t)g->nx * (g->ny + 1);
-  const unsigned nbe = (unsigned)((nedges + kAThreads - 1) / kAThreads);
-  if (g->order == 2) hipLaunchKernelGGL((k_adj_edges<T, 2>), dim3(nbe), dim3(kAThreads), 0, st, *g, p);
-  else hipLaunchKernelGGL((k_adj_edges<T, 1>), dim3(nbe), dim3(kAThreads), 0, st, *g, p);
+  const dim3 gx((unsigned)((g->nx + 1 + kAThreads - 1) / kAThreads), (unsigned)g->ny);
+  const dim3 gy((unsigned)((g->nx + kAThreads - 1) / kAThreads), (unsigned)(g->ny + 1));
+  if (g->order == 2) {
+    hipLaunchKernelGGL((k_adj_edges<T, 2, 0>), gx, dim3(kAThreads), 0, st, *g, p);
+    hipLaunchKernelGGL((k_adj_edges<T, 2, 3>), gy, dim3(kAThreads), 0, st, *g, p);
+  } else {
+    hipLaunchKernelGGL((k_adj_edges<T, 1, 0>), gx, dim3(kAThreads), 0, st, *g, p);
+    hipLaunchKernelGGL((k_adj_edges<T, 1, 3>), gy, dim3(kAThreads), 0, st, *g, p);
+  }
   rc = check_launch("k_adj_edges");
   if (rc) return rc;
   const unsigned nbg = (unsigned)((total + kAThreads - 1) / kAThreads);
@@ -433,6 +433,7 @@ int hp_adjoint_step(const hp_grid_desc* g, const hp_adjoint_desc* ad, void* stre
   if (g->n_classes > kGradStride - 1) return fail(HP_EINVAL, "the adjoint supports at most 15 Manning classes");
   if (g->infil_ks) return fail(HP_EINVAL, "the adjoint does not cover infiltration");
   if (ad->slope_mode != 0 && ad->slope_mode != 1) return fail(HP_EINVAL, "slope_mode must be 0 or 1");
+  if (g->ny + 1 > 65535) return fail(HP_EINVAL, "the adjoint launches one block row per grid row: ny < 65535");
   if (!ad->flux_ready) {
     int rc = grid_flux_launch(g, (hipStream_t)stream);  // recompute the step's flux on U_k
     if (rc) return rc;

@@ -320,13 +320,14 @@ __host__ __device__ inline void friction_vjp(T h, T n, T dt, T qx, T qy, T a_ox,
     a_qy += a_oy;
     return;
   }
-  const T h13 = pow(h, T(-1.0 / 3.0));
+  using P = Prec<T>;
+  const T h13 = P::rcbrt_friction(h);
   const T C_f = g * n * n * h13;
   const T qn2 = qx * qx + qy * qy;
-  const T q_norm = sqrt(qn2);
-  const T lim = T(5) * h * sqrt(h * g);
+  const T q_norm = P::sqrt_(qn2);
+  const T lim = T(5) * h * P::sqrt_(h * g);
   const T x = (T(4) * dt * C_f) / (h * h) * q_norm;
-  const T S = sqrt(T(1) + x);
+  const T S = P::sqrt_(T(1) + x);
   const T k = T(2) / (T(1) + S);
   T a_k = T(0);
   const T qc[2] = {qx, qy};

@@ -75,9 +75,11 @@ class Calibration:
         self.grads = torch.zeros(self.nblocks, _lib.GRAD_STRIDE, dtype=torch.float64, device=dev)
         self.gn = torch.zeros(shp, dtype=torch.float64, device=dev)
         # states before each step of a segment and the steps' fluxes
-        self.segU = None if self.ck == 1 else torch.zeros((self.ck,) + tuple(s.U.shape), dtype=s.dtype, device=dev)
-        self.segD = None if self.ck == 1 else torch.zeros_like(self.segU)
-        self.fluxes: dict[int, torch.Tensor] = {}
+        # states before each step of a segment and the steps' fluxes (with
+        # checkpoint=1: of every step, filled by run())
+        n_keep = self.K if self.ck == 1 else self.ck
+        self.segU = torch.zeros((n_keep,) + tuple(s.U.shape), dtype=s.dtype, device=dev)
+        self.segD = torch.zeros_like(self.segU)
         ad = HpAdjointDesc()
         ad.lam_h, ad.lam_qx, ad.lam_qy = (t.data_ptr() for t in self.lam)
         ad.cot_h, ad.cot_qx, ad.cot_qy = (t.data_ptr() for t in self.cot)
@@ -126,14 +128,17 @@ class Calibration:
         self.set_theta(theta)
         self._restore_initial()
         self.ckpts.clear()
-        self.fluxes.clear()
         self._record_gauges(0)
         for k in range(self.K):
             if keep_checkpoints and k % self.ck == 0:
-                self.ckpts[k] = self.s.U.clone()
+                if self.ck == 1:
+                    self.segU[k].copy_(self.s.U)
+                    self.ckpts[k] = self.segU[k]
+                else:
+                    self.ckpts[k] = self.s.U.clone()
             self._step(k)
             if keep_checkpoints and self.ck == 1:
-                self.fluxes[k] = self.s.D.clone()
+                self.segD[k].copy_(self.s.D)
             self._record_gauges(k + 1)
         self.sched_ready = True
         return self.loss()
@@ -181,7 +186,7 @@ class Calibration:
             a = starts[si]
             b = min(a + self.ck, self.K)
             if self.ck == 1:
-                states = {a: (self.ckpts[a], self.fluxes[a])}
+                states = {a: (self.segU[a], self.segD[a])}
             else:
                 # recompute the segment: state before each step, then its flux
                 s.U.copy_(self.ckpts[a])
