@@ -55,6 +55,27 @@ def profiled_traffic(kernel: str, workload: str):
     return None
 
 
+def cached_case(size: int):
+    """The roofline case (slow to generate at 8192^2), kept in the temporary
+    directory between runs of one session."""
+    import tempfile
+
+    import numpy as np
+
+    from hipims_amd import generators as gen
+    path = os.path.join(tempfile.gettempdir(), f"hipims_roofline_case_{size}.npz")
+    if os.path.exists(path):
+        with np.load(path, allow_pickle=False) as f:
+            c = gen.Case(f"roofline_{size}", f["mask"], f["h"], f["qx"], f["qy"], f["z"], float(f["dx"]),
+                         f["landuse"], f["manning"], float(f["cfl"]), float(f["first_dt"]), rain_table=None,
+                         rain_field=f["rain_field"])
+        return c
+    c = gen.roofline_case(size, 2.0)
+    np.savez(path, mask=c.mask, h=c.h, qx=c.qx, qy=c.qy, z=c.z, dx=c.dx, landuse=c.landuse, manning=c.manning,
+             cfl=c.cfl, first_dt=c.first_dt, rain_field=c.rain_field)
+    return c
+
+
 def workload_name(a) -> str:
     if a.case == "runoff":
         return f"rainfall_runoff_{a.size}x{a.size}_dx5_{a.dtype}_order{a.order}"
@@ -404,7 +425,7 @@ def main():
     torch.cuda.set_device(dev)
     dtype = torch.float32 if a.dtype == "fp32" else torch.float64
     esize = 4 if dtype == torch.float32 else 8
-    case = gen.rainfall_runoff(a.size, 5.0) if a.case == "runoff" else gen.roofline_case(a.size, 2.0)
+    case = gen.rainfall_runoff(a.size, 5.0) if a.case == "runoff" else cached_case(a.size)
     g = GridSolver(case.mask, case.h, case.qx, case.qy, case.z, case.dx, cfl=case.cfl, order=a.order, dtype=dtype,
                    device=dev, first_dt=case.first_dt, manning=case.manning, landuse=case.landuse)
     if case.rain_table is not None:
