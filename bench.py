@@ -146,6 +146,9 @@ def parse():
     p.add_argument("--dtype", choices=["fp32", "fp64"], default="fp32")
     p.add_argument("--order", type=int, choices=[1, 2], default=2)
     p.add_argument("--tile-rows", type=int, default=0, help="flux tile height (0: the library default)")
+    p.add_argument("--mass-sums", action="store_true",
+                   help="also keep the FP64 volume / rain / boundary-flux sums each step (diagnostics the "
+                        "reference step does not compute)")
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     p.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, available cores)")
     p.add_argument("--no-cpu", action="store_true")
@@ -435,6 +438,7 @@ def main():
     field = case.rain_table is None
     if a.tile_rows:
         g.set_tile_rows(a.tile_rows)
+    g.set_mass_sums(a.mass_sums)
     cells = case.n_valid
 
     g.step(a.warmup)
@@ -474,7 +478,7 @@ def main():
                                                                           "Gaussian random rain field)"),
         "config": {"workload": workload_name(a), "cells": cells, "dx_m": case.dx, "order": a.order,
                    "rain": "station 50 mm/h" if not field else "field 30 +- 10 mm/h",
-                   "tile_rows": a.tile_rows or (8 if esize == 4 else 4),
+                   "tile_rows": a.tile_rows or (8 if esize == 4 else 4), "mass_sums": a.mass_sums,
                    "sim_time_s": ctrl["t"]},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,

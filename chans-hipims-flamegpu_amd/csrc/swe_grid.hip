@@ -42,6 +42,7 @@ constexpr int kTX = 64;
 constexpr int kThreads = 256;
 constexpr int kChunk = 4096;  // plane elements per update block
 constexpr int kMaxStations = 1024;
+constexpr int kRateCache = 64;  // station rates kept in LDS per block; further stations are evaluated per cell
 
 __device__ inline bool stopped(const hp_ctrl* c) { return c->t_stop > 0.0 && c->t >= c->t_stop; }
 
@@ -217,13 +218,15 @@ __device__ inline void solve_tile_edge(const FluxTile<T, TY>& s, const hp_grid_d
 
 // Fold a block's CFL minimum and FP64 sums into one partial record.
 template <class T>
-__device__ inline void block_partial(T mn, double vol, double fsum, double rsum, Partial* part) {
+__device__ inline void block_partial(T mn, double vol, double fsum, double rsum, Partial* part, bool sums) {
   __shared__ double s_red[4][kThreads / 64];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   mn = wave_min(mn);
-  vol = wave_sum(vol);
-  fsum = wave_sum(fsum);
-  rsum = wave_sum(rsum);
+  if (sums) {
+    vol = wave_sum(vol);
+    fsum = wave_sum(fsum);
+    rsum = wave_sum(rsum);
+  }
   if (lane == 0) {
     s_red[0][w] = (double)mn;
     s_red[1][w] = vol;
@@ -246,7 +249,8 @@ __device__ inline void block_partial(T mn, double vol, double fsum, double rsum,
 // Per-cell inputs of the update that do not depend on the fluxes; loaded
 // before the flux work so their latency hides behind it.
 template <class T> struct CellIn {
-  T rain;   // rain rate (m/s)
+  T rain;   // rain rate (m/s) of a field; of a station once resolved
+  int st;   // station index (station rain)
   T hmax;   // h_max before the step
   T n;      // Manning n of the cell's class
   T F;      // cumulative infiltrated depth (when infiltration is on)
@@ -300,6 +304,7 @@ template <class T, int ORDER, int TY, bool FUSED>
 __device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by, Partial* part) {
   __shared__ FluxTile<T, TY> s;
   __shared__ T s_man[FUSED ? 256 : 1];
+  __shared__ T s_rate[FUSED ? kRateCache : 1];
   using Tile = FluxTile<T, TY>;
   constexpr int LH = Tile::LH;
   constexpr int CPT = TY * kTX / kThreads;  // own cells per thread
@@ -321,20 +326,23 @@ __device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by, Partial*
 
   // the thread's own cells: column x0 + lane, rows y0 + wv + 4k
   CellIn<T> cin[FUSED ? CPT : 1];
+  RainRows rr{};
   if (FUSED) {
     for (int k = tid; k < a.n_classes; k += kThreads) s_man[k] = ((const T*)a.manning)[k];
-    const RainRows rr = rain_rows(a, a.ctrl->t);
+    rr = rain_rows(a, a.ctrl->t);
+    if (a.rain_mode == HP_RAIN_STATION && tid < kRateCache) s_rate[tid] = station_rate<T>(rr, tid, a.rain_scale);
     const int gx = x0 + lane;
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
       const int gy = y0 + wv + 4 * k;
-      cin[k] = CellIn<T>{T(0), T(0), T(0), T(0), 0, false};
+      cin[k] = CellIn<T>{T(0), 0, T(0), T(0), T(0), 0, false};
       if (gx < nx && gy < ny) {
         const int64_t gi = (int64_t)gy * pitch + gx;
         const int cls = a.landuse[gi];
         cin[k].on = cls != 255;
         cin[k].cls = cls;
-        cin[k].rain = cell_rain<T>(a, rr, gi);
+        if (a.rain_mode == HP_RAIN_STATION) cin[k].st = a.station ? (int)a.station[gi] : 0;
+        else if (a.rain_mode == HP_RAIN_FIELD) cin[k].rain = T(a.rain_field[gi]) * T(a.rain_scale);
         cin[k].hmax = ((const T*)a.h_max)[gi];
         if (a.infil_ks) cin[k].F = ((const T*)a.infil_F)[gi];
       }
@@ -411,6 +419,7 @@ __device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by, Partial*
   T* __restrict__ dqx = (T*)a.dqx;
   T* __restrict__ dqy = (T*)a.dqy;
   const T cfl_dx = T(a.cfl) * T(a.dx);
+  const bool sums = a.mass_sums != 0;
   T mn = T(INFINITY);
   double vol = 0.0, fsum = 0.0, rsum = 0.0;
 #pragma unroll
@@ -457,6 +466,8 @@ __device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by, Partial*
     } else {
       CellIn<T> in = cin[k];
       in.n = s_man[in.cls];
+      if (a.rain_mode == HP_RAIN_STATION)
+        in.rain = (in.st >= 0 && in.st < kRateCache) ? s_rate[in.st] : station_rate<T>(rr, in.st, a.rain_scale);
       T hn = s.h[r][c], qxn = s.qx[r][c], qyn = s.qy[r][c], hm = in.hmax;
       // infiltration over the flux list (SWE_CUDA.py:439-443)
       T dF = T(0);
@@ -468,16 +479,18 @@ __device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by, Partial*
       T cand, vh, fh, rh;
       update_state<T>(in, hn, qxn, qyn, ah, aqx, aqy, dF, dt, cfl_dx, hm, cand, vh, fh, rh);
       mn = fmin(mn, cand);
-      vol += (double)vh;
-      fsum += (double)fh;
-      rsum += (double)rh;
+      if (sums) {
+        vol += (double)vh;
+        fsum += (double)fh;
+        rsum += (double)rh;
+      }
       ((T*)a.h_out)[gi] = hn;
       ((T*)a.qx_out)[gi] = qxn;
       ((T*)a.qy_out)[gi] = qyn;
       if (hm != in.hmax) ((T*)a.h_max)[gi] = hm;
     }
   }
-  if (FUSED) block_partial<T>(mn, vol, fsum, rsum, part);
+  if (FUSED) block_partial<T>(mn, vol, fsum, rsum, part, sums);
 }
 
 // Workgroups of a launch are handed to the 8 XCDs round-robin (block b runs
@@ -565,6 +578,7 @@ __global__ __launch_bounds__(kThreads) void k_infiltration(hp_grid_desc a) {
 template <class T, int MODE>
 __device__ inline void update_chunk(const hp_grid_desc& a, int64_t chunk, Partial* __restrict__ part) {
   __shared__ T s_man[256];
+  __shared__ T s_rate[kRateCache];
   const int tid = threadIdx.x;
   if (stopped(a.ctrl)) {
     if (tid == 0) *part = Partial{INFINITY, 0.0, 0.0, 0.0};
@@ -576,6 +590,7 @@ __device__ inline void update_chunk(const hp_grid_desc& a, int64_t chunk, Partia
   for (int k = tid; k < a.n_classes; k += kThreads) s_man[k] = ((const T*)a.manning)[k];
   const RainRows rr = rain_rows(a, a.ctrl->t);
   const int mode = a.rain_mode;
+  if (mode == HP_RAIN_STATION && tid < kRateCache) s_rate[tid] = station_rate<T>(rr, tid, a.rain_scale);
   __syncthreads();
 
   const T* __restrict__ dhp = (const T*)a.dh;
@@ -591,6 +606,7 @@ __device__ inline void update_chunk(const hp_grid_desc& a, int64_t chunk, Partia
   const T* __restrict__ dfp = (const T*)a.infil_dF;
   const T rscale = T(a.rain_scale);
 
+  const bool sums = a.mass_sums != 0;
   T mn = T(INFINITY);
   double vol = 0.0, fsum = 0.0, rsum = 0.0;
   const int64_t total = a.pitch * (int64_t)a.ny;  // a multiple of 64
@@ -613,7 +629,10 @@ __device__ inline void update_chunk(const hp_grid_desc& a, int64_t chunk, Partia
         VecI16 vs{};
         if (st) vs = ldv<VecI16>(st + i0);
 #pragma unroll
-        for (int j = 0; j < kVec; ++j) vr.x[j] = station_rate<T>(rr, vs.x[j], a.rain_scale);
+        for (int j = 0; j < kVec; ++j) {
+          const int k = vs.x[j];
+          vr.x[j] = (k >= 0 && k < kRateCache) ? s_rate[k] : station_rate<T>(rr, k, a.rain_scale);
+        }
       } else if (mode == HP_RAIN_FIELD) {
         const VecF32 vf = ldv<VecF32>(rf + i0);
 #pragma unroll
@@ -630,7 +649,7 @@ __device__ inline void update_chunk(const hp_grid_desc& a, int64_t chunk, Partia
           vhm.x[j] = fmax(h, vhm.x[j]);
           mn = fmin(mn, cfl_candidate<T>(h, vqx.x[j], vqy.x[j], cfl_dx));
         }
-        vol += (double)h;
+        if (sums) vol += (double)h;
         continue;
       }
       const T dhf = vdh.x[j];
@@ -638,7 +657,7 @@ __device__ inline void update_chunk(const hp_grid_desc& a, int64_t chunk, Partia
       const T dht = (dhf + rdt) - vdf.x[j];  // flux, + rain, - infiltration (the reference's order)
       const bool listed = (MODE == HP_UPDATE_EULER) ? (fabs(dht) > T(0) || h > hs) : (fabs(dht) > T(0) || h >= T(0));
       if (!listed) {
-        vol += (double)h;
+        if (sums) vol += (double)h;
         continue;
       }
       h += dht;
@@ -661,9 +680,11 @@ __device__ inline void update_chunk(const hp_grid_desc& a, int64_t chunk, Partia
         vhm.x[j] = fmax(h, vhm.x[j]);
         mn = fmin(mn, cfl_candidate<T>(h, qx, qy, cfl_dx));
       }
-      vol += (double)h;
-      fsum += (double)dhf;
-      rsum += (double)rdt;
+      if (sums) {
+        vol += (double)h;
+        fsum += (double)dhf;
+        rsum += (double)rdt;
+      }
     }
     if (MODE != HP_UPDATE_CFL) {
       stv(hp_ + i0, vh);
@@ -672,7 +693,7 @@ __device__ inline void update_chunk(const hp_grid_desc& a, int64_t chunk, Partia
     }
     if (MODE == HP_UPDATE_STEP || MODE == HP_UPDATE_CFL) stv(hmp + i0, vhm);
   }
-  block_partial<T>(mn, vol, fsum, rsum, part);
+  block_partial<T>(mn, vol, fsum, rsum, part, sums);
 }
 
 template <class T, int MODE>
@@ -688,7 +709,7 @@ __global__ __launch_bounds__(kThreads) void k_update_multi(const hp_grid_desc* _
 }
 
 // Fold a domain's partials and advance its time control.
-__device__ inline void finalize_domain(hp_ctrl* ctrl, const Partial* parts, int n, double dx) {
+__device__ inline void finalize_domain(hp_ctrl* ctrl, const Partial* parts, int n, double dx, bool sums) {
   __shared__ double s_red[4][16];
   const int tid = threadIdx.x;
   if (stopped(ctrl)) return;
@@ -738,22 +759,25 @@ __device__ inline void finalize_domain(hp_ctrl* ctrl, const Partial* parts, int 
     }
     ctrl->dt = dt_new;
     ctrl->t = t + dt_new;
-    ctrl->volume = vol * area;
-    ctrl->flux_volume += fs * area;
-    ctrl->rain_volume += rs * area;
+    if (sums) {
+      ctrl->volume = vol * area;
+      ctrl->flux_volume += fs * area;
+      ctrl->rain_volume += rs * area;
+    }
     ctrl->steps += 1.0;
   }
 }
 
-__global__ __launch_bounds__(1024) void k_finalize(hp_ctrl* ctrl, const Partial* parts, int n, double dx) {
-  finalize_domain(ctrl, parts, n, dx);
+__global__ __launch_bounds__(1024) void k_finalize(hp_ctrl* ctrl, const Partial* parts, int n, double dx,
+                                                   int sums) {
+  finalize_domain(ctrl, parts, n, dx, sums != 0);
 }
 
 __global__ __launch_bounds__(256) void k_finalize_multi(const hp_grid_desc* __restrict__ doms,
                                                         const int2* __restrict__ ranges, const Partial* parts) {
   const int2 r = ranges[blockIdx.x];
   const hp_grid_desc& a = doms[blockIdx.x];
-  finalize_domain(a.ctrl, parts + r.x, r.y, a.dx);
+  finalize_domain(a.ctrl, parts + r.x, r.y, a.dx, a.mass_sums != 0);
 }
 
 // --- host side -------------------------------------------------------------
@@ -885,7 +909,8 @@ static int launch_update(const hp_grid_desc* g, int mode, hipStream_t st) {
   int rc = check_launch("k_update");
   if (rc) return rc;
   if (mode == HP_UPDATE_STEP || mode == HP_UPDATE_CFL) {
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, st, g->ctrl, (const Partial*)g->work, (int)nb, g->dx);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, st, g->ctrl, (const Partial*)g->work, (int)nb, g->dx,
+                       g->mass_sums);
     return check_launch("k_finalize");
   }
   return HP_OK;
@@ -992,7 +1017,8 @@ int hp_grid_step_fused(const hp_grid_desc* g, int32_t nsteps, void* stream) {
   for (int32_t k = 0; k < nsteps; ++k) {
     rc = launch_step_fused(&d, st);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, st, d.ctrl, (const Partial*)d.work, (int)nt, d.dx);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, st, d.ctrl, (const Partial*)d.work, (int)nt, d.dx,
+                       d.mass_sums);
     rc = check_launch("k_finalize");
     if (rc) return rc;
     std::swap(d.h, d.h_out);

@@ -58,7 +58,10 @@ template <> struct Prec<float> {
   static constexpr bool local_datum = true;
   __device__ static inline float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
   __device__ static inline float sqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
-  __device__ static inline float rcbrt_friction(float h) { return rcbrtf(h); }
+  // h^(-1/3) = exp2(-log2(h) / 3) with the hardware log2 / exp2 (h >= 1e-6)
+  __device__ static inline float rcbrt_friction(float h) {
+    return __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(h) * (-1.0f / 3.0f));
+  }
   // x * (dt / dx) with the quotient computed once per block
   __device__ static inline float scale(float x, float, float, float dt_dx) { return x * dt_dx; }
   // sqrt(g * base^2 / g) is |base| exactly; no square root needed
@@ -305,24 +308,14 @@ __host__ __device__ inline void implicit_friction(T h, T n, T dt, T& qx, T& qy) 
     // cancellation-free: (1 - sqrt(1 + x)) / (-(x/2) / q_k) == 2 q_k / (1 + sqrt(1 + x))
     const T x = P::div(T(4) * dt * C_f, h * h) * q_norm;
     const T k = P::div(T(2), T(1) + P::sqrt_(T(1) + x));
-    if (fabs(qx) > T(1.0e-10)) {
-      const T tmp = k * qx;
-      if (fabs(tmp) > lim) {
-        qx = T(2) * tmp - qx;
-        if (qx * tmp <= T(0)) qx = T(0);
-      } else {
-        qx = tmp;
-      }
-    }
-    if (fabs(qy) > T(1.0e-10)) {
-      const T tmp = k * qy;
-      if (fabs(tmp) > lim) {
-        qy = T(2) * tmp - qy;
-        if (qy * tmp <= T(0)) qy = T(0);
-      } else {
-        qy = tmp;
-      }
-    }
+    // per component, as selects: q' = k q, or past the Froude limit the
+    // reflected 2 q' - q, zeroed on a sign change; |q| <= 1e-10 unchanged
+    const T tx = k * qx, ty = k * qy;
+    const T rx = T(2) * tx - qx, ry = T(2) * ty - qy;
+    const T ox = fabs(tx) > lim ? (rx * tx <= T(0) ? T(0) : rx) : tx;
+    const T oy = fabs(ty) > lim ? (ry * ty <= T(0) ? T(0) : ry) : ty;
+    qx = fabs(qx) > T(1.0e-10) ? ox : qx;
+    qy = fabs(qy) > T(1.0e-10) ? oy : qy;
   } else {
     if (fabs(qx) > T(1.0e-10)) {
       const T tmp = (T(1) - P::sqrt_(T(1) + (T(4) * dt * C_f) / (h * h) * q_norm)) /

@@ -66,12 +66,4 @@ template <class T> __device__ inline T station_rate(const RainRows& rr, int k, d
   return T((lo + rr.per * (hi - lo)) * scale);
 }
 
-// Rain rate of plane element i for the descriptor's rain mode.
-template <class T>
-__device__ inline T cell_rain(const hp_grid_desc& a, const RainRows& rr, int64_t i) {
-  if (a.rain_mode == HP_RAIN_STATION) return station_rate<T>(rr, a.station ? (int)a.station[i] : 0, a.rain_scale);
-  if (a.rain_mode == HP_RAIN_FIELD) return T(a.rain_field[i]) * T(a.rain_scale);
-  return T(0);
-}
-
 }  // namespace hp

@@ -22,7 +22,7 @@ HP_OK, HP_EINVAL, HP_ELAUNCH, HP_ESPACE = 0, -1, -2, -3
 HP_F32, HP_F64 = 0, 1
 HP_RAIN_NONE, HP_RAIN_STATION, HP_RAIN_FIELD = 0, 1, 2
 HP_UPDATE_STEP, HP_UPDATE_FRICTION, HP_UPDATE_EULER, HP_UPDATE_CFL = 0, 1, 2, 3
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 _P = ctypes.c_void_p
 _I64 = ctypes.c_int64
@@ -48,7 +48,7 @@ class HpGridDesc(ctypes.Structure):
         ("dtype", _I32), ("order", _I32), ("nx", _I32), ("ny", _I32), ("pitch", _I64),
         ("dx", ctypes.c_double), ("cfl", ctypes.c_double),
         ("cell0_x", _I32), ("cell0_y", _I32), ("n_classes", _I32), ("rain_mode", _I32),
-        ("tile_rows", _I32), ("reserved_i32", _I32),
+        ("tile_rows", _I32), ("mass_sums", _I32),
         ("h", _P), ("qx", _P), ("qy", _P), ("z", _P), ("h_max", _P),
         ("dh", _P), ("dqx", _P), ("dqy", _P),
         ("h_out", _P), ("qx_out", _P), ("qy_out", _P),

@@ -92,7 +92,8 @@ class GridSolver:
 
     def __init__(self, mask, h, qx, qy, z, dx, *, cfl=0.5, order=2, dtype=torch.float64, device="cuda",
                  first_dt=1.0e-4, t=0.0, export_T=math.inf, export_n=0, manning=(0.0,), landuse=None,
-                 given_depth=None, given_wl=None, given_q=None, ctrl=None, tile_rows=0, t_stop=0.0):
+                 given_depth=None, given_wl=None, given_q=None, ctrl=None, tile_rows=0, t_stop=0.0,
+                 mass_sums=True):
         self.device = torch.device(device)
         if self.device.type == "cuda" and self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())
@@ -139,6 +140,7 @@ class GridSolver:
         if t_stop:
             self.ctrl[_lib.CTRL_T_STOP] = float(t_stop)
         self.tile_rows = int(tile_rows)
+        self.mass_sums = bool(mass_sums)
         self.given_depth = self.given_wl = self.given_q = None
         self.set_boundary(given_depth, given_wl, given_q)
         self.station = None
@@ -229,6 +231,12 @@ class GridSolver:
         self.tile_rows = int(rows)
         self._desc = None
 
+    def set_mass_sums(self, on: bool):
+        """Keep the FP64 volume / rain / boundary-flux sums of the control
+        block up to date (on by default; the reference computes none)."""
+        self.mass_sums = bool(on)
+        self._desc = None
+
     def set_t_stop(self, t_stop: float):
         """The passes of this domain do nothing once t >= t_stop (0: never)."""
         self.ctrl[_lib.CTRL_T_STOP] = float(t_stop)
@@ -273,6 +281,7 @@ class GridSolver:
         d.n_classes = int(self.manning.numel())
         d.rain_mode = self.rain_mode
         d.tile_rows = self.tile_rows
+        d.mass_sums = int(self.mass_sums)
         for f in ("h", "qx", "qy", "z", "h_max", "dh", "dqx", "dqy", "type_slot", "landuse", "slot_code",
                   "manning"):
             setattr(d, f, getattr(self, f).data_ptr())

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define HP_ABI_VERSION 4
+#define HP_ABI_VERSION 5
 
 enum hp_status {
   HP_OK = 0,
@@ -143,7 +143,8 @@ typedef struct hp_grid_desc {
   int32_t n_classes;        /* entries of manning */
   int32_t rain_mode;        /* hp_rain_mode */
   int32_t tile_rows;        /* flux tile height: float 4, 8 or 16, double 4 (0: 8 / 4) */
-  int32_t reserved_i32;
+  int32_t mass_sums;        /* 1: keep hp_ctrl's FP64 volume / rain / boundary-flux sums up to
+                               date (diagnostics the reference does not compute); 0: skip them */
   /* state planes (dtype) */
   void *h, *qx, *qy, *z, *h_max;
   /* flux accumulators written by the flux pass (dtype) */
