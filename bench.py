@@ -145,7 +145,7 @@ def parse():
                         "random rain field (mean 30, std 10 mm/h)")
     p.add_argument("--dtype", choices=["fp32", "fp64"], default="fp32")
     p.add_argument("--order", type=int, choices=[1, 2], default=2)
-    p.add_argument("--tile-rows", type=int, default=0, help="flux tile height (0: the library default)")
+    p.add_argument("--tile-rows", type=int, default=0, help="flux tile height (0: the fastest of the legal heights, timed per step path)")
     p.add_argument("--mass-sums", action="store_true",
                    help="also keep the FP64 volume / rain / boundary-flux sums each step (diagnostics the "
                         "reference step does not compute)")
@@ -436,14 +436,24 @@ def main():
     else:
         g.set_rain_field(case.rain_field)
     field = case.rain_table is None
-    if a.tile_rows:
-        g.set_tile_rows(a.tile_rows)
     g.set_mass_sums(a.mass_sums)
     cells = case.n_valid
 
     g.step(a.warmup)
     torch.cuda.synchronize()
+    # Tile height per path: the fixed --tile-rows, or the fastest legal one
+    # timed on this domain (the tuner restores the state it steps).
+    rows, tuned = {}, {}
+    for path in ("flux", "fused"):
+        if a.tile_rows:
+            rows[path] = a.tile_rows
+        else:
+            from hipims_amd import autotune
+            rows[path] = autotune.tune(g, path=path, apply=False)
+            tuned[path] = dict(getattr(g, "tune_timings", {}))
+    g.set_tile_rows(rows["flux"])
     two = time_two_pass(g, a.steps)
+    g.set_tile_rows(rows["fused"])
     fused = time_fused(g, a.steps)
     ctrl = g.ctrl_dict()
     assert np.isfinite(ctrl["t"]) and ctrl["dt"] > 0
@@ -478,7 +488,10 @@ def main():
                                                                           "Gaussian random rain field)"),
         "config": {"workload": workload_name(a), "cells": cells, "dx_m": case.dx, "order": a.order,
                    "rain": "station 50 mm/h" if not field else "field 30 +- 10 mm/h",
-                   "tile_rows": a.tile_rows or (8 if esize == 4 else 4), "mass_sums": a.mass_sums,
+                   "tile_rows": rows["fused" if best["path"] == "fused" else "flux"],
+                   "tile_rows_per_path": {"two_pass": rows["flux"], "fused": rows["fused"]},
+                   "tile_rows_tuning_ms": {p: {str(k): v for k, v in t.items()} for p, t in tuned.items()},
+                   "mass_sums": a.mass_sums,
                    "sim_time_s": ctrl["t"]},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,

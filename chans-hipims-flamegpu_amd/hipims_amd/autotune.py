@@ -1,11 +1,13 @@
-"""Flux tile-shape autotuner.
+"""Flux tile-height autotuner.
 
 The flux pass stages 64 x TY cells plus a 2-cell halo in LDS.  A taller tile
-re-reads fewer halo rows but holds more LDS per block (fewer blocks per CU);
-which wins depends on dtype, order and domain size.  tune() times each legal
-TY on the caller's own domain state (restoring the state afterwards) with HIP
-events and keeps the fastest; results are memoised per (dtype, order, size
-class) for the process.
+re-solves fewer halo edges but holds more LDS per block, so fewer blocks fit
+a CU; which wins depends on dtype, order, domain size and on the step path
+(the single-pass step carries the update in the same block).  tune() times
+each legal TY on the caller's own domain with HIP events - the flux pass
+alone, or whole single-pass steps - restores the domain's state afterwards
+and keeps the fastest; results are memoised per (dtype, order, size class,
+path) for the process.
 """
 from __future__ import annotations
 
@@ -17,30 +19,45 @@ CHOICES = {torch.float32: (4, 8, 16), torch.float64: (4,)}
 _cache: dict = {}
 
 
-def _key(s: GridSolver):
+def _key(s: GridSolver, path: str):
     cells = s.layout.ny * s.layout.pitch
-    return (s.dtype, s.order, max(0, cells.bit_length() - 18))
+    return (s.dtype, s.order, max(0, cells.bit_length() - 18), path)
 
 
-def tune(s: GridSolver, reps: int = 20, apply: bool = True) -> int:
-    key = _key(s)
+def _time(s: GridSolver, path: str, reps: int) -> float:
+    run = (lambda: s.step_fused(2)) if path == "fused" else s.flux
+    run()  # warm-up (and the fused path's second buffers)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        run()
+    b.record()
+    torch.cuda.synchronize(s.device)
+    return a.elapsed_time(b) / reps
+
+
+def tune(s: GridSolver, reps: int = 20, apply: bool = True, path: str = "flux") -> int:
+    """Best tile height for `path` ("flux": the two-pass flux pass,
+    "fused": the single-pass step) on this domain."""
+    if path not in ("flux", "fused"):
+        raise ValueError("path must be 'flux' or 'fused'")
+    key = _key(s, path)
     if key in _cache:
         best = _cache[key]
     else:
-        saved = [t.clone() for t in (s.dh, s.dqx, s.dqy)]
+        saved = [s.U.clone(), s.D.clone(), s.h_max.clone(), s.ctrl.clone(), None if s.F is None else s.F.clone()]
+        saved_rows = s.tile_rows
         timings = {}
         for ty in CHOICES[s.dtype]:
             s.set_tile_rows(ty)
-            s.flux()  # warm-up (flux only reads state, writes the accumulators)
-            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            a.record()
-            for _ in range(reps):
-                s.flux()
-            b.record()
-            torch.cuda.synchronize(s.device)
-            timings[ty] = a.elapsed_time(b) / reps
-        for t, v in zip((s.dh, s.dqx, s.dqy), saved):
-            t.copy_(v)
+            timings[ty] = _time(s, path, reps)
+            s.U.copy_(saved[0])
+            s.D.copy_(saved[1])
+            s.h_max.copy_(saved[2])
+            s.ctrl.copy_(saved[3])
+            if saved[4] is not None:
+                s.F.copy_(saved[4])
+        s.set_tile_rows(saved_rows)
         best = min(timings, key=timings.get)
         _cache[key] = best
         s.tune_timings = timings

@@ -373,3 +373,40 @@ def test_fused_step_matches_two_pass_bitwise(cuda, dtype, order):
     ca, cb = a.ctrl_dict(), b.ctrl_dict()
     assert ca["t"] == cb["t"] and ca["dt"] == cb["dt"] and ca["steps"] == cb["steps"] == 31
     assert abs(ca["volume"] - cb["volume"]) <= 1e-12 * ca["volume"]
+
+
+@pytest.mark.parametrize("path", ["flux", "fused"])
+def test_autotune_restores_state_and_tile_height_is_neutral(cuda, path):
+    """The tuner leaves the state as it found it, and every legal tile height
+    gives the same cells bit for bit (each edge is solved by one formula
+    whichever tile holds it)."""
+    from hipims_amd import autotune
+    say(f"[autotune {path}]")
+    c = gen.rainfall_runoff(300, 5.0, rain_mmh=120.0, relief=6.0)
+
+    def mk():
+        g = GridSolver(c.mask, c.h, c.qx, c.qy, c.z, c.dx, cfl=0.5, order=2, dtype=torch.float32, first_dt=0.5,
+                       manning=c.manning, landuse=c.landuse)
+        g.set_station_rain(c.rain_table)
+        g.step(5)
+        return g
+    g = mk()
+    before = [t.clone() for t in (g.h, g.qx, g.qy, g.h_max, g.ctrl)]
+    autotune._cache.clear()
+    best = autotune.tune(g, reps=3, path=path, apply=False)
+    torch.cuda.synchronize()
+    assert best in autotune.CHOICES[torch.float32]
+    say(f"  timings {g.tune_timings} -> {best}")
+    for t0, t1 in zip(before, (g.h, g.qx, g.qy, g.h_max, g.ctrl)):
+        assert torch.equal(t0, t1)
+    ref = None
+    for ty in autotune.CHOICES[torch.float32]:
+        s = mk()
+        s.set_tile_rows(ty)
+        s.step_fused(12) if path == "fused" else s.step(12)
+        torch.cuda.synchronize()
+        cur = [s.h.clone(), s.qx.clone(), s.qy.clone()]
+        if ref is None:
+            ref = cur
+        for a, b in zip(ref, cur):
+            assert torch.equal(a, b), ty
