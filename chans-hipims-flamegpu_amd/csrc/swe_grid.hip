@@ -92,8 +92,25 @@ template <class T> __device__ inline void to_global(T fn, T ft, int d, T& gx, T&
 }
 
 // --- reductions ------------------------------------------------------------
-struct Partial {
-  double min_dt, vol, flux, rain;
+// Per-block results of a pass that ends a step, in the workspace: the CFL
+// minimum of every block in one array (all the finalize pass reads when the
+// mass sums are off), then, when they are on, one FP64 record per block.
+struct Sums {
+  double vol, flux, rain, pad;
+};
+
+__host__ __device__ constexpr int64_t align256(int64_t x) { return (x + 255) & ~int64_t(255); }
+__host__ __device__ constexpr int64_t partials_bytes(int64_t n) { return align256(n * 8) + n * (int64_t)sizeof(Sums); }
+
+struct Partials {
+  double* mins;
+  Sums* sums;
+  __host__ __device__ Partials(void* work, int64_t n)
+      : mins((double*)work), sums((Sums*)((char*)work + align256(n * 8))) {}
+  __device__ void put(int64_t i, double mn, double vol, double flux, double rain, bool with_sums) const {
+    mins[i] = mn;
+    if (with_sums) sums[i] = Sums{vol, flux, rain, 0.0};
+  }
 };
 
 template <class T> __device__ inline T wave_min(T v) {
@@ -216,9 +233,10 @@ __device__ inline void solve_tile_edge(const FluxTile<T, TY>& s, const hp_grid_d
   }
 }
 
-// Fold a block's CFL minimum and FP64 sums into one partial record.
+// Fold a block's CFL minimum and FP64 sums into its partial slot.
 template <class T>
-__device__ inline void block_partial(T mn, double vol, double fsum, double rsum, Partial* part, bool sums) {
+__device__ inline void block_partial(T mn, double vol, double fsum, double rsum, const Partials& part, int64_t idx,
+                                     bool sums) {
   __shared__ double s_red[4][kThreads / 64];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   mn = wave_min(mn);
@@ -235,14 +253,14 @@ __device__ inline void block_partial(T mn, double vol, double fsum, double rsum,
   }
   __syncthreads();
   if (tid == 0) {
-    Partial p{s_red[0][0], s_red[1][0], s_red[2][0], s_red[3][0]};
+    double m = s_red[0][0], v = s_red[1][0], f = s_red[2][0], r = s_red[3][0];
     for (int k = 1; k < kThreads / 64; ++k) {
-      p.min_dt = fmin(p.min_dt, s_red[0][k]);
-      p.vol += s_red[1][k];
-      p.flux += s_red[2][k];
-      p.rain += s_red[3][k];
+      m = fmin(m, s_red[0][k]);
+      v += s_red[1][k];
+      f += s_red[2][k];
+      r += s_red[3][k];
     }
-    *part = p;
+    part.put(idx, m, v, f, r, sums);
   }
 }
 
@@ -301,7 +319,7 @@ __device__ inline void update_state(const CellIn<T>& in, T& h, T& qx, T& qy, T d
 // planes stay untouched for the neighbouring tiles' halos), h_max, and one
 // partial record (CFL minimum, FP64 sums) per tile.
 template <class T, int ORDER, int TY, bool FUSED>
-__device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by, Partial* part) {
+__device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by, const Partials& part, int64_t pidx) {
   __shared__ FluxTile<T, TY> s;
   __shared__ T s_man[FUSED ? 256 : 1];
   __shared__ T s_rate[FUSED ? kRateCache : 1];
@@ -310,7 +328,7 @@ __device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by, Partial*
   constexpr int CPT = TY * kTX / kThreads;  // own cells per thread
   static_assert(TY * kTX % kThreads == 0, "tile cells per thread");
   if (stopped(a.ctrl)) {
-    if (FUSED && threadIdx.x == 0) *part = Partial{INFINITY, 0.0, 0.0, 0.0};
+    if (FUSED && threadIdx.x == 0) part.put(pidx, INFINITY, 0.0, 0.0, 0.0, a.mass_sums != 0);
     return;
   }
   const int tid = threadIdx.x;
@@ -490,7 +508,7 @@ __device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by, Partial*
       if (hm != in.hmax) ((T*)a.h_max)[gi] = hm;
     }
   }
-  if (FUSED) block_partial<T>(mn, vol, fsum, rsum, part, sums);
+  if (FUSED) block_partial<T>(mn, vol, fsum, rsum, part, pidx, sums);
 }
 
 // Workgroups of a launch are handed to the 8 XCDs round-robin (block b runs
@@ -507,7 +525,7 @@ __device__ inline int xcd_band(int b, int n) {
 template <class T, int ORDER, int TY>
 __global__ __launch_bounds__(kThreads) void k_flux(hp_grid_desc a, int ntx) {
   const int tile = xcd_band(blockIdx.x, gridDim.x);
-  flux_tile<T, ORDER, TY, false>(a, tile % ntx, tile / ntx, nullptr);
+  flux_tile<T, ORDER, TY, false>(a, tile % ntx, tile / ntx, Partials(nullptr, 0), 0);
 }
 
 template <class T, int ORDER, int TY>
@@ -516,14 +534,14 @@ __global__ __launch_bounds__(kThreads) void k_flux_multi(const hp_grid_desc* __r
   const int2 m = map[xcd_band(blockIdx.x, gridDim.x)];
   const hp_grid_desc& a = doms[m.x];
   const int ntx = (a.nx + kTX - 1) / kTX;
-  flux_tile<T, ORDER, TY, false>(a, m.y % ntx, m.y / ntx, nullptr);
+  flux_tile<T, ORDER, TY, false>(a, m.y % ntx, m.y / ntx, Partials(nullptr, 0), 0);
 }
 
 // Single-pass step: reads (h, qx, qy), writes (h_out, qx_out, qy_out).
 template <class T, int ORDER, int TY>
 __global__ __launch_bounds__(kThreads) void k_step_fused(hp_grid_desc a, int ntx) {
   const int tile = xcd_band(blockIdx.x, gridDim.x);
-  flux_tile<T, ORDER, TY, true>(a, tile % ntx, tile / ntx, (Partial*)a.work + tile);
+  flux_tile<T, ORDER, TY, true>(a, tile % ntx, tile / ntx, Partials(a.work, gridDim.x), blockIdx.x);
 }
 
 // Infiltration pass over plane elements: a valid cell listed by the flux
@@ -576,12 +594,12 @@ __global__ __launch_bounds__(kThreads) void k_infiltration(hp_grid_desc a) {
 // Processes plane elements [chunk*kChunk, (chunk+1)*kChunk) and writes one
 // partial record.
 template <class T, int MODE>
-__device__ inline void update_chunk(const hp_grid_desc& a, int64_t chunk, Partial* __restrict__ part) {
+__device__ inline void update_chunk(const hp_grid_desc& a, int64_t chunk, const Partials& part, int64_t pidx) {
   __shared__ T s_man[256];
   __shared__ T s_rate[kRateCache];
   const int tid = threadIdx.x;
   if (stopped(a.ctrl)) {
-    if (tid == 0) *part = Partial{INFINITY, 0.0, 0.0, 0.0};
+    if (tid == 0) part.put(pidx, INFINITY, 0.0, 0.0, 0.0, a.mass_sums != 0);
     return;
   }
   const T dt = T(a.ctrl->dt);
@@ -693,38 +711,55 @@ __device__ inline void update_chunk(const hp_grid_desc& a, int64_t chunk, Partia
     }
     if (MODE == HP_UPDATE_STEP || MODE == HP_UPDATE_CFL) stv(hmp + i0, vhm);
   }
-  block_partial<T>(mn, vol, fsum, rsum, part, sums);
+  block_partial<T>(mn, vol, fsum, rsum, part, pidx, sums);
 }
 
 template <class T, int MODE>
 __global__ __launch_bounds__(kThreads) void k_update(hp_grid_desc a) {
-  update_chunk<T, MODE>(a, blockIdx.x, (Partial*)a.work + blockIdx.x);
+  update_chunk<T, MODE>(a, blockIdx.x, Partials(a.work, gridDim.x), blockIdx.x);
 }
 
 template <class T, int MODE>
 __global__ __launch_bounds__(kThreads) void k_update_multi(const hp_grid_desc* __restrict__ doms,
-                                                           const int2* __restrict__ map, Partial* parts) {
+                                                           const int2* __restrict__ map, void* parts) {
   const int2 m = map[blockIdx.x];
-  update_chunk<T, MODE>(doms[m.x], m.y, parts + blockIdx.x);
+  update_chunk<T, MODE>(doms[m.x], m.y, Partials(parts, gridDim.x), blockIdx.x);
 }
 
-// Fold a domain's partials and advance its time control.
-__device__ inline void finalize_domain(hp_ctrl* ctrl, const Partial* parts, int n, double dx, bool sums) {
+// Fold a domain's partials in a fixed order and advance its time control.
+// The minima are read eight per thread before any is used, so one block
+// covers many thousands of partials in a few memory round trips.
+__device__ inline void finalize_domain(hp_ctrl* ctrl, const double* __restrict__ mins, const Sums* __restrict__ part,
+                                       int n, double dx, bool sums) {
+  constexpr int kU = 8;
   __shared__ double s_red[4][16];
   const int tid = threadIdx.x;
   if (stopped(ctrl)) return;
   double mn = INFINITY, vol = 0.0, fs = 0.0, rs = 0.0;
-  for (int k = tid; k < n; k += blockDim.x) {
-    const Partial p = parts[k];
-    mn = fmin(mn, p.min_dt);
-    vol += p.vol;
-    fs += p.flux;
-    rs += p.rain;
+  for (int base = tid; base < n; base += blockDim.x * kU) {
+    double v[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int k = base + u * (int)blockDim.x;
+      v[u] = k < n ? mins[k] : INFINITY;
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) mn = fmin(mn, v[u]);
+  }
+  if (sums) {
+    for (int k = tid; k < n; k += blockDim.x) {
+      const Sums p = part[k];
+      vol += p.vol;
+      fs += p.flux;
+      rs += p.rain;
+    }
   }
   mn = wave_min(mn);
-  vol = wave_sum(vol);
-  fs = wave_sum(fs);
-  rs = wave_sum(rs);
+  if (sums) {
+    vol = wave_sum(vol);
+    fs = wave_sum(fs);
+    rs = wave_sum(rs);
+  }
   const int lane = tid & 63, w = tid >> 6;
   const int nw = (blockDim.x + 63) >> 6;
   if (lane == 0) {
@@ -768,16 +803,18 @@ __device__ inline void finalize_domain(hp_ctrl* ctrl, const Partial* parts, int 
   }
 }
 
-__global__ __launch_bounds__(1024) void k_finalize(hp_ctrl* ctrl, const Partial* parts, int n, double dx,
-                                                   int sums) {
-  finalize_domain(ctrl, parts, n, dx, sums != 0);
+__global__ __launch_bounds__(1024) void k_finalize(hp_ctrl* ctrl, void* work, int n, double dx, int sums) {
+  const Partials P(work, n);
+  finalize_domain(ctrl, P.mins, P.sums, n, dx, sums != 0);
 }
 
 __global__ __launch_bounds__(256) void k_finalize_multi(const hp_grid_desc* __restrict__ doms,
-                                                        const int2* __restrict__ ranges, const Partial* parts) {
+                                                        const int2* __restrict__ ranges, void* parts,
+                                                        int n_total) {
   const int2 r = ranges[blockIdx.x];
   const hp_grid_desc& a = doms[blockIdx.x];
-  finalize_domain(a.ctrl, parts + r.x, r.y, a.dx, a.mass_sums != 0);
+  const Partials P(parts, n_total);
+  finalize_domain(a.ctrl, P.mins + r.x, P.sums + r.x, r.y, a.dx, a.mass_sums != 0);
 }
 
 // --- host side -------------------------------------------------------------
@@ -897,7 +934,7 @@ static void launch_update_mode(const hp_grid_desc* g, int64_t nb, hipStream_t st
 
 static int launch_update(const hp_grid_desc* g, int mode, hipStream_t st) {
   const int64_t nb = update_blocks(g);
-  if (!g->work || g->work_bytes < nb * (int64_t)sizeof(Partial))
+  if (!g->work || g->work_bytes < partials_bytes(nb))
     return fail(HP_ESPACE, "workspace smaller than hp_grid_workspace_bytes()");
   switch (mode) {
     case HP_UPDATE_STEP: launch_update_mode<HP_UPDATE_STEP>(g, nb, st); break;
@@ -909,7 +946,7 @@ static int launch_update(const hp_grid_desc* g, int mode, hipStream_t st) {
   int rc = check_launch("k_update");
   if (rc) return rc;
   if (mode == HP_UPDATE_STEP || mode == HP_UPDATE_CFL) {
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, st, g->ctrl, (const Partial*)g->work, (int)nb, g->dx,
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, st, g->ctrl, g->work, (int)nb, g->dx,
                        g->mass_sums);
     return check_launch("k_finalize");
   }
@@ -922,7 +959,6 @@ struct EnsembleLayout {
   int64_t n_ftiles, n_ublocks;
 };
 
-static int64_t align256(int64_t x) { return (x + 255) & ~int64_t(255); }
 
 static EnsembleLayout ensemble_layout(int32_t n, const hp_grid_desc* d) {
   EnsembleLayout L{};
@@ -935,7 +971,7 @@ static EnsembleLayout ensemble_layout(int32_t n, const hp_grid_desc* d) {
   L.off_umap = L.off_fmap + align256(L.n_ftiles * 8);
   L.off_fin = L.off_umap + align256(L.n_ublocks * 8);
   L.off_parts = L.off_fin + align256((int64_t)n * 8);
-  L.total = L.off_parts + align256(L.n_ublocks * (int64_t)sizeof(Partial));
+  L.total = L.off_parts + align256(partials_bytes(L.n_ublocks));
   return L;
 }
 
@@ -948,7 +984,7 @@ static void launch_flux_multi(const hp_ensemble* e, hipStream_t st) {
 template <class T>
 static void launch_update_multi(const hp_ensemble* e, hipStream_t st) {
   hipLaunchKernelGGL((k_update_multi<T, HP_UPDATE_STEP>), dim3((unsigned)e->n_update_blocks), dim3(kThreads), 0, st,
-                     (const hp_grid_desc*)e->doms, (const int2*)e->update_map, (Partial*)e->partials);
+                     (const hp_grid_desc*)e->doms, (const int2*)e->update_map, e->partials);
 }
 
 static int ensemble_step_once(const hp_ensemble* e, hipStream_t st) {
@@ -970,7 +1006,8 @@ static int ensemble_step_once(const hp_ensemble* e, hipStream_t st) {
   rc = check_launch("k_update_multi");
   if (rc) return rc;
   hipLaunchKernelGGL(k_finalize_multi, dim3((unsigned)e->n_domains), dim3(256), 0, st,
-                     (const hp_grid_desc*)e->doms, (const int2*)e->finalize_map, (const Partial*)e->partials);
+                     (const hp_grid_desc*)e->doms, (const int2*)e->finalize_map, e->partials,
+                     (int)e->n_update_blocks);
   return check_launch("k_finalize_multi");
 }
 
@@ -1001,7 +1038,7 @@ extern "C" {
 int64_t hp_grid_workspace_bytes(const hp_grid_desc* g) {
   if (!g || g->pitch <= 0 || g->ny <= 0 || g->nx <= 0) return 0;
   const int64_t n = update_blocks(g) > flux_tiles(g) ? update_blocks(g) : flux_tiles(g);
-  return n * (int64_t)sizeof(Partial);
+  return partials_bytes(n);
 }
 
 int hp_grid_step_fused(const hp_grid_desc* g, int32_t nsteps, void* stream) {
@@ -1010,14 +1047,14 @@ int hp_grid_step_fused(const hp_grid_desc* g, int32_t nsteps, void* stream) {
   if (nsteps < 0) return fail(HP_EINVAL, "nsteps must be >= 0");
   if (!g->h_out || !g->qx_out || !g->qy_out) return fail(HP_EINVAL, "the fused step needs h_out, qx_out, qy_out");
   const int64_t nt = flux_tiles(g);
-  if (!g->work || g->work_bytes < nt * (int64_t)sizeof(Partial))
+  if (!g->work || g->work_bytes < partials_bytes(nt))
     return fail(HP_ESPACE, "workspace smaller than hp_grid_workspace_bytes()");
   hipStream_t st = (hipStream_t)stream;
   hp_grid_desc d = *g;
   for (int32_t k = 0; k < nsteps; ++k) {
     rc = launch_step_fused(&d, st);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, st, d.ctrl, (const Partial*)d.work, (int)nt, d.dx,
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, st, d.ctrl, d.work, (int)nt, d.dx,
                        d.mass_sums);
     rc = check_launch("k_finalize");
     if (rc) return rc;
