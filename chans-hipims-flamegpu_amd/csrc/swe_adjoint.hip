@@ -31,10 +31,26 @@
 namespace hp {
 
 constexpr int kAThreads = 256;
-constexpr int kAChunk = 4096;  // plane elements per block of the pointwise passes
+constexpr int kAChunk = 1024;  // plane elements per block of the pointwise passes
 constexpr int kGradStride = 16;  // doubles per block: classes 0..14, rain at 15
+constexpr int kARateCache = 64;  // station rates kept in LDS per block
 
-int grid_flux_launch(const hp_grid_desc* g, hipStream_t st);  // swe_grid.hip
+int grid_flux_launch(const hp_grid_desc* g, hipStream_t st);                   // swe_grid.hip
+int grid_update_launch(const hp_grid_desc* g, bool finalize, hipStream_t st);  // swe_grid.hip
+
+// (t, dt) of step k between the control block and a schedule [K+1][2]:
+// save 1 records the control block's values, save 0 imposes the schedule's.
+__global__ void k_schedule(hp_ctrl* ctrl, double* sched, int k, int save) {
+  if (threadIdx.x == 0) {
+    if (save) {
+      sched[2 * (int64_t)k] = ctrl->t;
+      sched[2 * (int64_t)k + 1] = ctrl->dt;
+    } else {
+      ctrl->t = sched[2 * (int64_t)k];
+      ctrl->dt = sched[2 * (int64_t)k + 1];
+    }
+  }
+}
 
 template <class T> struct AdjPlanes {
   T *lh, *lqx, *lqy;
@@ -72,6 +88,7 @@ template <class T>
 __global__ __launch_bounds__(kAThreads) void k_adj_update(hp_grid_desc a, AdjPlanes<T> p, double* grads,
                                                           double* gn) {
   __shared__ T s_man[256];
+  __shared__ T s_rb[kARateCache], s_rs[kARateCache];  // station rates: unscaled, scaled
   __shared__ double s_r[kAThreads / 64];
   const int tid = threadIdx.x;
   const double t_now = a.ctrl->t;
@@ -81,7 +98,12 @@ __global__ __launch_bounds__(kAThreads) void k_adj_update(hp_grid_desc a, AdjPla
   const int mode = a.rain_mode;
   for (int k = tid; k < a.n_classes; k += kAThreads) s_man[k] = ((const T*)a.manning)[k];
   const RainRows rr = rain_rows(a, t_now);
+  if (mode == HP_RAIN_STATION && tid < kARateCache) {
+    s_rb[tid] = station_rate<T>(rr, tid, 1.0);
+    s_rs[tid] = station_rate<T>(rr, tid, a.rain_scale);
+  }
   __syncthreads();
+  const unsigned pitch32 = (unsigned)a.pitch;  // the plane has fewer than 2^31 elements
   const T* __restrict__ h = (const T*)a.h;
   const T* __restrict__ qx = (const T*)a.qx;
   const T* __restrict__ qy = (const T*)a.qy;
@@ -106,8 +128,10 @@ __global__ __launch_bounds__(kAThreads) void k_adj_update(hp_grid_desc a, AdjPla
       if (a.station) vs = ldv<VecI16>(a.station + i0);
 #pragma unroll
       for (int j = 0; j < kVec; ++j) {
-        rbv.x[j] = station_rate<T>(rr, vs.x[j], 1.0);
-        rsc.x[j] = station_rate<T>(rr, vs.x[j], a.rain_scale);
+        const int st = vs.x[j];
+        const bool cached = st >= 0 && st < kARateCache;
+        rbv.x[j] = cached ? s_rb[st] : station_rate<T>(rr, st, 1.0);
+        rsc.x[j] = cached ? s_rs[st] : station_rate<T>(rr, st, a.rain_scale);
       }
     } else if (mode == HP_RAIN_FIELD) {
       const VecF32 vf = ldv<VecF32>(a.rain_field + i0);
@@ -141,8 +165,8 @@ __global__ __launch_bounds__(kAThreads) void k_adj_update(hp_grid_desc a, AdjPla
       lh.x[j] = a_h1;
       lqx.x[j] = a_qsx;
       lqy.x[j] = a_qsy;
-      const int64_t i = i0 + j;
-      const int y = (int)(i / a.pitch), x = (int)(i - (int64_t)y * a.pitch);
+      // the kVec elements share a row (pitch is a multiple of kVec)
+      const int y = (int)((unsigned)i0 / pitch32), x = (int)((unsigned)i0 - (unsigned)y * pitch32) + j;
       if (flux_listed<T>(a, h, x, y, t_T)) {
         ch.x[j] = a_h1;
         cqx.x[j] = a_qsx;
@@ -373,6 +397,20 @@ __global__ __launch_bounds__(kAThreads) void k_adj_gather(hp_grid_desc a, AdjPla
   p.lqy[i] += sy;
 }
 
+// Start of one reverse step of a calibration sweep: the loss cotangent of
+// the depths after step k and the step's frozen t and dt.
+template <class T>
+__global__ void k_reverse_prologue(hp_ctrl* ctrl, const double* __restrict__ sched, int k, T* lam_h,
+                                   const int64_t* __restrict__ off, const double* __restrict__ h_rec,
+                                   const double* __restrict__ obs, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) {
+    ctrl->t = sched[2 * (int64_t)k];
+    ctrl->dt = sched[2 * (int64_t)k + 1];
+  }
+  if (i < n) lam_h[off[i]] += T(2.0 * (h_rec[i] - obs[i]));
+}
+
 template <class T>
 __global__ void k_adj_loss(T* lam_h, const int64_t* __restrict__ off, const double* __restrict__ h_rec,
                            const double* __restrict__ obs, int n, double weight) {
@@ -434,6 +472,7 @@ int hp_adjoint_step(const hp_grid_desc* g, const hp_adjoint_desc* ad, void* stre
   if (g->infil_ks) return fail(HP_EINVAL, "the adjoint does not cover infiltration");
   if (ad->slope_mode != 0 && ad->slope_mode != 1) return fail(HP_EINVAL, "slope_mode must be 0 or 1");
   if (g->ny + 1 > 65535) return fail(HP_EINVAL, "the adjoint launches one block row per grid row: ny < 65535");
+  if (g->pitch * (int64_t)g->ny >= (int64_t(1) << 31)) return fail(HP_EINVAL, "the adjoint covers planes of fewer than 2^31 elements");
   if (!ad->flux_ready) {
     int rc = grid_flux_launch(g, (hipStream_t)stream);  // recompute the step's flux on U_k
     if (rc) return rc;
@@ -463,6 +502,95 @@ int hp_adjoint_add_loss(int dtype, void* lam_h, const int64_t* offsets, const do
     hipLaunchKernelGGL(k_adj_loss<double>, dim3(nb), dim3(256), 0, (hipStream_t)stream, (double*)lam_h, offsets,
                        h_rec, obs, n, weight);
   return check_launch("k_adj_loss");
+}
+
+// Three state (or flux) planes of a saved step, one after another.
+static hp_grid_desc with_segment(hp_grid_desc d, void* u, void* f, int64_t plane_bytes) {
+  if (u) {
+    d.h = u;
+    d.qx = (char*)u + plane_bytes;
+    d.qy = (char*)u + 2 * plane_bytes;
+  }
+  if (f) {
+    d.dh = f;
+    d.dqx = (char*)f + plane_bytes;
+    d.dqy = (char*)f + 2 * plane_bytes;
+  }
+  return d;
+}
+
+int hp_calib_forward(const hp_grid_desc* g, double* sched, int32_t record, int32_t k0, int32_t nsteps, void* seg_u,
+                     void* seg_d, int64_t seg_stride, int32_t last_flux_only, const int64_t* gauge_off,
+                     int32_t n_gauges, void* hist, void* stream) {
+  if (!g || !sched) return fail(HP_EINVAL, "null descriptor or schedule");
+  if (nsteps < 0 || k0 < 0) return fail(HP_EINVAL, "k0 and nsteps must be >= 0");
+  if (g->infil_ks) return fail(HP_EINVAL, "calibration runs do not cover infiltration");
+  if ((seg_u == nullptr) != (seg_d == nullptr)) return fail(HP_EINVAL, "seg_u and seg_d go together");
+  const int64_t plane_bytes = g->pitch * (int64_t)g->ny * (g->dtype == HP_F32 ? 4 : 8);
+  if (seg_u && seg_stride < 3 * plane_bytes) return fail(HP_EINVAL, "seg_stride smaller than three planes");
+  if (hist && n_gauges > 0 && !gauge_off) return fail(HP_EINVAL, "hist needs gauge offsets");
+  hipStream_t st = (hipStream_t)stream;
+  for (int32_t j = 0; j < nsteps; ++j) {
+    const int32_t k = k0 + j;
+    hipLaunchKernelGGL(k_schedule, dim3(1), dim3(64), 0, st, g->ctrl, sched, k, record ? 1 : 0);
+    int rc = check_launch("k_schedule");
+    if (rc) return rc;
+    hp_grid_desc d = *g;
+    if (seg_u) {
+      char* su = (char*)seg_u + (int64_t)j * seg_stride;
+      const void* src[3] = {g->h, g->qx, g->qy};
+      if ((char*)g->qx == (char*)g->h + plane_bytes && (char*)g->qy == (char*)g->qx + plane_bytes) {
+        if (hipMemcpyAsync(su, g->h, 3 * plane_bytes, hipMemcpyDeviceToDevice, st) != hipSuccess)
+          return fail(HP_ELAUNCH, "state copy failed");
+      } else {
+        for (int p = 0; p < 3; ++p)
+          if (hipMemcpyAsync(su + p * plane_bytes, src[p], plane_bytes, hipMemcpyDeviceToDevice, st) != hipSuccess)
+            return fail(HP_ELAUNCH, "state copy failed");
+      }
+      d = with_segment(d, nullptr, (char*)seg_d + (int64_t)j * seg_stride, plane_bytes);
+    }
+    rc = grid_flux_launch(&d, st);
+    if (rc) return rc;
+    if (last_flux_only && j == nsteps - 1) break;
+    rc = grid_update_launch(&d, record != 0, st);
+    if (rc) return rc;
+    if (hist && n_gauges > 0) {
+      char* row = (char*)hist + (int64_t)(k + 1) * n_gauges * (g->dtype == HP_F32 ? 4 : 8);
+      rc = hp_gather(g->dtype, g->h, gauge_off, n_gauges, row, stream);
+      if (rc) return rc;
+    }
+  }
+  return HP_OK;
+}
+
+int hp_calib_reverse(const hp_grid_desc* g, const hp_adjoint_desc* ad, double* sched, int32_t k_lo, int32_t k_hi,
+                     int32_t seg_k0, void* seg_u, void* seg_d, int64_t seg_stride, const int64_t* gauge_off,
+                     int32_t n_gauges, const double* h_rec, const double* obs, void* stream) {
+  if (!g || !ad || !sched || !seg_u || !seg_d) return fail(HP_EINVAL, "null descriptor, schedule or segment");
+  if (!ad->flux_ready) return fail(HP_EINVAL, "the reverse sweep runs on saved fluxes: flux_ready must be 1");
+  if (k_lo < seg_k0 || k_hi < k_lo) return fail(HP_EINVAL, "need seg_k0 <= k_lo <= k_hi");
+  const int64_t plane_bytes = g->pitch * (int64_t)g->ny * (g->dtype == HP_F32 ? 4 : 8);
+  if (seg_stride < 3 * plane_bytes) return fail(HP_EINVAL, "seg_stride smaller than three planes");
+  if (n_gauges > 0 && (!gauge_off || !h_rec || !obs)) return fail(HP_EINVAL, "the loss needs gauges, h_rec and obs");
+  hipStream_t st = (hipStream_t)stream;
+  const unsigned nb = (unsigned)((n_gauges > 0 ? n_gauges + 255 : 256) / 256);
+  for (int32_t k = k_hi - 1; k >= k_lo; --k) {
+    const double* hr = h_rec ? h_rec + (int64_t)(k + 1) * n_gauges : nullptr;
+    const double* ob = obs ? obs + (int64_t)(k + 1) * n_gauges : nullptr;
+    if (g->dtype == HP_F32)
+      hipLaunchKernelGGL(k_reverse_prologue<float>, dim3(nb), dim3(256), 0, st, g->ctrl, sched, k,
+                         (float*)ad->lam_h, gauge_off, hr, ob, n_gauges);
+    else
+      hipLaunchKernelGGL(k_reverse_prologue<double>, dim3(nb), dim3(256), 0, st, g->ctrl, sched, k,
+                         (double*)ad->lam_h, gauge_off, hr, ob, n_gauges);
+    int rc = check_launch("k_reverse_prologue");
+    if (rc) return rc;
+    const int64_t off = (int64_t)(k - seg_k0) * seg_stride;
+    const hp_grid_desc d = with_segment(*g, (char*)seg_u + off, (char*)seg_d + off, plane_bytes);
+    rc = hp_adjoint_step(&d, ad, stream);
+    if (rc) return rc;
+  }
+  return HP_OK;
 }
 
 }  // extern "C"

@@ -22,7 +22,7 @@ HP_OK, HP_EINVAL, HP_ELAUNCH, HP_ESPACE = 0, -1, -2, -3
 HP_F32, HP_F64 = 0, 1
 HP_RAIN_NONE, HP_RAIN_STATION, HP_RAIN_FIELD = 0, 1, 2
 HP_UPDATE_STEP, HP_UPDATE_FRICTION, HP_UPDATE_EULER, HP_UPDATE_CFL = 0, 1, 2, 3
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 _P = ctypes.c_void_p
 _I64 = ctypes.c_int64
@@ -104,6 +104,10 @@ SIGNATURES = {
     "hp_adjoint_step": (_I, [ctypes.POINTER(HpGridDesc), ctypes.POINTER(HpAdjointDesc), _P]),
     "hp_adjoint_finish": (_I, [ctypes.POINTER(HpGridDesc), ctypes.POINTER(HpAdjointDesc), _P]),
     "hp_adjoint_add_loss": (_I, [_I, _P, _P, _P, _P, _I32, ctypes.c_double, _P]),
+    "hp_calib_forward": (_I, [ctypes.POINTER(HpGridDesc), _P, _I32, _I32, _I32, _P, _P, _I64, _I32, _P, _I32, _P,
+                              _P]),
+    "hp_calib_reverse": (_I, [ctypes.POINTER(HpGridDesc), ctypes.POINTER(HpAdjointDesc), _P, _I32, _I32, _I32, _P,
+                              _P, _I64, _P, _I32, _P, _P, _P]),
     "hp_gather": (_I, [_I, _P, _P, _I64, _P, _P]),
     "hp_scatter": (_I, [_I, _P, _P, _I64, _P, _P]),
 }

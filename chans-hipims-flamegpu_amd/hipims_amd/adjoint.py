@@ -11,7 +11,9 @@ backward(): the state before every `checkpoint`-th step is kept from the
 forward run; segments are recomputed last to first, storing the state before
 each step and the step's flux, and the hand-written adjoint kernels
 (hp_adjoint_step, with the saved flux) run the steps in reverse directly on
-the stored planes.  checkpoint=1 keeps every state and flux from the forward
+the stored planes.  The per-step sequences of the forward run, of a
+segment's recomputation and of its reverse sweep are native loops
+(hp_calib_forward, hp_calib_reverse): one call per segment.  checkpoint=1 keeps every state and flux from the forward
 run and recomputes nothing (K x 6 planes of device memory).  Parameter
 gradients are accumulated in FP64 (per cell for Manning, per block for the
 rain multiplier) and folded at the end.  ``CalibrationLoss.apply(theta,
@@ -114,49 +116,47 @@ class Calibration:
         check(lib().hp_gather(_lib.dtype_code(self.s.dtype), self.s.h.data_ptr(), self.gauge_off.data_ptr(),
                               self.G, self.hist[k].data_ptr(), _lib.stream_ptr(self.s.device)), "hp_gather")
 
-    def _step(self, k):
-        """Step k with the frozen (t_k, dt_k) once the schedule is known."""
+    def _seg_stride(self) -> int:
+        return self.segU[0].numel() * self.segU.element_size()
+
+    def _forward(self, sched, record, k0, n, seg=False, last_flux_only=False, gauges=True):
+        """Steps k0 .. k0+n-1 in one native call (hp_calib_forward)."""
         s = self.s
-        if self.sched_ready:
-            s.ctrl[0:2].copy_(self.sched[k])
-        else:
-            self.sched[k].copy_(s.ctrl[0:2])
-        s.step(1)
+        check(lib().hp_calib_forward(
+            ctypes.byref(s.desc), sched.data_ptr(), int(record), int(k0), int(n),
+            self.segU.data_ptr() if seg else None, self.segD.data_ptr() if seg else None,
+            self._seg_stride() if seg else 0, int(last_flux_only),
+            self.gauge_off.data_ptr() if gauges else None, self.G if gauges else 0,
+            self.hist.data_ptr() if gauges else None, _lib.stream_ptr(s.device)), "hp_calib_forward")
 
     def run(self, theta, keep_checkpoints=True) -> torch.Tensor:
-        """Forward run; returns the FP64 loss (needs obs)."""
+        """Forward run; returns the FP64 loss (needs obs).  The first run
+        records the model's own time steps; later runs follow them."""
         self.set_theta(theta)
         self._restore_initial()
         self.ckpts.clear()
         self._record_gauges(0)
-        for k in range(self.K):
-            if keep_checkpoints and k % self.ck == 0:
-                if self.ck == 1:
-                    self.segU[k].copy_(self.s.U)
-                    self.ckpts[k] = self.segU[k]
-                else:
-                    self.ckpts[k] = self.s.U.clone()
-            self._step(k)
-            if keep_checkpoints and self.ck == 1:
-                self.segD[k].copy_(self.s.D)
-            self._record_gauges(k + 1)
+        record = not self.sched_ready
+        if keep_checkpoints and self.ck == 1:
+            # every state and flux kept: nothing is recomputed in reverse
+            self._forward(self.sched, record, 0, self.K, seg=True)
+            self.ckpts = {0: self.segU[0]}
+        else:
+            for a in range(0, self.K, self.ck):
+                if keep_checkpoints:
+                    self.ckpts[a] = self.s.U.clone()
+                self._forward(self.sched, record, a, min(self.ck, self.K - a))
         self.sched_ready = True
         return self.loss()
 
     def simulate(self, theta) -> torch.Tensor:
         """Forward run with the model's own CFL time steps (nothing recorded or
         frozen); returns the gauge depths [steps + 1, G] in FP64."""
-        saved = self.sched_ready
-        self.sched_ready = False
-        sched = self.sched.clone()
         self.set_theta(theta)
         self._restore_initial()
         self._record_gauges(0)
-        for k in range(self.K):
-            self.s.step(1)
-            self._record_gauges(k + 1)
-        self.sched.copy_(sched)
-        self.sched_ready = saved
+        scratch = torch.zeros_like(self.sched)
+        self._forward(scratch, True, 0, self.K)
         return self.hist.double().clone()
 
     def loss(self) -> torch.Tensor:
@@ -171,45 +171,34 @@ class Calibration:
         s = self.s
         if not self.ckpts:
             raise RuntimeError("run() with checkpoints first")
+        if self.obs is None:
+            raise RuntimeError("no observations set")
         for t in self.lam:
             t.zero_()
         self.grads.zero_()
         self.gn.zero_()
         h_rec = self.hist.double().contiguous()
+        obs = self.obs.contiguous()
         stream = _lib.stream_ptr(s.device)
-        dcode = _lib.dtype_code(s.dtype)
-        base = s.desc
-        d = HpGridDesc.from_buffer_copy(base)
-        plane = s.h.numel() * s.h.element_size()
-        starts = sorted(self.ckpts)
-        for si in range(len(starts) - 1, -1, -1):
-            a = starts[si]
-            b = min(a + self.ck, self.K)
-            if self.ck == 1:
-                states = {a: (self.segU[a], self.segD[a])}
-            else:
-                # recompute the segment: state before each step, then its flux
+        stride = self._seg_stride()
+
+        def reverse(k_lo, k_hi, seg_k0):
+            check(lib().hp_calib_reverse(
+                ctypes.byref(s.desc), ctypes.byref(self.adesc), self.sched.data_ptr(), int(k_lo), int(k_hi),
+                int(seg_k0), self.segU.data_ptr(), self.segD.data_ptr(), stride, self.gauge_off.data_ptr(), self.G,
+                h_rec.data_ptr(), obs.data_ptr(), stream), "hp_calib_reverse")
+
+        if self.ck == 1:
+            reverse(0, self.K, 0)
+        else:
+            for a in sorted(self.ckpts, reverse=True):
+                b = min(a + self.ck, self.K)
+                # recompute the segment on the frozen schedule: the state
+                # before each step and its flux (the last step's flux only)
                 s.U.copy_(self.ckpts[a])
-                states = {}
-                for k in range(a, b):
-                    j = k - a
-                    self.segU[j].copy_(s.U)
-                    s.ctrl[0:2].copy_(self.sched[k])
-                    s.flux()
-                    self.segD[j].copy_(s.D)
-                    if k < b - 1:
-                        s.update()
-                    states[k] = (self.segU[j], self.segD[j])
-            for k in range(b - 1, a - 1, -1):
-                check(lib().hp_adjoint_add_loss(dcode, self.lam[0].data_ptr(), self.gauge_off.data_ptr(),
-                                                h_rec[k + 1].data_ptr(), self.obs[k + 1].data_ptr(), self.G, 1.0,
-                                                stream), "hp_adjoint_add_loss")
-                U, D = states[k]
-                d.h, d.qx, d.qy = U.data_ptr(), U.data_ptr() + plane, U.data_ptr() + 2 * plane
-                d.dh, d.dqx, d.dqy = D.data_ptr(), D.data_ptr() + plane, D.data_ptr() + 2 * plane
-                s.ctrl[0:2].copy_(self.sched[k])
-                check(lib().hp_adjoint_step(ctypes.byref(d), ctypes.byref(self.adesc), stream), "hp_adjoint_step")
-        check(lib().hp_adjoint_finish(ctypes.byref(base), ctypes.byref(self.adesc), stream), "hp_adjoint_finish")
+                self._forward(self.sched, False, a, b - a, seg=True, last_flux_only=True, gauges=False)
+                reverse(a, b, a)
+        check(lib().hp_adjoint_finish(ctypes.byref(s.desc), ctypes.byref(self.adesc), stream), "hp_adjoint_finish")
         g = self.grads.sum(dim=0).cpu()
         return torch.cat([g[: self.C], g[_lib.GRAD_STRIDE - 1: _lib.GRAD_STRIDE]])
 
