@@ -328,12 +328,18 @@ def adjoint_mode(a):
     torch.cuda.set_device(dev)
     c = gen.rainfall_runoff(a.adj_size, 5.0)
 
+    rows = 0
+
     def solver():
         s = GridSolver(c.mask, c.h, c.qx, c.qy, c.z, c.dx, cfl=c.cfl, order=a.adj_order, dtype=torch.float32,
-                       device=dev, first_dt=c.first_dt, manning=c.manning, landuse=c.landuse)
+                       device=dev, first_dt=c.first_dt, manning=c.manning, landuse=c.landuse, tile_rows=rows)
         s.set_station_rain(c.rain_table)
         return s
 
+    # one tile height for the plain and the calibration runs: the fastest
+    # single-pass one on this domain
+    from hipims_amd import autotune
+    rows = autotune.tune(solver(), path="fused", apply=False)
     pilot = solver()
     pilot.step(a.adj_steps)
     h = pilot.raster(pilot.h)
@@ -342,14 +348,16 @@ def adjoint_mode(a):
     gauges = wet[rng.choice(len(wet), size=min(a.adj_gauges, len(wet)), replace=False)]
     del pilot
 
-    # plain forward: the model's own CFL steps, one call
+    # plain forward: the model's own CFL steps, one call of the library's
+    # fastest step (the single-pass kernel, as the calibration runs use)
     plain = solver()
-    plain.step(5)
+    plain.step_fused(6)
     torch.cuda.synchronize()
     plain = solver()
+    plain.ensure_out()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    plain.step(a.adj_steps)
+    plain.step_fused(a.adj_steps)
     torch.cuda.synchronize()
     fwd_s = time.perf_counter() - t0
 
@@ -386,7 +394,8 @@ def adjoint_mode(a):
                  "grad_equal": bool(torch.equal(grad_all, grad))}
     del cal_all
     torch.cuda.empty_cache()
-    line = {"mode": "adjoint", "size": a.adj_size, "order": a.adj_order, "slopes": a.adj_slopes,
+    line = {"mode": "adjoint", "size": a.adj_size, "order": a.adj_order, "slopes": a.adj_slopes, "tile_rows": rows,
+            "plain_path": "single-pass step (k_step_fused + k_finalize)",
             "steps": a.adj_steps, "checkpoint": a.adj_checkpoint,
             "gauges": int(len(gauges)), "dtype": "f32 state, f64 gradient sums", "loss": float(loss),
             "grad": [float(v) for v in grad], "plain_forward_s": fwd_s, "recorded_forward_s": run_s,

@@ -37,18 +37,13 @@ constexpr int kARateCache = 64;  // station rates kept in LDS per block
 
 int grid_flux_launch(const hp_grid_desc* g, hipStream_t st);                   // swe_grid.hip
 int grid_update_launch(const hp_grid_desc* g, bool finalize, hipStream_t st);  // swe_grid.hip
+int grid_fused_launch(const hp_grid_desc* g, bool save_flux, bool finalize, hipStream_t st);  // swe_grid.hip
 
-// (t, dt) of step k between the control block and a schedule [K+1][2]:
-// save 1 records the control block's values, save 0 imposes the schedule's.
-__global__ void k_schedule(hp_ctrl* ctrl, double* sched, int k, int save) {
+// Records (t, dt) of the control block as step k of a schedule [K+1][2].
+__global__ void k_schedule_save(const hp_ctrl* ctrl, double* sched, int k) {
   if (threadIdx.x == 0) {
-    if (save) {
-      sched[2 * (int64_t)k] = ctrl->t;
-      sched[2 * (int64_t)k + 1] = ctrl->dt;
-    } else {
-      ctrl->t = sched[2 * (int64_t)k];
-      ctrl->dt = sched[2 * (int64_t)k + 1];
-    }
+    sched[2 * (int64_t)k] = ctrl->t;
+    sched[2 * (int64_t)k + 1] = ctrl->dt;
   }
 }
 
@@ -397,20 +392,6 @@ __global__ __launch_bounds__(kAThreads) void k_adj_gather(hp_grid_desc a, AdjPla
   p.lqy[i] += sy;
 }
 
-// Start of one reverse step of a calibration sweep: the loss cotangent of
-// the depths after step k and the step's frozen t and dt.
-template <class T>
-__global__ void k_reverse_prologue(hp_ctrl* ctrl, const double* __restrict__ sched, int k, T* lam_h,
-                                   const int64_t* __restrict__ off, const double* __restrict__ h_rec,
-                                   const double* __restrict__ obs, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) {
-    ctrl->t = sched[2 * (int64_t)k];
-    ctrl->dt = sched[2 * (int64_t)k + 1];
-  }
-  if (i < n) lam_h[off[i]] += T(2.0 * (h_rec[i] - obs[i]));
-}
-
 template <class T>
 __global__ void k_adj_loss(T* lam_h, const int64_t* __restrict__ off, const double* __restrict__ h_rec,
                            const double* __restrict__ obs, int n, double weight) {
@@ -519,74 +500,97 @@ static hp_grid_desc with_segment(hp_grid_desc d, void* u, void* f, int64_t plane
   return d;
 }
 
-int hp_calib_forward(const hp_grid_desc* g, double* sched, int32_t record, int32_t k0, int32_t nsteps, void* seg_u,
-                     void* seg_d, int64_t seg_stride, int32_t last_flux_only, const int64_t* gauge_off,
-                     int32_t n_gauges, void* hist, void* stream) {
-  if (!g || !sched) return fail(HP_EINVAL, "null descriptor or schedule");
+int hp_calib_forward(const hp_grid_desc* g, double* sched, const hp_ctrl* frozen, int32_t k0, int32_t nsteps,
+                     void* seg_u, void* seg_d, int64_t seg_stride, const int64_t* gauge_off, int32_t n_gauges,
+                     void* hist, void* stream) {
+  if (!g) return fail(HP_EINVAL, "null descriptor");
   if (nsteps < 0 || k0 < 0) return fail(HP_EINVAL, "k0 and nsteps must be >= 0");
   if (g->infil_ks) return fail(HP_EINVAL, "calibration runs do not cover infiltration");
   if ((seg_u == nullptr) != (seg_d == nullptr)) return fail(HP_EINVAL, "seg_u and seg_d go together");
   const int64_t plane_bytes = g->pitch * (int64_t)g->ny * (g->dtype == HP_F32 ? 4 : 8);
   if (seg_u && seg_stride < 3 * plane_bytes) return fail(HP_EINVAL, "seg_stride smaller than three planes");
+  if (!seg_u && (!g->h_out || !g->qx_out || !g->qy_out))
+    return fail(HP_EINVAL, "without saved segments the steps alternate with h_out, qx_out, qy_out");
   if (hist && n_gauges > 0 && !gauge_off) return fail(HP_EINVAL, "hist needs gauge offsets");
   hipStream_t st = (hipStream_t)stream;
+  void* own[3] = {g->h, g->qx, g->qy};
+  void* alt[3] = {g->h_out, g->qx_out, g->qy_out};
+  auto copy3 = [&](void* const* dst, void* const* src) -> int {
+    for (int p = 0; p < 3; ++p)
+      if (hipMemcpyAsync(dst[p], src[p], plane_bytes, hipMemcpyDeviceToDevice, st) != hipSuccess)
+        return fail(HP_ELAUNCH, "state copy failed");
+    return HP_OK;
+  };
+  auto seg_planes = [&](int32_t j, void** out) {
+    char* b = (char*)seg_u + (int64_t)j * seg_stride;
+    out[0] = b;
+    out[1] = b + plane_bytes;
+    out[2] = b + 2 * plane_bytes;
+  };
+  int rc;
+  if (seg_u && nsteps > 0) {  // the state before the first step
+    void* s0[3];
+    seg_planes(0, s0);
+    if ((rc = copy3(s0, own))) return rc;
+  }
   for (int32_t j = 0; j < nsteps; ++j) {
     const int32_t k = k0 + j;
-    hipLaunchKernelGGL(k_schedule, dim3(1), dim3(64), 0, st, g->ctrl, sched, k, record ? 1 : 0);
-    int rc = check_launch("k_schedule");
-    if (rc) return rc;
     hp_grid_desc d = *g;
-    if (seg_u) {
-      char* su = (char*)seg_u + (int64_t)j * seg_stride;
-      const void* src[3] = {g->h, g->qx, g->qy};
-      if ((char*)g->qx == (char*)g->h + plane_bytes && (char*)g->qy == (char*)g->qx + plane_bytes) {
-        if (hipMemcpyAsync(su, g->h, 3 * plane_bytes, hipMemcpyDeviceToDevice, st) != hipSuccess)
-          return fail(HP_ELAUNCH, "state copy failed");
-      } else {
-        for (int p = 0; p < 3; ++p)
-          if (hipMemcpyAsync(su + p * plane_bytes, src[p], plane_bytes, hipMemcpyDeviceToDevice, st) != hipSuccess)
-            return fail(HP_ELAUNCH, "state copy failed");
-      }
-      d = with_segment(d, nullptr, (char*)seg_d + (int64_t)j * seg_stride, plane_bytes);
+    if (frozen) {
+      d.ctrl = const_cast<hp_ctrl*>(frozen) + k;  // read only: no time control pass runs
+    } else if (sched) {
+      hipLaunchKernelGGL(k_schedule_save, dim3(1), dim3(64), 0, st, g->ctrl, sched, k);
+      if ((rc = check_launch("k_schedule_save"))) return rc;
     }
-    rc = grid_flux_launch(&d, st);
-    if (rc) return rc;
-    if (last_flux_only && j == nsteps - 1) break;
-    rc = grid_update_launch(&d, record != 0, st);
-    if (rc) return rc;
+    void *in[3], *out[3];
+    if (seg_u) {
+      // state j in the segment, state j + 1 into the next slot (the caller's
+      // planes after the last step), the flux into the segment's flux slot
+      seg_planes(j, in);
+      if (j + 1 < nsteps) seg_planes(j + 1, out);
+      else for (int p = 0; p < 3; ++p) out[p] = own[p];
+      char* f = (char*)seg_d + (int64_t)j * seg_stride;
+      d.dh = f;
+      d.dqx = f + plane_bytes;
+      d.dqy = f + 2 * plane_bytes;
+    } else {
+      for (int p = 0; p < 3; ++p) {
+        in[p] = (j & 1) ? alt[p] : own[p];
+        out[p] = (j & 1) ? own[p] : alt[p];
+      }
+    }
+    d.h = in[0]; d.qx = in[1]; d.qy = in[2];
+    d.h_out = out[0]; d.qx_out = out[1]; d.qy_out = out[2];
+    if ((rc = grid_fused_launch(&d, seg_u != nullptr, frozen == nullptr, st))) return rc;
     if (hist && n_gauges > 0) {
       char* row = (char*)hist + (int64_t)(k + 1) * n_gauges * (g->dtype == HP_F32 ? 4 : 8);
-      rc = hp_gather(g->dtype, g->h, gauge_off, n_gauges, row, stream);
-      if (rc) return rc;
+      if ((rc = hp_gather(g->dtype, out[0], gauge_off, n_gauges, row, stream))) return rc;
     }
   }
+  if (!seg_u && (nsteps & 1)) return copy3(own, alt);  // the result in the caller's planes
   return HP_OK;
 }
 
-int hp_calib_reverse(const hp_grid_desc* g, const hp_adjoint_desc* ad, double* sched, int32_t k_lo, int32_t k_hi,
-                     int32_t seg_k0, void* seg_u, void* seg_d, int64_t seg_stride, const int64_t* gauge_off,
-                     int32_t n_gauges, const double* h_rec, const double* obs, void* stream) {
-  if (!g || !ad || !sched || !seg_u || !seg_d) return fail(HP_EINVAL, "null descriptor, schedule or segment");
+int hp_calib_reverse(const hp_grid_desc* g, const hp_adjoint_desc* ad, const hp_ctrl* frozen, int32_t k_lo,
+                     int32_t k_hi, int32_t seg_k0, void* seg_u, void* seg_d, int64_t seg_stride,
+                     const int64_t* gauge_off, int32_t n_gauges, const double* h_rec, const double* obs,
+                     void* stream) {
+  if (!g || !ad || !frozen || !seg_u || !seg_d) return fail(HP_EINVAL, "null descriptor, control blocks or segment");
   if (!ad->flux_ready) return fail(HP_EINVAL, "the reverse sweep runs on saved fluxes: flux_ready must be 1");
   if (k_lo < seg_k0 || k_hi < k_lo) return fail(HP_EINVAL, "need seg_k0 <= k_lo <= k_hi");
   const int64_t plane_bytes = g->pitch * (int64_t)g->ny * (g->dtype == HP_F32 ? 4 : 8);
   if (seg_stride < 3 * plane_bytes) return fail(HP_EINVAL, "seg_stride smaller than three planes");
   if (n_gauges > 0 && (!gauge_off || !h_rec || !obs)) return fail(HP_EINVAL, "the loss needs gauges, h_rec and obs");
-  hipStream_t st = (hipStream_t)stream;
-  const unsigned nb = (unsigned)((n_gauges > 0 ? n_gauges + 255 : 256) / 256);
   for (int32_t k = k_hi - 1; k >= k_lo; --k) {
-    const double* hr = h_rec ? h_rec + (int64_t)(k + 1) * n_gauges : nullptr;
-    const double* ob = obs ? obs + (int64_t)(k + 1) * n_gauges : nullptr;
-    if (g->dtype == HP_F32)
-      hipLaunchKernelGGL(k_reverse_prologue<float>, dim3(nb), dim3(256), 0, st, g->ctrl, sched, k,
-                         (float*)ad->lam_h, gauge_off, hr, ob, n_gauges);
-    else
-      hipLaunchKernelGGL(k_reverse_prologue<double>, dim3(nb), dim3(256), 0, st, g->ctrl, sched, k,
-                         (double*)ad->lam_h, gauge_off, hr, ob, n_gauges);
-    int rc = check_launch("k_reverse_prologue");
-    if (rc) return rc;
+    int rc;
+    if (n_gauges > 0) {
+      rc = hp_adjoint_add_loss(g->dtype, ad->lam_h, gauge_off, h_rec + (int64_t)(k + 1) * n_gauges,
+                               obs + (int64_t)(k + 1) * n_gauges, n_gauges, 1.0, stream);
+      if (rc) return rc;
+    }
     const int64_t off = (int64_t)(k - seg_k0) * seg_stride;
-    const hp_grid_desc d = with_segment(*g, (char*)seg_u + off, (char*)seg_d + off, plane_bytes);
+    hp_grid_desc d = with_segment(*g, (char*)seg_u + off, (char*)seg_d + off, plane_bytes);
+    d.ctrl = const_cast<hp_ctrl*>(frozen) + k;  // t and dt of step k; read only
     rc = hp_adjoint_step(&d, ad, stream);
     if (rc) return rc;
   }

@@ -319,7 +319,8 @@ __device__ inline void update_state(const CellIn<T>& in, T& h, T& qx, T& qy, T d
 // planes stay untouched for the neighbouring tiles' halos), h_max, and one
 // partial record (CFL minimum, FP64 sums) per tile.
 template <class T, int ORDER, int TY, bool FUSED>
-__device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by, const Partials& part, int64_t pidx) {
+__device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by, const Partials& part, int64_t pidx,
+                                 bool save_flux = false) {
   __shared__ FluxTile<T, TY> s;
   __shared__ T s_man[FUSED ? 256 : 1];
   __shared__ T s_rate[FUSED ? kRateCache : 1];
@@ -482,6 +483,11 @@ __device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by, const Pa
       dqx[gi] = aqx;
       dqy[gi] = aqy;
     } else {
+      if (save_flux) {  // the step's flux as well (a calibration run's saved steps)
+        dh[gi] = ah;
+        dqx[gi] = aqx;
+        dqy[gi] = aqy;
+      }
       CellIn<T> in = cin[k];
       in.n = s_man[in.cls];
       if (a.rain_mode == HP_RAIN_STATION)
@@ -539,9 +545,9 @@ __global__ __launch_bounds__(kThreads) void k_flux_multi(const hp_grid_desc* __r
 
 // Single-pass step: reads (h, qx, qy), writes (h_out, qx_out, qy_out).
 template <class T, int ORDER, int TY>
-__global__ __launch_bounds__(kThreads) void k_step_fused(hp_grid_desc a, int ntx) {
+__global__ __launch_bounds__(kThreads) void k_step_fused(hp_grid_desc a, int ntx, int save_flux) {
   const int tile = xcd_band(blockIdx.x, gridDim.x);
-  flux_tile<T, ORDER, TY, true>(a, tile % ntx, tile / ntx, Partials(a.work, gridDim.x), blockIdx.x);
+  flux_tile<T, ORDER, TY, true>(a, tile % ntx, tile / ntx, Partials(a.work, gridDim.x), blockIdx.x, save_flux != 0);
 }
 
 // Infiltration pass over plane elements: a valid cell listed by the flux
@@ -898,24 +904,30 @@ static int launch_flux(const hp_grid_desc* g, hipStream_t st) {
 }
 
 template <class T, int ORDER, int TY>
-static int launch_fused_t(const hp_grid_desc* g, hipStream_t st) {
+static int launch_fused_t(const hp_grid_desc* g, hipStream_t st, int save_flux) {
   const int ntx = (g->nx + kTX - 1) / kTX, nty = (g->ny + TY - 1) / TY;
-  hipLaunchKernelGGL((k_step_fused<T, ORDER, TY>), dim3(ntx * nty), dim3(kThreads), 0, st, *g, ntx);
+  hipLaunchKernelGGL((k_step_fused<T, ORDER, TY>), dim3(ntx * nty), dim3(kThreads), 0, st, *g, ntx, save_flux);
   return check_launch("k_step_fused");
 }
 
-static int launch_step_fused(const hp_grid_desc* g, hipStream_t st) {
+static int launch_step_fused(const hp_grid_desc* g, hipStream_t st, int save_flux = 0) {
   const int ty = flux_rows(g);
   if (g->dtype == HP_F32) {
     if (g->order == 2) {
-      if (ty == 4) return launch_fused_t<float, 2, 4>(g, st);
-      return ty == 16 ? launch_fused_t<float, 2, 16>(g, st) : launch_fused_t<float, 2, 8>(g, st);
+      if (ty == 4) return launch_fused_t<float, 2, 4>(g, st, save_flux);
+      return ty == 16 ? launch_fused_t<float, 2, 16>(g, st, save_flux) : launch_fused_t<float, 2, 8>(g, st, save_flux);
     }
-    if (ty == 4) return launch_fused_t<float, 1, 4>(g, st);
-    return ty == 16 ? launch_fused_t<float, 1, 16>(g, st) : launch_fused_t<float, 1, 8>(g, st);
+    if (ty == 4) return launch_fused_t<float, 1, 4>(g, st, save_flux);
+    return ty == 16 ? launch_fused_t<float, 1, 16>(g, st, save_flux) : launch_fused_t<float, 1, 8>(g, st, save_flux);
   }
-  if (g->order == 2) return launch_fused_t<double, 2, 4>(g, st);
-  return launch_fused_t<double, 1, 4>(g, st);
+  if (g->order == 2) return launch_fused_t<double, 2, 4>(g, st, save_flux);
+  return launch_fused_t<double, 1, 4>(g, st, save_flux);
+}
+
+static int launch_finalize_fused(const hp_grid_desc* g, hipStream_t st) {
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, st, g->ctrl, g->work, (int)flux_tiles(g), g->dx,
+                     g->mass_sums);
+  return check_launch("k_finalize");
 }
 
 static int launch_infiltration(const hp_grid_desc* g, hipStream_t st) {
@@ -1030,6 +1042,21 @@ int grid_update_launch(const hp_grid_desc* g, bool finalize, hipStream_t st) {
   return launch_update(g, HP_UPDATE_STEP, st, finalize);
 }
 
+// Single-pass step (h, qx, qy -> h_out, qx_out, qy_out; with save_flux the
+// step's flux also into dh, dqx, dqy) and, with finalize, its time control;
+// shared with the calibration runs.
+int grid_fused_launch(const hp_grid_desc* g, bool save_flux, bool finalize, hipStream_t st) {
+  int rc = check_desc(g);
+  if (rc) return rc;
+  if (!g->h_out || !g->qx_out || !g->qy_out) return fail(HP_EINVAL, "the fused step needs h_out, qx_out, qy_out");
+  if (save_flux && (!g->dh || !g->dqx || !g->dqy)) return fail(HP_EINVAL, "saving the flux needs dh, dqx, dqy");
+  if (!g->work || g->work_bytes < partials_bytes(flux_tiles(g)))
+    return fail(HP_ESPACE, "workspace smaller than hp_grid_workspace_bytes()");
+  rc = launch_step_fused(g, st, save_flux ? 1 : 0);
+  if (rc || !finalize) return rc;
+  return launch_finalize_fused(g, st);
+}
+
 // Flux pass launch shared with the adjoint (which recomputes it).
 int grid_flux_launch(const hp_grid_desc* g, hipStream_t st) {
   int rc = check_desc(g);
@@ -1062,9 +1089,7 @@ int hp_grid_step_fused(const hp_grid_desc* g, int32_t nsteps, void* stream) {
   for (int32_t k = 0; k < nsteps; ++k) {
     rc = launch_step_fused(&d, st);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, st, d.ctrl, d.work, (int)nt, d.dx,
-                       d.mass_sums);
-    rc = check_launch("k_finalize");
+    rc = launch_finalize_fused(&d, st);
     if (rc) return rc;
     std::swap(d.h, d.h_out);
     std::swap(d.qx, d.qx_out);

@@ -104,8 +104,7 @@ SIGNATURES = {
     "hp_adjoint_step": (_I, [ctypes.POINTER(HpGridDesc), ctypes.POINTER(HpAdjointDesc), _P]),
     "hp_adjoint_finish": (_I, [ctypes.POINTER(HpGridDesc), ctypes.POINTER(HpAdjointDesc), _P]),
     "hp_adjoint_add_loss": (_I, [_I, _P, _P, _P, _P, _I32, ctypes.c_double, _P]),
-    "hp_calib_forward": (_I, [ctypes.POINTER(HpGridDesc), _P, _I32, _I32, _I32, _P, _P, _I64, _I32, _P, _I32, _P,
-                              _P]),
+    "hp_calib_forward": (_I, [ctypes.POINTER(HpGridDesc), _P, _P, _I32, _I32, _P, _P, _I64, _P, _I32, _P, _P]),
     "hp_calib_reverse": (_I, [ctypes.POINTER(HpGridDesc), ctypes.POINTER(HpAdjointDesc), _P, _I32, _I32, _I32, _P,
                               _P, _I64, _P, _I32, _P, _P, _P]),
     "hp_gather": (_I, [_I, _P, _P, _I64, _P, _P]),

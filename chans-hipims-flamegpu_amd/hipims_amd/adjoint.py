@@ -48,6 +48,7 @@ class Calibration:
         if slopes not in ("exact", "frozen"):
             raise ValueError("slopes must be 'exact' or 'frozen'")
         self.s = s = solver
+        s.ensure_out()  # the calibration runs use the single-pass step
         self.K = int(steps)
         self.ck = int(checkpoint)
         dev = s.device
@@ -60,6 +61,8 @@ class Calibration:
         self.hmax0 = s.h_max.clone()
         self.ctrl0 = s.ctrl.clone()
         self.sched = torch.zeros(self.K + 1, 2, dtype=torch.float64, device=dev)
+        # one read-only control block per step of the frozen schedule
+        self.frozen = torch.zeros(self.K + 1, _lib.CTRL_DOUBLES, dtype=torch.float64, device=dev)
         self.sched_ready = False
         self.hist = torch.zeros(self.K + 1, self.G, dtype=s.dtype, device=dev)
         self.obs = None if obs is None else torch.as_tensor(obs, dtype=torch.float64, device=dev)
@@ -119,13 +122,17 @@ class Calibration:
     def _seg_stride(self) -> int:
         return self.segU[0].numel() * self.segU.element_size()
 
-    def _forward(self, sched, record, k0, n, seg=False, last_flux_only=False, gauges=True):
-        """Steps k0 .. k0+n-1 in one native call (hp_calib_forward)."""
+    def _forward(self, mode, k0, n, seg=False, gauges=True):
+        """Steps k0 .. k0+n-1 in one native call (hp_calib_forward) of the
+        single-pass step, with the states and fluxes kept in segU / segD when
+        seg is set.  mode: "record" (own CFL steps, schedule saved), "free"
+        (own CFL steps, nothing saved) or "frozen" (the recorded schedule)."""
         s = self.s
         check(lib().hp_calib_forward(
-            ctypes.byref(s.desc), sched.data_ptr(), int(record), int(k0), int(n),
+            ctypes.byref(s.desc), self.sched.data_ptr() if mode == "record" else None,
+            self.frozen.data_ptr() if mode == "frozen" else None, int(k0), int(n),
             self.segU.data_ptr() if seg else None, self.segD.data_ptr() if seg else None,
-            self._seg_stride() if seg else 0, int(last_flux_only),
+            self._seg_stride() if seg else 0,
             self.gauge_off.data_ptr() if gauges else None, self.G if gauges else 0,
             self.hist.data_ptr() if gauges else None, _lib.stream_ptr(s.device)), "hp_calib_forward")
 
@@ -136,17 +143,19 @@ class Calibration:
         self._restore_initial()
         self.ckpts.clear()
         self._record_gauges(0)
-        record = not self.sched_ready
+        mode = "frozen" if self.sched_ready else "record"
         if keep_checkpoints and self.ck == 1:
             # every state and flux kept: nothing is recomputed in reverse
-            self._forward(self.sched, record, 0, self.K, seg=True)
+            self._forward(mode, 0, self.K, seg=True)
             self.ckpts = {0: self.segU[0]}
         else:
             for a in range(0, self.K, self.ck):
                 if keep_checkpoints:
                     self.ckpts[a] = self.s.U.clone()
-                self._forward(self.sched, record, a, min(self.ck, self.K - a))
-        self.sched_ready = True
+                self._forward(mode, a, min(self.ck, self.K - a))
+        if not self.sched_ready:
+            self.frozen[:, 0:2].copy_(self.sched)
+            self.sched_ready = True
         return self.loss()
 
     def simulate(self, theta) -> torch.Tensor:
@@ -155,8 +164,7 @@ class Calibration:
         self.set_theta(theta)
         self._restore_initial()
         self._record_gauges(0)
-        scratch = torch.zeros_like(self.sched)
-        self._forward(scratch, True, 0, self.K)
+        self._forward("free", 0, self.K)
         return self.hist.double().clone()
 
     def loss(self) -> torch.Tensor:
@@ -184,7 +192,7 @@ class Calibration:
 
         def reverse(k_lo, k_hi, seg_k0):
             check(lib().hp_calib_reverse(
-                ctypes.byref(s.desc), ctypes.byref(self.adesc), self.sched.data_ptr(), int(k_lo), int(k_hi),
+                ctypes.byref(s.desc), ctypes.byref(self.adesc), self.frozen.data_ptr(), int(k_lo), int(k_hi),
                 int(seg_k0), self.segU.data_ptr(), self.segD.data_ptr(), stride, self.gauge_off.data_ptr(), self.G,
                 h_rec.data_ptr(), obs.data_ptr(), stream), "hp_calib_reverse")
 
@@ -196,7 +204,7 @@ class Calibration:
                 # recompute the segment on the frozen schedule: the state
                 # before each step and its flux (the last step's flux only)
                 s.U.copy_(self.ckpts[a])
-                self._forward(self.sched, False, a, b - a, seg=True, last_flux_only=True, gauges=False)
+                self._forward("frozen", a, b - a, seg=True, gauges=False)
                 reverse(a, b, a)
         check(lib().hp_adjoint_finish(ctypes.byref(s.desc), ctypes.byref(self.adesc), stream), "hp_adjoint_finish")
         g = self.grads.sum(dim=0).cpu()

@@ -352,13 +352,17 @@ class GridSolver:
         finally:
             d.infil_dF = None
 
-    def step_fused(self, nsteps: int = 1):
-        """nsteps single-pass steps (same results as step(); dh/dqx/dqy untouched)."""
+    def ensure_out(self):
+        """Allocate the second state buffer of the single-pass step."""
         if self.h_out is None:
             self.h_out = torch.zeros_like(self.h)
             self.qx_out = torch.zeros_like(self.h)
             self.qy_out = torch.zeros_like(self.h)
             self._desc = None
+
+    def step_fused(self, nsteps: int = 1):
+        """nsteps single-pass steps (same results as step(); dh/dqx/dqy untouched)."""
+        self.ensure_out()
         check(lib().hp_grid_step_fused(ctypes.byref(self.desc), int(nsteps), self._stream()), "hp_grid_step_fused")
 
     # -- views -----------------------------------------------------------------

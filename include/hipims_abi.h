@@ -285,29 +285,33 @@ int hp_adjoint_add_loss(int dtype, void* lam_h, const int64_t* offsets, const do
                         int32_t n, double weight, void* stream);
 
 /* ---- Calibration loops (no reference counterpart): the per-step sequences
- * of a gauge-misfit calibration run natively, one call per segment.
+ * of a gauge-misfit calibration run natively, one call per segment, on the
+ * single-pass step.
  *
- * hp_calib_forward runs steps k0 .. k0+nsteps-1.  record 1: the model's own
- * CFL steps, with (t, dt) before step k saved to sched[k]; record 0: the
- * frozen schedule, t and dt of step k taken from sched[k] and no time
- * control pass.  sched: device FP64 [K+1][2].  seg_u / seg_d (both null or
- * both set): before step k the state planes h, qx, qy are copied to
- * seg_u + (k-k0)*seg_stride bytes (three planes of pitch*ny elements one
- * after another) and the step's flux is written to seg_d + (k-k0)*seg_stride
- * instead of dh/dqx/dqy.  last_flux_only 1: the last step stops after its
- * flux (the reverse sweep's recomputation).  hist (may be null): the depths
- * at the n_gauges plane offsets after step k go to row k+1 (grid dtype). */
-int hp_calib_forward(const hp_grid_desc* g, double* sched, int32_t record, int32_t k0, int32_t nsteps, void* seg_u,
-                     void* seg_d, int64_t seg_stride, int32_t last_flux_only, const int64_t* gauge_off,
-                     int32_t n_gauges, void* hist, void* stream);
+ * hp_calib_forward runs steps k0 .. k0+nsteps-1.  frozen null: the model's
+ * own CFL steps on g->ctrl, with (t, dt) before step k saved to sched[k]
+ * when sched (device FP64 [K+1][2]) is given.  frozen non-null: step k
+ * reads its t and dt from the control block frozen[k] (device, read only)
+ * and no time control pass runs.  seg_u / seg_d (both null or both set):
+ * the state before step k is kept at seg_u + (k-k0)*seg_stride bytes (three
+ * planes of pitch*ny elements one after another; the first is copied from
+ * h, qx, qy, the others are written by the steps) and the step's flux at
+ * seg_d + (k-k0)*seg_stride; the state after the last step goes to h, qx,
+ * qy.  Without segments the steps alternate between h, qx, qy and h_out,
+ * qx_out, qy_out and end in h, qx, qy.  hist (may be null): the depths at
+ * the n_gauges plane offsets after step k go to row k+1 (grid dtype). */
+int hp_calib_forward(const hp_grid_desc* g, double* sched, const hp_ctrl* frozen, int32_t k0, int32_t nsteps,
+                     void* seg_u, void* seg_d, int64_t seg_stride, const int64_t* gauge_off, int32_t n_gauges,
+                     void* hist, void* stream);
 /* hp_calib_reverse runs the reverse sweep over steps k_hi-1 down to k_lo:
  * per step the loss cotangent of the depths after it (hp_adjoint_add_loss
- * with row k+1 of h_rec and obs, FP64 [K+1][n_gauges]), t and dt of step k
- * from sched, and hp_adjoint_step on the state and flux saved at
+ * with row k+1 of h_rec and obs, FP64 [K+1][n_gauges]) and hp_adjoint_step
+ * with the control block frozen[k] on the state and flux saved at
  * seg_u / seg_d + (k - seg_k0)*seg_stride bytes (flux_ready must be 1). */
-int hp_calib_reverse(const hp_grid_desc* g, const hp_adjoint_desc* a, double* sched, int32_t k_lo, int32_t k_hi,
-                     int32_t seg_k0, void* seg_u, void* seg_d, int64_t seg_stride, const int64_t* gauge_off,
-                     int32_t n_gauges, const double* h_rec, const double* obs, void* stream);
+int hp_calib_reverse(const hp_grid_desc* g, const hp_adjoint_desc* a, const hp_ctrl* frozen, int32_t k_lo,
+                     int32_t k_hi, int32_t seg_k0, void* seg_u, void* seg_d, int64_t seg_stride,
+                     const int64_t* gauge_off, int32_t n_gauges, const double* h_rec, const double* obs,
+                     void* stream);
 
 /* out[i] = plane[offsets[i]]  (compact gather, Godunov.get_h). */
 int hp_gather(int dtype, const void* plane, const int64_t* offsets, int64_t n, void* out, void* stream);

@@ -291,7 +291,7 @@ def _fp32_errors(h, qx, qy, o64):
     q64 = np.hypot(o64.qx_c, o64.qy_c)
     dh = np.abs(h - o64.h_c)
     dq = np.abs(q - q64)
-    return {"max_dh": dh.max(), "l1_h": dh.sum() / o64.h_c.sum(), "l1_q": dq.sum() / q64.sum(),
+    return {"max_dh": dh.max(), "max_dq": dq.max(), "l1_h": dh.sum() / o64.h_c.sum(), "l1_q": dq.sum() / q64.sum(),
             "p999_dq": float(np.quantile(dq, 0.999)), "vol": abs(h.sum() - o64.h_c.sum()) / o64.h_c.sum()}
 
 
