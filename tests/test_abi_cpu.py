@@ -59,6 +59,32 @@ def test_host_side_argument_checks():
     assert L.hp_grid_workspace_bytes(None) == 0
 
 
+def test_calibration_loop_argument_checks():
+    L = _lib.lib()
+    d = HpGridDesc()
+    d.order, d.nx, d.ny, d.pitch, d.dx = 1, 64, 64, 64, 1.0
+    assert L.hp_calib_forward(None, None, None, 0, 1, None, None, 0, None, 0, None, None) == _lib.HP_EINVAL
+    assert L.hp_calib_forward(ctypes.byref(d), None, None, 0, -1, None, None, 0, None, 0, None, None) == _lib.HP_EINVAL
+    # a segment needs both its state and its flux slots, and room for three planes
+    assert L.hp_calib_forward(ctypes.byref(d), None, None, 0, 1, 16, None, 0, None, 0, None, None) == _lib.HP_EINVAL
+    assert b"together" in L.hp_last_error()
+    assert L.hp_calib_forward(ctypes.byref(d), None, None, 0, 1, 16, 16, 64, None, 0, None, None) == _lib.HP_EINVAL
+    assert b"seg_stride" in L.hp_last_error()
+    # without segments the steps need the second state buffer
+    assert L.hp_calib_forward(ctypes.byref(d), None, None, 0, 1, None, None, 0, None, 0, None, None) == _lib.HP_EINVAL
+    assert b"h_out" in L.hp_last_error()
+    ad = _lib.HpAdjointDesc()
+    assert L.hp_calib_reverse(ctypes.byref(d), ctypes.byref(ad), None, 0, 1, 0, 16, 16, 1 << 20, None, 0, None,
+                              None, None) == _lib.HP_EINVAL
+    ad.flux_ready = 0
+    assert L.hp_calib_reverse(ctypes.byref(d), ctypes.byref(ad), 16, 0, 1, 0, 16, 16, 1 << 20, None, 0, None,
+                              None, None) == _lib.HP_EINVAL
+    assert b"flux_ready" in L.hp_last_error()
+    ad.flux_ready = 1
+    assert L.hp_calib_reverse(ctypes.byref(d), ctypes.byref(ad), 16, 2, 1, 0, 16, 16, 1 << 20, None, 0, None,
+                              None, None) == _lib.HP_EINVAL
+
+
 def test_ops_reject_host_tensors_like_the_reference():
     h = torch.zeros(4, dtype=torch.float64)
     with pytest.raises(RuntimeError, match="must be a CUDA tensor"):
