@@ -55,6 +55,22 @@ def profiled_traffic(kernel: str, workload: str):
     return None
 
 
+def profiled_valu(kernel: str, workload: str):
+    """VALU lane-instructions per cell of `kernel` on `workload` from the
+    rocprofv3 SQ_INSTS_VALU run summarised in profiles/, or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("| SQ_INSTS_VALU |") and "(" in line:
+            return float(line.split("(")[1].split()[0])
+    return None
+
+
+# Non-packed VALU issue rate of MI355X: 256 CUs x 64 lanes per clock x 2.4 GHz.
+VALU_LANE_OPS_PER_S = 256 * 64 * 2.4e9
+
+
 def cached_case(size: int):
     """The roofline case (slow to generate at 8192^2), kept in the temporary
     directory between runs of one session."""
@@ -132,6 +148,33 @@ def time_fused(g, steps):
     total = ev[0].elapsed_time(ev[-1])
     return {"path": "fused", "steps": 2 * pairs, "total_ms": total, "wall_s": wall,
             "kernels_ms": {"k_step_fused": total / (2 * pairs)}}
+
+
+def time_fused_kernel(g, steps):
+    """The single-pass kernel alone: `steps` launches back to back on the
+    current dt (frozen control blocks, no time-control pass, the buffers
+    alternating), one call, HIP events around it; the state is restored
+    afterwards."""
+    import ctypes
+
+    import torch
+
+    from hipims_amd import _lib
+    steps += steps & 1
+    g.ensure_out()
+    saved = [g.U.clone(), g.h_max.clone()]
+    frozen = g.ctrl.reshape(1, -1).repeat(steps, 1).contiguous()
+    stream = torch.cuda.current_stream(g.device)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    a.record(stream)
+    _lib.check(_lib.lib().hp_calib_forward(ctypes.byref(g.desc), None, frozen.data_ptr(), 0, steps, None, None, 0,
+                                           None, 0, None, _lib.stream_ptr(g.device)), "hp_calib_forward")
+    b.record(stream)
+    torch.cuda.synchronize()
+    g.U.copy_(saved[0])
+    g.h_max.copy_(saved[1])
+    return a.elapsed_time(b) / steps
 
 
 def parse():
@@ -464,6 +507,7 @@ def main():
     two = time_two_pass(g, a.steps)
     g.set_tile_rows(rows["fused"])
     fused = time_fused(g, a.steps)
+    fused_kernel_ms = time_fused_kernel(g, min(a.steps, 400))
     ctrl = g.ctrl_dict()
     assert np.isfinite(ctrl["t"]) and ctrl["dt"] > 0
 
@@ -471,10 +515,14 @@ def main():
     best = fused if fused["total_ms"] < two["total_ms"] else two
     ms_step = best["total_ms"] / best["steps"]
     value = cells * best["steps"] / (best["total_ms"] / 1e3)
-    per_kernel = best["kernels_ms"]
+    per_kernel = dict(best["kernels_ms"])
+    if best["path"] == "fused":
+        # the kernel's own duration (the step adds the time-control pass)
+        per_kernel = {"k_step_fused": fused_kernel_ms, "k_finalize+gaps": ms_step - fused_kernel_ms}
     dom = max(per_kernel, key=per_kernel.get)
     achieved = B[dom] * cells / (per_kernel[dom] / 1e3) / 1e9
     two_pass = (B["k_flux"] + B["k_update"]) * value / 1e9
+    valu = profiled_valu(dom, workload_tag(a))
     t_wall = best["wall_s"]
 
     cpu = None
@@ -505,7 +553,10 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": profiled_traffic(dom, workload_tag(a)),
-                     "alg_bytes_per_cell": B[dom], "kernel_ms": per_kernel[dom]},
+                     "alg_bytes_per_cell": B[dom], "kernel_ms": per_kernel[dom],
+                     "valu_lane_instr_per_cell": valu,
+                     "valu_issue_frac": None if valu is None else
+                     valu * cells / (per_kernel[dom] / 1e3) / VALU_LANE_OPS_PER_S},
         "kernels_ms": per_kernel,
         "path": best["path"],
         "paths_ms_per_step": {"two_pass": two["total_ms"] / two["steps"], "fused": fused["total_ms"] / fused["steps"]},

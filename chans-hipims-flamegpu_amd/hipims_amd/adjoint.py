@@ -66,7 +66,10 @@ class Calibration:
         self.sched_ready = False
         self.hist = torch.zeros(self.K + 1, self.G, dtype=s.dtype, device=dev)
         self.obs = None if obs is None else torch.as_tensor(obs, dtype=torch.float64, device=dev)
-        self.ckpts: dict[int, list[torch.Tensor]] = {}
+        self.ckpts: dict[int, torch.Tensor] = {}
+        # checkpoint states, allocated once (no allocation inside a run)
+        self.ckpt_store = None if self.ck == 1 else torch.zeros(
+            ((self.K + self.ck - 1) // self.ck,) + tuple(s.U.shape), dtype=s.dtype, device=dev)
         # adjoint planes
         L = s.layout
         self.cpitch = int(np.ceil((L.nx + 1) / 64) * 64)
@@ -151,7 +154,9 @@ class Calibration:
         else:
             for a in range(0, self.K, self.ck):
                 if keep_checkpoints:
-                    self.ckpts[a] = self.s.U.clone()
+                    slot = self.ckpt_store[a // self.ck]
+                    slot.copy_(self.s.U)
+                    self.ckpts[a] = slot
                 self._forward(mode, a, min(self.ck, self.K - a))
         if not self.sched_ready:
             self.frozen[:, 0:2].copy_(self.sched)
