@@ -67,8 +67,21 @@ def profiled_valu(kernel: str, workload: str):
     return None
 
 
-# Non-packed VALU issue rate of MI355X: 256 CUs x 64 lanes per clock x 2.4 GHz.
+# Non-packed VALU issue rate of MI355X at its peak clock: 256 CUs x 64 lanes
+# per clock x 2.4 GHz; the rate measured by `--mode valu` (profiles/) is used
+# when present.
 VALU_LANE_OPS_PER_S = 256 * 64 * 2.4e9
+
+
+def valu_peak():
+    """(lane-instructions/s, source): the FMA probe's measured rate from
+    profiles/valu_probe_fma.md, else the peak-clock figure."""
+    path = os.path.join(ROOT, "profiles", "valu_probe_fma.md")
+    if os.path.exists(path):
+        for line in open(path):
+            if line.startswith("| FMA lane-instructions per second |"):
+                return float(line.split("|")[2].split()[0]), "measured (profiles/valu_probe_fma.md)"
+    return VALU_LANE_OPS_PER_S, "peak clock"
 
 
 def cached_case(size: int):
@@ -195,8 +208,9 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     p.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, available cores)")
     p.add_argument("--no-cpu", action="store_true")
-    p.add_argument("--mode", choices=["step", "copy", "ensemble", "adjoint"], default="step",
+    p.add_argument("--mode", choices=["step", "copy", "valu", "ensemble", "adjoint"], default="step",
                    help="copy: a plain device copy of known size, the reference point for HBM byte counters; "
+                        "valu: an FP32 FMA chain kernel, the reference point for the VALU issue rate; "
                         "ensemble: 64 uneven scenarios, graph-replayed ensemble vs the same run one by one "
                         "through the Godunov driver; adjoint: gradient cost against a plain forward run")
     p.add_argument("--copy-mib", type=int, default=1024)
@@ -235,6 +249,33 @@ def copy_calibration(a):
     ms = s.elapsed_time(e) / a.steps
     print(json.dumps({"mode": "copy", "bytes_read_per_copy": n * 4, "bytes_written_per_copy": n * 4,
                       "ms_per_copy": ms, "GBps_read_plus_write": 2 * n * 4 / (ms / 1e3) / 1e9}), flush=True)
+
+
+def valu_calibration(a):
+    """The FP32 FMA probe (hp_probe_fma): eight independent non-packed FMA
+    chains per lane, 32 blocks of 256 lanes per CU; HIP events around
+    --steps launches give FMA lane-instructions per second."""
+    import torch
+
+    from hipims_amd import _lib
+    L = _lib.lib()
+    dev = torch.device("cuda", 0)
+    blocks = 256 * 32
+    out = torch.empty(blocks * 256, dtype=torch.float32, device=dev)
+    st = _lib.stream_ptr(dev)
+    _lib.check(L.hp_probe_fma(blocks, out.data_ptr(), st), "hp_probe_fma")
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(a.steps):
+        _lib.check(L.hp_probe_fma(blocks, out.data_ptr(), st), "hp_probe_fma")
+    e.record()
+    torch.cuda.synchronize()
+    ms = s.elapsed_time(e) / a.steps
+    lanes = blocks * 256 * int(L.hp_probe_fma_per_lane())
+    print(json.dumps({"mode": "valu", "fma_lane_instructions_per_launch": lanes, "ms_per_launch": ms,
+                      "fma_lane_instructions_per_s": lanes / (ms / 1e3),
+                      "fp32_tflops_non_packed": 2 * lanes / (ms / 1e3) / 1e12}), flush=True)
 
 
 def _ensemble_case(spec):
@@ -462,6 +503,8 @@ def main():
     a = parse()
     if a.mode == "copy":
         return copy_calibration(a)
+    if a.mode == "valu":
+        return valu_calibration(a)
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
                                    "described, not implemented)"}))
@@ -523,6 +566,7 @@ def main():
     achieved = B[dom] * cells / (per_kernel[dom] / 1e3) / 1e9
     two_pass = (B["k_flux"] + B["k_update"]) * value / 1e9
     valu = profiled_valu(dom, workload_tag(a))
+    vpeak, vsrc = valu_peak()
     t_wall = best["wall_s"]
 
     cpu = None
@@ -555,8 +599,8 @@ def main():
                      "traffic": profiled_traffic(dom, workload_tag(a)),
                      "alg_bytes_per_cell": B[dom], "kernel_ms": per_kernel[dom],
                      "valu_lane_instr_per_cell": valu,
-                     "valu_issue_frac": None if valu is None else
-                     valu * cells / (per_kernel[dom] / 1e3) / VALU_LANE_OPS_PER_S},
+                     "valu_issue_frac": None if valu is None else valu * cells / (per_kernel[dom] / 1e3) / vpeak,
+                     "valu_peak_lane_instr_per_s": vpeak, "valu_peak_source": vsrc},
         "kernels_ms": per_kernel,
         "path": best["path"],
         "paths_ms_per_step": {"two_pass": two["total_ms"] / two["steps"], "fused": fused["total_ms"] / fused["steps"]},

@@ -6,7 +6,9 @@
 // hp_adjoint_step computes lambda_k and accumulates the gradient of the
 // Manning coefficient of every landcover class and of the rain multiplier:
 //
-//   recompute   the flux pass on U_k (the state planes hold U_k on entry)
+//   recompute   the flux pass on U_k (the state planes hold U_k on entry),
+//               unless the caller passes the flux saved by the forward run
+//               (flux_ready)
 //   k_adj_update  pointwise: reverse of rain + friction + update; writes the
 //               direct part of lambda_k, the cotangents of the flux
 //               accumulators (zero where the cell is not flux-listed) and
@@ -17,6 +19,11 @@
 //   k_adj_gather per cell: lambda_k += the contributions of the 8 edges that
 //               read it
 // Every write goes to the writing thread's own cell or edge slot: no atomics.
+//
+// hp_calib_forward / hp_calib_reverse run the step loops of a calibration
+// natively: the forward steps on the single-pass kernel (recording the time
+// schedule, or following it through per-step control blocks, and keeping
+// each segment's states and fluxes), and the reverse sweep over a segment.
 
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -36,7 +43,6 @@ constexpr int kGradStride = 16;  // doubles per block: classes 0..14, rain at 15
 constexpr int kARateCache = 64;  // station rates kept in LDS per block
 
 int grid_flux_launch(const hp_grid_desc* g, hipStream_t st);                   // swe_grid.hip
-int grid_update_launch(const hp_grid_desc* g, bool finalize, hipStream_t st);  // swe_grid.hip
 int grid_fused_launch(const hp_grid_desc* g, bool save_flux, bool finalize, hipStream_t st);  // swe_grid.hip
 
 // Records (t, dt) of the control block as step k of a schedule [K+1][2].

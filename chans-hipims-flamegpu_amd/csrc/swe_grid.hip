@@ -944,7 +944,7 @@ static void launch_update_mode(const hp_grid_desc* g, int64_t nb, hipStream_t st
   else hipLaunchKernelGGL((k_update<double, MODE>), dim3((unsigned)nb), dim3(kThreads), 0, st, *g);
 }
 
-static int launch_update(const hp_grid_desc* g, int mode, hipStream_t st, bool finalize = true) {
+static int launch_update(const hp_grid_desc* g, int mode, hipStream_t st) {
   const int64_t nb = update_blocks(g);
   if (!g->work || g->work_bytes < partials_bytes(nb))
     return fail(HP_ESPACE, "workspace smaller than hp_grid_workspace_bytes()");
@@ -957,7 +957,7 @@ static int launch_update(const hp_grid_desc* g, int mode, hipStream_t st, bool f
   }
   int rc = check_launch("k_update");
   if (rc) return rc;
-  if (finalize && (mode == HP_UPDATE_STEP || mode == HP_UPDATE_CFL)) {
+  if (mode == HP_UPDATE_STEP || mode == HP_UPDATE_CFL) {
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, st, g->ctrl, g->work, (int)nb, g->dx,
                        g->mass_sums);
     return check_launch("k_finalize");
@@ -1032,14 +1032,6 @@ template <class T>
 __global__ void k_scatter(T* __restrict__ plane, const int64_t* __restrict__ off, int64_t n, const T* __restrict__ in) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) plane[off[i]] = in[i];
-}
-
-// Update pass of a step (rain, friction, h_max) with or without the time
-// control pass, shared with the calibration runs (frozen schedules skip it).
-int grid_update_launch(const hp_grid_desc* g, bool finalize, hipStream_t st) {
-  int rc = check_desc(g);
-  if (rc) return rc;
-  return launch_update(g, HP_UPDATE_STEP, st, finalize);
 }
 
 // Single-pass step (h, qx, qy -> h_out, qx_out, qy_out; with save_flux the

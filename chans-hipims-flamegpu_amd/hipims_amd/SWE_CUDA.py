@@ -287,6 +287,23 @@ class Godunov:
     def _h_max(self):
         return self.solver.compact(self.solver.h_max)
 
+    def raster_update(self):
+        """The compact state into raster tensors h, wl, z, qx, qy at the cells
+        of `internal` (SWE_CUDA.py:918-923).  A caller that has set these
+        rasters (and `internal`) gets them updated in place, as with the
+        reference; missing ones are created with zeros outside the valid
+        cells."""
+        internal = getattr(self, "internal", None)
+        if internal is None:
+            internal = torch.as_tensor(self._mask > 0, device=self._device)
+        for name, val in (("h", self._h_internal), ("wl", self._wl_internal), ("z", self._z_internal),
+                          ("qx", self._qx_internal), ("qy", self._qy_internal)):
+            r = getattr(self, name, None)
+            if r is None:
+                r = torch.zeros(self._mask.shape, dtype=val.dtype, device=val.device)
+                setattr(self, name, r)
+            r[internal] = val.to(dtype=r.dtype, device=r.device)
+
     def observeGauges_write(self, observe_index, dataStoreList, n):
         if n % 10 == 0:
             row = [self.t.item()]

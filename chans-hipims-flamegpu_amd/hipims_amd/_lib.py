@@ -107,6 +107,8 @@ SIGNATURES = {
     "hp_calib_forward": (_I, [ctypes.POINTER(HpGridDesc), _P, _P, _I32, _I32, _P, _P, _I64, _P, _I32, _P, _P]),
     "hp_calib_reverse": (_I, [ctypes.POINTER(HpGridDesc), ctypes.POINTER(HpAdjointDesc), _P, _I32, _I32, _I32, _P,
                               _P, _I64, _P, _I32, _P, _P, _P]),
+    "hp_probe_fma_per_lane": (_I64, []),
+    "hp_probe_fma": (_I, [_I32, _P, _P]),
     "hp_gather": (_I, [_I, _P, _P, _I64, _P, _P]),
     "hp_scatter": (_I, [_I, _P, _P, _I64, _P, _P]),
 }

@@ -313,6 +313,13 @@ int hp_calib_reverse(const hp_grid_desc* g, const hp_adjoint_desc* a, const hp_c
                      const int64_t* gauge_off, int32_t n_gauges, const double* h_rec, const double* obs,
                      void* stream);
 
+/* ---- VALU issue-rate probe (no reference counterpart): `blocks` blocks of
+ * 256 lanes each run hp_probe_fma_per_lane() dependent-free FP32 FMAs and
+ * write one float to out[block * 256 + lane]; timing a launch calibrates the
+ * vector-instruction roofline of the flux kernels. */
+int64_t hp_probe_fma_per_lane(void);
+int hp_probe_fma(int32_t blocks, float* out, void* stream);
+
 /* out[i] = plane[offsets[i]]  (compact gather, Godunov.get_h). */
 int hp_gather(int dtype, const void* plane, const int64_t* offsets, int64_t n, void* out, void* stream);
 /* plane[offsets[i]] = in[i]  (compact scatter). */

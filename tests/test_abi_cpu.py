@@ -57,6 +57,8 @@ def test_host_side_argument_checks():
     assert L.hp_gather(_lib.HP_F64, None, None, 0, None, None) == _lib.HP_OK
     assert L.hp_station_rain(_lib.HP_F64, -1, None, None, None, 1, None, None) == _lib.HP_EINVAL
     assert L.hp_grid_workspace_bytes(None) == 0
+    assert L.hp_probe_fma(0, None, None) == _lib.HP_EINVAL
+    assert L.hp_probe_fma_per_lane() == 8 * 4096
 
 
 def test_calibration_loop_argument_checks():

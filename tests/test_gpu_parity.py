@@ -284,6 +284,14 @@ def test_godunov_dropin_server_sequence(oracle_lib, cuda, tmp_path):
     names = sorted(os.listdir(tmp_path))
     assert any(n.startswith("h_tensor([120.") for n in names), names
     assert {n.split("_tensor")[0] for n in names} == {"h", "qx", "qy", "wl", "h_max"}
+    # raster_update (SWE_CUDA.py:918-923): rasters at the valid cells
+    num.raster_update()
+    valid = c.mask > 0
+    assert np.array_equal(num.h.cpu().numpy()[valid], num.get_h().cpu().numpy())
+    assert rel_err(num.h.cpu().numpy()[valid], o.h_c) <= 1e-9
+    assert np.allclose(num.wl.cpu().numpy()[valid], num.get_h().cpu().numpy() + c.z[valid], rtol=0, atol=1e-12)
+    assert np.array_equal(num.z.cpu().numpy()[valid], c.z[valid])
+    assert np.all(num.h.cpu().numpy()[~valid] == 0)
 
 
 def _fp32_errors(h, qx, qy, o64):
