@@ -400,7 +400,10 @@ def adjoint_mode(a):
     0.8 or 1.2 (seeded); gradients w.r.t. the 6 class n and the rain
     multiplier.  Cost ratio = (forward with checkpoints + reverse with
     recomputation) / plain forward of the same steps; the gradient is checked
-    against central differences (eps = 1e-3 relative) on the frozen schedule."""
+    against central differences (eps = 1e-3 relative) on the frozen schedule,
+    and an FP64 run of the same calibration gives the reference gradient
+    (checked against its own central differences).  Relative errors are
+    max |difference| / max |gradient| over the seven parameters."""
     import numpy as np
     import torch
 
@@ -496,6 +499,31 @@ def adjoint_mode(a):
         g = grad.numpy()
         line["fd"] = fd.tolist()
         line["fd_max_rel_err"] = float(np.abs(g - fd).max() / np.abs(g).max())
+        # FP64 reference: the same gauges, observations and loss on the FP64
+        # solver (its own recorded schedule), its adjoint gradient against its
+        # own central differences, and the FP32 gradient against it
+        s64 = GridSolver(c.mask, c.h, c.qx, c.qy, c.z, c.dx, cfl=c.cfl, order=a.adj_order, dtype=torch.float64,
+                         device=dev, first_dt=c.first_dt, manning=c.manning, landuse=c.landuse)
+        s64.set_station_rain(c.rain_table)
+        cal64 = Calibration(s64, gauges, a.adj_steps, a.adj_checkpoint, obs=cal.obs, slopes=a.adj_slopes)
+        cal64.run(th0)
+        g64 = cal64.gradient().numpy()
+        line["fp64"] = {"grad": g64.tolist(), "fp32_vs_fp64_max_rel": float(np.abs(g - g64).max() / np.abs(g64).max())}
+        # eps 1e-3 as for FP32; eps 1e-6 keeps the perturbed runs on the same
+        # side of the wet/dry and limiter switches, where the discrete model
+        # is differentiable
+        for eps in (1e-3, 1e-6):
+            fd64 = []
+            for j in range(th0.numel()):
+                d = eps * abs(float(th0[j]))
+                tp, tm = th0.clone(), th0.clone()
+                tp[j] += d
+                tm[j] -= d
+                fd64.append((cal64.run(tp).item() - cal64.run(tm).item()) / (2 * d))
+            fd64 = np.array(fd64)
+            line["fp64"][f"fd_eps{eps:g}"] = {
+                "fd": fd64.tolist(), "max_rel_err": float(np.abs(g64 - fd64).max() / np.abs(g64).max()),
+                "rel_err_per_param": (np.abs(g64 - fd64) / np.abs(fd64)).tolist()}
     print(json.dumps(line), flush=True)
 
 

@@ -53,6 +53,19 @@ __global__ void k_schedule_save(const hp_ctrl* ctrl, double* sched, int k) {
   }
 }
 
+// End of a calibration step: the gauge depths into their history row and,
+// when recording, (t, dt) of the next step (sched may be null).
+template <class T>
+__global__ void k_step_record(const T* __restrict__ h, const int64_t* __restrict__ off, int n, T* __restrict__ row,
+                              const hp_ctrl* ctrl, double* sched, int k_next) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) row[i] = h[off[i]];
+  if (sched && i == 0) {
+    sched[2 * (int64_t)k_next] = ctrl->t;
+    sched[2 * (int64_t)k_next + 1] = ctrl->dt;
+  }
+}
+
 template <class T> struct AdjPlanes {
   T *lh, *lqx, *lqy;
   T *ch, *cqx, *cqy;
@@ -539,15 +552,16 @@ int hp_calib_forward(const hp_grid_desc* g, double* sched, const hp_ctrl* frozen
     seg_planes(0, s0);
     if ((rc = copy3(s0, own))) return rc;
   }
+  if (!frozen && sched && nsteps > 0) {  // (t, dt) of the first step; the others follow each step
+    hipLaunchKernelGGL(k_schedule_save, dim3(1), dim3(64), 0, st, g->ctrl, sched, k0);
+    if ((rc = check_launch("k_schedule_save"))) return rc;
+  }
+  const bool gauges = hist && n_gauges > 0;
+  const int64_t esz = g->dtype == HP_F32 ? 4 : 8;
   for (int32_t j = 0; j < nsteps; ++j) {
     const int32_t k = k0 + j;
     hp_grid_desc d = *g;
-    if (frozen) {
-      d.ctrl = const_cast<hp_ctrl*>(frozen) + k;  // read only: no time control pass runs
-    } else if (sched) {
-      hipLaunchKernelGGL(k_schedule_save, dim3(1), dim3(64), 0, st, g->ctrl, sched, k);
-      if ((rc = check_launch("k_schedule_save"))) return rc;
-    }
+    if (frozen) d.ctrl = const_cast<hp_ctrl*>(frozen) + k;  // read only: no time control pass runs
     void *in[3], *out[3];
     if (seg_u) {
       // state j in the segment, state j + 1 into the next slot (the caller's
@@ -568,9 +582,18 @@ int hp_calib_forward(const hp_grid_desc* g, double* sched, const hp_ctrl* frozen
     d.h = in[0]; d.qx = in[1]; d.qy = in[2];
     d.h_out = out[0]; d.qx_out = out[1]; d.qy_out = out[2];
     if ((rc = grid_fused_launch(&d, seg_u != nullptr, frozen == nullptr, st))) return rc;
-    if (hist && n_gauges > 0) {
-      char* row = (char*)hist + (int64_t)(k + 1) * n_gauges * (g->dtype == HP_F32 ? 4 : 8);
-      if ((rc = hp_gather(g->dtype, out[0], gauge_off, n_gauges, row, stream))) return rc;
+    double* save = (!frozen && sched && j + 1 < nsteps) ? sched : nullptr;
+    if (gauges || save) {
+      const int n = gauges ? n_gauges : 0;
+      const unsigned nb = (unsigned)((n > 0 ? n : 1) + 255) / 256;
+      char* row = gauges ? (char*)hist + (int64_t)(k + 1) * n_gauges * esz : nullptr;
+      if (g->dtype == HP_F32)
+        hipLaunchKernelGGL(k_step_record<float>, dim3(nb), dim3(256), 0, st, (const float*)out[0], gauge_off, n,
+                           (float*)row, g->ctrl, save, k + 1);
+      else
+        hipLaunchKernelGGL(k_step_record<double>, dim3(nb), dim3(256), 0, st, (const double*)out[0], gauge_off, n,
+                           (double*)row, g->ctrl, save, k + 1);
+      if ((rc = check_launch("k_step_record"))) return rc;
     }
   }
   if (!seg_u && (nsteps & 1)) return copy3(own, alt);  // the result in the caller's planes
