@@ -418,3 +418,91 @@ def test_autotune_restores_state_and_tile_height_is_neutral(cuda, path):
             ref = cur
         for a, b in zip(ref, cur):
             assert torch.equal(a, b), ty
+
+
+def _godunov(c, cuda, order, first_dt):
+    from hipims_amd.SWE_CUDA import Godunov
+    num = Godunov(cuda, c.dx, c.cfl, 1.0e9, firstTimeStep=first_dt, secondOrder=order == 2,
+                  tensorType=torch.float64)
+    T = lambda a: torch.as_tensor(np.asarray(a))
+    num.init__fluidField_tensor(T(c.mask), T(c.h), T(c.qx), T(c.qy), T(c.h + c.z), T(c.z), cuda)
+    num.set__frictionField_tensor(c.manning, cuda)
+    num.set_landuse(T(c.mask), T(c.landuse), cuda)
+    return num
+
+
+def test_dropin_rk2_uniform_rain_matches_oracle(oracle_lib, cuda):
+    """rungeKutta_update (SWE_CUDA.py:870-892): two stages of flux, uniform
+    rain (SWE_CUDA.py:520-550) and friction/update on the same t and dt,
+    averaged with the state before; against the oracle's ops composed the
+    same way."""
+    say("[Godunov rungeKutta_update]")
+    orc = oracle_lib
+    c = gen.rainfall_runoff(40, 5.0, rain_mmh=300.0, relief=3.0)
+    table = np.array([[0.0, 2e-5], [200.0, 6e-5], [400.0, 0.0]])
+    num = _godunov(c, cuda, 2, 0.5)
+    for _ in range(10):  # a developed state from ordinary server steps
+        num.addFlux()
+        num.add_uniform_PrecipitationSource(table, cuda)
+        num.time_friction_euler_update_cuda(cuda)
+    h, qx, qy = (v.cpu().numpy().copy() for v in (num.get_h(), num.get_qx(), num.get_qy()))
+    z, lu = compact(c.mask, c.z), compact(c.mask, c.landuse, np.uint8)
+    idx = orc.build_index(c.mask)
+    t, dt = num.t.item(), num.dt.item()
+    i = int(np.searchsorted(table[:, 0], t, side="right")) - 1
+    per = (t - table[i, 0]) / (table[i + 1, 0] - table[i, 0])
+    rate = table[i, 1] + per * (table[i + 1, 1] - table[i, 1])
+    for _ in range(3):
+        h0, qx0, qy0 = h.copy(), qx.copy(), qy.copy()
+        for _stage in range(2):
+            dh, dqx, dqy, dz = (np.zeros(h.size) for _ in range(4))
+            wl = h + z
+            orc.flux(2, orc.flux_list(h, idx, t), dh, dqx, dqy, h, wl, z, qx, qy, idx, [[0.0, 0.0]], [[0.0, 0.0]],
+                     [[0.0, 0.0, 0.0]], c.dx, t, dt)
+            dh += rate * dt
+            orc.friction_update(orc.update_list(dh, h), dh, dqx, dqy, dz, lu, h, wl, qx, qy, z, c.manning, dt)
+        h, qx, qy = (h + h0) / 2.0, (qx + qx0) / 2.0, (qy + qy0) / 2.0
+        num.rungeKutta_update(table, cuda)
+    torch.cuda.synchronize()
+    errs = [rel_err(num.get_h().cpu().numpy(), h), rel_err(num.get_qx().cpu().numpy(), qx),
+            rel_err(num.get_qy().cpu().numpy(), qy)]
+    say(f"  t {t:.2f} s, dt {dt:.3f} s, rate {rate:.3e} m/s, rel err h/qx/qy {errs}")
+    assert max(errs) <= TOL64
+    assert num.t.item() == t and num.dt.item() == dt  # no time control in RK2
+
+
+def test_dropin_radar_rain_matches_per_cell_stations(cuda):
+    """addRadarRainfall (SWE_CUDA.py:556-557) adds rain * dt per cell; with
+    every cell its own station of the same constant rate the station path
+    (pinned against the oracle) must give the same steps bit for bit.  The
+    rates are exact in float32 (the field's storage type)."""
+    say("[Godunov addRadarRainfall]")
+    n = 20
+    mask, H, QX, QY, Z = box(n, n, wall=40)
+    rng = np.random.Generator(np.random.PCG64(5))
+    z = 0.2 * rng.standard_normal((n, n))
+    Z = gen.pad(z)
+    c = gen.Case("radar", mask, H, QX, QY, Z, 2.0, np.zeros_like(mask, dtype=np.int64), np.array([0.03]), 0.5, 0.5)
+    valid = mask > 0
+    nv = int(valid.sum())
+    rates = ((np.arange(nv) % 7) + 1) * 2.0 ** -15
+    st = np.full(mask.shape, -1, dtype=np.int64)
+    st[valid] = np.arange(nv)
+    table = np.vstack([np.concatenate([[0.0], rates]), np.concatenate([[1.0e6], rates])])
+    a = _godunov(c, cuda, 2, 0.5)
+    b = _godunov(c, cuda, 2, 0.5)
+    T = lambda v: torch.as_tensor(np.asarray(v))
+    a.set_distributed_rainfall_station_Mask(T(mask), T(st), cuda)
+    field = torch.as_tensor(rates, device=cuda)
+    for _ in range(20):
+        a.addFlux()
+        a.addStation_PrecipitationSource(table, cuda)
+        a.time_friction_euler_update_cuda(cuda)
+        b.addFlux()
+        b.addRadarRainfall(field)
+        b.time_friction_euler_update_cuda(cuda)
+    torch.cuda.synchronize()
+    for get in ("get_h", "get_qx", "get_qy"):
+        assert torch.equal(getattr(a, get)(), getattr(b, get)()), get
+    assert a.t.item() == b.t.item() and a.dt.item() == b.dt.item()
+    say(f"  t {a.t.item():.2f} s, depth sum {a.get_h().sum().item():.4e} m")
