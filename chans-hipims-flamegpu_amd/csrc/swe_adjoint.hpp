@@ -57,27 +57,27 @@ __host__ __device__ inline bool solve_edge_vjp(const FrameCell<T>& c, const Fram
     eL = c;
     eR = r;
     if (ORDER == 1 && !r_is_ghost) {
-      T rb = minmod<T>(c.z - l.z, r.z - c.z);
+      T rb = Prec<T>::limit(c.z - l.z, r.z - c.z);
       if (r.h > hs) eL.z += T(0.5) * rb;
-      rb = minmod<T>(r.z - rr.z, c.z - r.z);
+      rb = Prec<T>::limit(r.z - rr.z, c.z - r.z);
       if (c.h > hs) eR.z += T(0.5) * rb;
     }
   } else {
-    eL.h = c.h + T(0.5) * minmod<T>(c.h - l.h, r.h - c.h);
-    eL.z = c.z + T(0.5) * minmod<T>(c.z - l.z, r.z - c.z);
-    eR.h = r.h + T(0.5) * minmod<T>(r.h - rr.h, c.h - r.h);
-    eR.z = r.z + T(0.5) * minmod<T>(r.z - rr.z, c.z - r.z);
+    eL.h = c.h + T(0.5) * Prec<T>::limit(c.h - l.h, r.h - c.h);
+    eL.z = c.z + T(0.5) * Prec<T>::limit(c.z - l.z, r.z - c.z);
+    eR.h = r.h + T(0.5) * Prec<T>::limit(r.h - rr.h, c.h - r.h);
+    eR.z = r.z + T(0.5) * Prec<T>::limit(r.z - rr.z, c.z - r.z);
     zeroL = c.h < hs;
     zeroR = r.h < hs;
     if (zeroL) { eL.qn = T(0); eL.qt = T(0); }
     else {
-      eL.qn = c.qn + T(0.5) * minmod<T>(c.qn - l.qn, r.qn - c.qn);
-      eL.qt = c.qt + T(0.5) * minmod<T>(c.qt - l.qt, r.qt - c.qt);
+      eL.qn = c.qn + T(0.5) * Prec<T>::limit(c.qn - l.qn, r.qn - c.qn);
+      eL.qt = c.qt + T(0.5) * Prec<T>::limit(c.qt - l.qt, r.qt - c.qt);
     }
     if (zeroR) { eR.qn = T(0); eR.qt = T(0); }
     else {
-      eR.qn = r.qn + T(0.5) * minmod<T>(r.qn - rr.qn, c.qn - r.qn);
-      eR.qt = r.qt + T(0.5) * minmod<T>(r.qt - rr.qt, c.qt - r.qt);
+      eR.qn = r.qn + T(0.5) * Prec<T>::limit(r.qn - rr.qn, c.qn - r.qn);
+      eR.qt = r.qt + T(0.5) * Prec<T>::limit(r.qt - rr.qt, c.qt - r.qt);
     }
   }
   // ---- forward: velocities and hydrostatic reconstruction ----
