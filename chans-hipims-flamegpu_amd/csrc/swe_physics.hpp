@@ -49,6 +49,7 @@ template <class T> struct Prec;
 template <> struct Prec<double> {
   static constexpr bool local_datum = false;
   __host__ __device__ static inline double limit(double x, double y) { return minmod<double>(x, y); }
+  __host__ __device__ static inline double clamp(double x, double lo, double hi) { return fmax(fmin(x, hi), lo); }
   __host__ __device__ static inline double div(double a, double b) { return a / b; }
   __host__ __device__ static inline double sqrt_(double x) { return __builtin_sqrt(x); }
   __host__ __device__ static inline double rcbrt_friction(double h) { return pow(h, -1.0 / 3.0); }
@@ -67,6 +68,8 @@ template <> struct Prec<float> {
   // and two selects (it differs from the product form only when x * y
   // underflows to zero)
   __device__ static inline float limit(float x, float y) { return __builtin_amdgcn_fmed3f(x, y, 0.0f); }
+  // clamp to [lo, hi] (lo <= hi) as one median-of-three
+  __device__ static inline float clamp(float x, float lo, float hi) { return __builtin_amdgcn_fmed3f(x, lo, hi); }
   __device__ static inline float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
   __device__ static inline float sqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
   // h^(-1/3) = exp2(-log2(h) / 3) with the hardware log2 / exp2 (h >= 1e-6)
@@ -195,8 +198,8 @@ __host__ __device__ inline void hllc_edge(FrameCell<T>& eL, FrameCell<T>& eR, T 
   T wL = eL.h + eL.z;
   T wR = eR.h + eR.z;
   const T emax = fmax(etaC, etaR), emin = fmin(etaC, etaR);
-  wL = fmax(fmin(wL, emax), emin);
-  wR = fmax(fmin(wR, emax), emin);
+  wL = P::clamp(wL, emin, emax);
+  wR = P::clamp(wR, emin, emax);
   if (ORDER == 1) {
     const T m = (wL + wR) / T(2);
     const bool avg = (etaC - etaR) * (wL - wR) < T(0);
@@ -265,8 +268,9 @@ __host__ __device__ inline EdgeOut<T> solve_edge(const FrameCell<T>& c, const Fr
     eL = c;
     eR = r;
     if (!r_is_ghost) {
-      const T zl = eL.z + T(0.5) * Prec<T>::limit(c.z - l.z, r.z - c.z);
-      const T zr = eR.z + T(0.5) * Prec<T>::limit(r.z - rr.z, c.z - r.z);
+      const T dz = r.z - c.z;
+      const T zl = eL.z + T(0.5) * Prec<T>::limit(c.z - l.z, dz);
+      const T zr = eR.z + T(0.5) * Prec<T>::limit(r.z - rr.z, -dz);
       eL.z = r.h > hs ? zl : eL.z;
       eR.z = c.h > hs ? zr : eR.z;
     }
@@ -275,14 +279,17 @@ __host__ __device__ inline EdgeOut<T> solve_edge(const FrameCell<T>& c, const Fr
       eL = c;
       eR = r;
     } else {
-      eL.h = c.h + T(0.5) * Prec<T>::limit(c.h - l.h, r.h - c.h);
-      eL.z = c.z + T(0.5) * Prec<T>::limit(c.z - l.z, r.z - c.z);
-      eR.h = r.h + T(0.5) * Prec<T>::limit(r.h - rr.h, c.h - r.h);
-      eR.z = r.z + T(0.5) * Prec<T>::limit(r.z - rr.z, c.z - r.z);
-      const T qnL = c.qn + T(0.5) * Prec<T>::limit(c.qn - l.qn, r.qn - c.qn);
-      const T qtL = c.qt + T(0.5) * Prec<T>::limit(c.qt - l.qt, r.qt - c.qt);
-      const T qnR = r.qn + T(0.5) * Prec<T>::limit(r.qn - rr.qn, c.qn - r.qn);
-      const T qtR = r.qt + T(0.5) * Prec<T>::limit(r.qt - rr.qt, c.qt - r.qt);
+      // r - c serves both sides: c - r is its exact negation, and the
+      // limiter gives zero whenever either argument is zero
+      const T dh = r.h - c.h, dz = r.z - c.z, dqn = r.qn - c.qn, dqt = r.qt - c.qt;
+      eL.h = c.h + T(0.5) * Prec<T>::limit(c.h - l.h, dh);
+      eL.z = c.z + T(0.5) * Prec<T>::limit(c.z - l.z, dz);
+      eR.h = r.h + T(0.5) * Prec<T>::limit(r.h - rr.h, -dh);
+      eR.z = r.z + T(0.5) * Prec<T>::limit(r.z - rr.z, -dz);
+      const T qnL = c.qn + T(0.5) * Prec<T>::limit(c.qn - l.qn, dqn);
+      const T qtL = c.qt + T(0.5) * Prec<T>::limit(c.qt - l.qt, dqt);
+      const T qnR = r.qn + T(0.5) * Prec<T>::limit(r.qn - rr.qn, -dqn);
+      const T qtR = r.qt + T(0.5) * Prec<T>::limit(r.qt - rr.qt, -dqt);
       eL.qn = c.h < hs ? T(0) : qnL;
       eL.qt = c.h < hs ? T(0) : qtL;
       eR.qn = r.h < hs ? T(0) : qnR;
