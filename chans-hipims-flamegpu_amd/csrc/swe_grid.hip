@@ -204,7 +204,8 @@ __device__ inline void solve_tile_edge(const FluxTile<T, TY>& s, const hp_grid_d
   constexpr T nA = T(D == 0 ? 1 : -1);  // A's outward normal component along the edge axis
   if (tA != 0 && tB != 0) {
     const T zref = Prec<T>::local_datum ? s.z[ra][ca] : T(0);
-    const FrameCell<T> c = frame_cell_d<D>(s, ra, ca, zref);
+    FrameCell<T> c = frame_cell_d<D>(s, ra, ca, zref);
+    if (Prec<T>::local_datum) c.z = T(0);  // its own datum: z - z is exactly 0, known here at compile time
     const FrameCell<T> r = frame_cell_d<D>(s, rb, cb, zref);
     const FrameCell<T> lc = frame_cell_d<D>(s, rl, cl, zref);
     const FrameCell<T> rc = frame_cell_d<D>(s, rr2, cr2, zref);
