@@ -452,13 +452,17 @@ __device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by, const Pa
     bool listed = false;
     if (s.ty[r][c] != 0) {
       // fluxMask rule: t == 0, wet, or a wet valid neighbour E, N, W, S that
-      // is not compact cell 0 (fluxMaskGenerator_Kernel.cu:36); evaluated
-      // without short-circuit so the four LDS reads issue together
-      const bool w_e = (s.ty[r][c + 1] != 0) & (s.h[r][c + 1] > hs) & !(gx + 1 == a.cell0_x && gy == a.cell0_y);
-      const bool w_n = (s.ty[r - 1][c] != 0) & (s.h[r - 1][c] > hs) & !(gx == a.cell0_x && gy - 1 == a.cell0_y);
-      const bool w_w = (s.ty[r][c - 1] != 0) & (s.h[r][c - 1] > hs) & !(gx - 1 == a.cell0_x && gy == a.cell0_y);
-      const bool w_s = (s.ty[r + 1][c] != 0) & (s.h[r + 1][c] > hs) & !(gx == a.cell0_x && gy + 1 == a.cell0_y);
-      listed = t_zero | (s.h[r][c] > hs) | w_e | w_n | w_w | w_s;
+      // is not compact cell 0 (fluxMaskGenerator_Kernel.cu:36); the
+      // neighbours are looked at only for a dry cell (the four LDS reads of
+      // that case issue together)
+      listed = t_zero | (s.h[r][c] > hs);
+      if (!listed) {
+        const bool w_e = (s.ty[r][c + 1] != 0) & (s.h[r][c + 1] > hs) & !(gx + 1 == a.cell0_x && gy == a.cell0_y);
+        const bool w_n = (s.ty[r - 1][c] != 0) & (s.h[r - 1][c] > hs) & !(gx == a.cell0_x && gy - 1 == a.cell0_y);
+        const bool w_w = (s.ty[r][c - 1] != 0) & (s.h[r][c - 1] > hs) & !(gx - 1 == a.cell0_x && gy == a.cell0_y);
+        const bool w_s = (s.ty[r + 1][c] != 0) & (s.h[r + 1][c] > hs) & !(gx == a.cell0_x && gy + 1 == a.cell0_y);
+        listed = w_e | w_n | w_w | w_s;
+      }
       if (listed) {
         const T* eE = s.e[ly * (kTX + 1) + lx + 1];         // this cell is A
         const T* eN = s.e[Tile::NX + ly * kTX + lx];        // this cell is B
