@@ -73,6 +73,9 @@ template <class T> struct AdjPlanes {
   T* yc;
   int64_t cpitch, cplane;
   int frozen_slopes;
+  const int32_t* gauge_of;  // null: no loss term in this pass
+  const double* loss_h;
+  const double* loss_obs;
 };
 
 template <class T>
@@ -135,6 +138,12 @@ __global__ __launch_bounds__(kAThreads) void k_adj_update(hp_grid_desc a, AdjPla
     const VecT<T> vh = ldv<VecT<T>>(h + i0), vqx = ldv<VecT<T>>(qx + i0), vqy = ldv<VecT<T>>(qy + i0);
     const VecT<T> vdh = ldv<VecT<T>>(dh + i0), vdqx = ldv<VecT<T>>(dqx + i0), vdqy = ldv<VecT<T>>(dqy + i0);
     VecT<T> lh = ldv<VecT<T>>(p.lh + i0), lqx = ldv<VecT<T>>(p.lqx + i0), lqy = ldv<VecT<T>>(p.lqy + i0);
+    if (p.gauge_of) {  // the loss cotangent of the depths after the step
+      const VecI32 vg = ldv<VecI32>(p.gauge_of + i0);
+#pragma unroll
+      for (int j = 0; j < kVec; ++j)
+        if (vg.x[j] >= 0) lh.x[j] += T(2.0 * (p.loss_h[vg.x[j]] - p.loss_obs[vg.x[j]]));
+    }
     VecD vg = ldv<VecD>(gn + i0);
     VecT<T> ch{}, cqx{}, cqy{}, rbv{}, rsc{};
     if (mode == HP_RAIN_STATION) {
@@ -427,6 +436,9 @@ static int launch_adjoint(const hp_grid_desc* g, const hp_adjoint_desc* ad, hipS
   p.cpitch = ad->cpitch;
   p.cplane = ad->cpitch * (int64_t)(g->ny + 1);
   p.frozen_slopes = ad->slope_mode == 1;
+  p.gauge_of = ad->gauge_of;
+  p.loss_h = ad->loss_h;
+  p.loss_obs = ad->loss_obs;
   const int64_t total = g->pitch * (int64_t)g->ny;
   const unsigned nbu = (unsigned)((total + kAChunk - 1) / kAChunk);
   hipLaunchKernelGGL((k_adj_update<T>), dim3(nbu), dim3(kAThreads), 0, st, *g, p, ad->grads, ad->gn);
@@ -471,6 +483,7 @@ int hp_adjoint_step(const hp_grid_desc* g, const hp_adjoint_desc* ad, void* stre
   if (g->n_classes > kGradStride - 1) return fail(HP_EINVAL, "the adjoint supports at most 15 Manning classes");
   if (g->infil_ks) return fail(HP_EINVAL, "the adjoint does not cover infiltration");
   if (ad->slope_mode != 0 && ad->slope_mode != 1) return fail(HP_EINVAL, "slope_mode must be 0 or 1");
+  if (ad->gauge_of && (!ad->loss_h || !ad->loss_obs)) return fail(HP_EINVAL, "gauge_of needs loss_h and loss_obs");
   if (g->ny + 1 > 65535) return fail(HP_EINVAL, "the adjoint launches one block row per grid row: ny < 65535");
   if (g->pitch * (int64_t)g->ny >= (int64_t(1) << 31)) return fail(HP_EINVAL, "the adjoint covers planes of fewer than 2^31 elements");
   if (!ad->flux_ready) {
@@ -610,9 +623,14 @@ int hp_calib_reverse(const hp_grid_desc* g, const hp_adjoint_desc* ad, const hp_
   const int64_t plane_bytes = g->pitch * (int64_t)g->ny * (g->dtype == HP_F32 ? 4 : 8);
   if (seg_stride < 3 * plane_bytes) return fail(HP_EINVAL, "seg_stride smaller than three planes");
   if (n_gauges > 0 && (!gauge_off || !h_rec || !obs)) return fail(HP_EINVAL, "the loss needs gauges, h_rec and obs");
+  hp_adjoint_desc a = *ad;
+  if (n_gauges <= 0) a.gauge_of = nullptr;
   for (int32_t k = k_hi - 1; k >= k_lo; --k) {
     int rc;
-    if (n_gauges > 0) {
+    if (a.gauge_of) {  // the loss enters in the step's own update pass
+      a.loss_h = h_rec + (int64_t)(k + 1) * n_gauges;
+      a.loss_obs = obs + (int64_t)(k + 1) * n_gauges;
+    } else if (n_gauges > 0) {
       rc = hp_adjoint_add_loss(g->dtype, ad->lam_h, gauge_off, h_rec + (int64_t)(k + 1) * n_gauges,
                                obs + (int64_t)(k + 1) * n_gauges, n_gauges, 1.0, stream);
       if (rc) return rc;
@@ -620,7 +638,7 @@ int hp_calib_reverse(const hp_grid_desc* g, const hp_adjoint_desc* ad, const hp_
     const int64_t off = (int64_t)(k - seg_k0) * seg_stride;
     hp_grid_desc d = with_segment(*g, (char*)seg_u + off, (char*)seg_d + off, plane_bytes);
     d.ctrl = const_cast<hp_ctrl*>(frozen) + k;  // t and dt of step k; read only
-    rc = hp_adjoint_step(&d, ad, stream);
+    rc = hp_adjoint_step(&d, &a, stream);
     if (rc) return rc;
   }
   return HP_OK;

@@ -19,6 +19,9 @@ struct alignas(kVec) VecU8 {
 struct alignas(2 * kVec) VecI16 {
   int16_t x[kVec];
 };
+struct alignas(4 * kVec) VecI32 {
+  int32_t x[kVec];
+};
 struct alignas(4 * kVec) VecF32 {
   float x[kVec];
 };
@@ -30,9 +33,9 @@ template <class V, class T> __device__ inline void stv(T* p, const V& v) { *rein
 
 // Station rain at time t_now (SWE_CUDA.py:472-495): linear in time between
 // the table rows that bracket t_now, 0 from the table's last row on, scaled
-// by rain_scale.  The bracketing rows are found once per block; each cell
-// then evaluates its own station's rate from the two rows (a few cached
-// loads), so no per-block staging array is needed.
+// by rain_scale.  The bracketing rows are found once per block; the passes
+// keep the first stations' rates in LDS and evaluate any further station's
+// rate per cell from the two rows.
 struct RainRows {
   const double* lo;  // station columns of the rows bracketing t_now
   const double* hi;

@@ -97,6 +97,14 @@ class Calibration:
         ad.gn = self.gn.data_ptr()
         ad.slope_mode = 1 if slopes == "frozen" else 0
         ad.flux_ready = 1
+        # gauge index per cell: the reverse sweep adds the loss cotangent in
+        # the adjoint's own update pass (one gauge per cell; otherwise a
+        # separate small kernel per step does it)
+        self.gauge_of = None
+        if torch.unique(self.gauge_off).numel() == self.G and self.G > 0:
+            self.gauge_of = torch.full(shp, -1, dtype=torch.int32, device=dev)
+            self.gauge_of.view(-1)[self.gauge_off] = torch.arange(self.G, dtype=torch.int32, device=dev)
+            ad.gauge_of = self.gauge_of.data_ptr()
         self.adesc = ad
         if self.C > _lib.GRAD_STRIDE - 1:
             raise ValueError("at most 15 Manning classes")

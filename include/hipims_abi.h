@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define HP_ABI_VERSION 6
+#define HP_ABI_VERSION 7
 
 enum hp_status {
   HP_OK = 0,
@@ -263,7 +263,12 @@ int hp_ensemble_step(const hp_ensemble* e, int32_t nsteps, void* stream);
  * exact derivative of the step; over long horizons its frozen-branch
  * linearisation grows, as that of MUSCL with forward Euler does); 1 holds
  * the limited slopes fixed (the derivative of the first-order flux at the
- * reconstructed states, stable over long horizons, an approximation). */
+ * reconstructed states, stable over long horizons, an approximation).
+ * gauge_of (may be null): an int32 plane like h holding, at each gauge cell,
+ * the gauge's index g (-1 elsewhere; each cell at most one gauge); with it
+ * the step first adds the squared-misfit cotangent 2 (loss_h[g] -
+ * loss_obs[g]) (FP64 rows of the depths after the step and the observations)
+ * to lam_h at the gauge cells, in the same pass. */
 typedef struct hp_adjoint_desc {
   void *lam_h, *lam_qx, *lam_qy;
   void *cot_h, *cot_qx, *cot_qy;
@@ -273,6 +278,9 @@ typedef struct hp_adjoint_desc {
   double* gn;
   int32_t slope_mode;
   int32_t flux_ready;
+  const int32_t* gauge_of;
+  const double* loss_h;
+  const double* loss_obs;
 } hp_adjoint_desc;
 
 int64_t hp_adjoint_grad_blocks(const hp_grid_desc* g);
@@ -304,9 +312,10 @@ int hp_calib_forward(const hp_grid_desc* g, double* sched, const hp_ctrl* frozen
                      void* seg_u, void* seg_d, int64_t seg_stride, const int64_t* gauge_off, int32_t n_gauges,
                      void* hist, void* stream);
 /* hp_calib_reverse runs the reverse sweep over steps k_hi-1 down to k_lo:
- * per step the loss cotangent of the depths after it (hp_adjoint_add_loss
- * with row k+1 of h_rec and obs, FP64 [K+1][n_gauges]) and hp_adjoint_step
- * with the control block frozen[k] on the state and flux saved at
+ * per step the loss cotangent of the depths after it (row k+1 of h_rec and
+ * obs, FP64 [K+1][n_gauges]; inside the step's update pass when a->gauge_of
+ * is set, else through hp_adjoint_add_loss) and hp_adjoint_step with the
+ * control block frozen[k] on the state and flux saved at
  * seg_u / seg_d + (k - seg_k0)*seg_stride bytes (flux_ready must be 1). */
 int hp_calib_reverse(const hp_grid_desc* g, const hp_adjoint_desc* a, const hp_ctrl* frozen, int32_t k_lo,
                      int32_t k_hi, int32_t seg_k0, void* seg_u, void* seg_d, int64_t seg_stride,
