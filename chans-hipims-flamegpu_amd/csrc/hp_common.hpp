@@ -1,5 +1,6 @@
-// Error reporting shared by every C-ABI entry point: a negative status code
-// plus a per-thread message readable through hp_last_error().
+// Error reporting shared by every C-ABI entry point (a negative status code
+// plus a per-thread message readable through hp_last_error()) and the tile
+// order shared by the tiled kernels.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -21,6 +22,17 @@ inline int check_launch(const char* what) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(HP_ELAUNCH, std::string(what) + ": " + hipGetErrorString(e));
   return HP_OK;
+}
+
+// Workgroups of a launch are handed to the 8 XCDs round-robin (block b runs
+// on XCD b % 8), and each XCD has its own L2.  Tile numbers are reassigned so
+// that every XCD walks one contiguous band of tiles: x- and y-neighbours then
+// share their halo lines in one L2 instead of fetching them from HBM twice.
+constexpr int kXcds = 8;
+__device__ inline int xcd_band(int b, int n) {
+  const int q = n / kXcds, rem = n - q * kXcds;
+  const int x = b % kXcds, local = b / kXcds;
+  return x * q + min(x, rem) + local;
 }
 
 }  // namespace hp

@@ -97,6 +97,38 @@ __device__ inline bool flux_listed(const hp_grid_desc& a, const T* h, int x, int
   return false;
 }
 
+// Reverse of rain + friction + update for one plane element.  On entry
+// (lh, lqx, lqy) hold the cotangent of the element's state after the step, on
+// return that of its state before the step (the update adds the flux
+// increments and the rain to the state, so the same cotangent reaches both).
+// Returns whether the element was updated: only then is its cotangent also
+// that of its flux increments (when the flux pass listed it).  The Manning
+// derivative is added to gn, the rain-multiplier term to rsum.
+template <class T>
+__device__ inline bool adj_update_cell(int cls, T h0, T qx, T qy, T dh, T dqx, T dqy, T rb, T rs, T dt,
+                                       const T* s_man, T& lh, T& lqx, T& lqy, double& gn, double& rsum) {
+  if (cls == 255) {
+    lh = lqx = lqy = T(0);
+    return false;
+  }
+  const T rdt = rs * dt;
+  const T dht = dh + rdt;
+  if (!(fabs(dht) > T(0) || h0 >= T(0))) return false;  // not updated: lambda passes through
+  const T h1 = h0 + dht;
+  T a_h1 = lh, a_qsx = T(0), a_qsy = T(0);
+  if (!(h1 < T(kHs))) {
+    T a_hf = T(0), a_n = T(0);
+    friction_vjp<T>(h1, s_man[cls], dt, qx + dqx, qy + dqy, lqx, lqy, a_qsx, a_qsy, a_hf, a_n);
+    a_h1 += a_hf;
+    gn += (double)a_n;
+  }
+  rsum += (double)a_h1 * (double)rb * (double)dt;
+  lh = a_h1;
+  lqx = a_qsx;
+  lqy = a_qsy;
+  return true;
+}
+
 // Reverse of rain + friction + update, kVec consecutive elements per thread
 // per access.  The derivative w.r.t. the Manning value of a cell's class goes
 // into the cell's FP64 running sum gn (folded per class once, by
@@ -111,7 +143,6 @@ __global__ __launch_bounds__(kAThreads) void k_adj_update(hp_grid_desc a, AdjPla
   const double t_now = a.ctrl->t;
   const T dt = T(a.ctrl->dt);
   const T t_T = T(t_now);
-  const T hs = T(kHs);
   const int mode = a.rain_mode;
   for (int k = tid; k < a.n_classes; k += kAThreads) s_man[k] = ((const T*)a.manning)[k];
   const RainRows rr = rain_rows(a, t_now);
@@ -166,34 +197,15 @@ __global__ __launch_bounds__(kAThreads) void k_adj_update(hp_grid_desc a, AdjPla
     }
 #pragma unroll
     for (int j = 0; j < kVec; ++j) {
-      const int cls = vl.x[j];
-      if (cls == 255) {
-        lh.x[j] = lqx.x[j] = lqy.x[j] = T(0);
+      if (!adj_update_cell<T>(vl.x[j], vh.x[j], vqx.x[j], vqy.x[j], vdh.x[j], vdqx.x[j], vdqy.x[j], rbv.x[j], rsc.x[j],
+                              dt, s_man, lh.x[j], lqx.x[j], lqy.x[j], vg.x[j], rsum))
         continue;
-      }
-      const T h0 = vh.x[j];
-      const T rdt = rsc.x[j] * dt;
-      const T dht = vdh.x[j] + rdt;
-      if (!(fabs(dht) > T(0) || h0 >= T(0))) continue;  // not updated: lambda passes through
-      const T h1 = h0 + dht;
-      T a_h1 = lh.x[j], a_qsx = T(0), a_qsy = T(0);
-      if (!(h1 < hs)) {
-        T a_hf = T(0), a_n = T(0);
-        friction_vjp<T>(h1, s_man[cls], dt, vqx.x[j] + vdqx.x[j], vqy.x[j] + vdqy.x[j], lqx.x[j], lqy.x[j], a_qsx,
-                        a_qsy, a_hf, a_n);
-        a_h1 += a_hf;
-        vg.x[j] += (double)a_n;
-      }
-      rsum += (double)a_h1 * (double)rbv.x[j] * (double)dt;
-      lh.x[j] = a_h1;
-      lqx.x[j] = a_qsx;
-      lqy.x[j] = a_qsy;
       // the kVec elements share a row (pitch is a multiple of kVec)
       const int y = (int)((unsigned)i0 / pitch32), x = (int)((unsigned)i0 - (unsigned)y * pitch32) + j;
       if (flux_listed<T>(a, h, x, y, t_T)) {
-        ch.x[j] = a_h1;
-        cqx.x[j] = a_qsx;
-        cqy.x[j] = a_qsy;
+        ch.x[j] = lh.x[j];
+        cqx.x[j] = lqx.x[j];
+        cqy.x[j] = lqy.x[j];
       }
     }
     stv(p.lh + i0, lh);
@@ -273,18 +285,19 @@ __device__ inline void to_global_adj(const FrameAdj<T>& f, int d, T& ax, T& ay) 
   }
 }
 
-// One edge per thread.  Edges [0, nxe) are x-edges (row y, between columns
-// j-1 = A and j = B, j in [0, nx]); the rest are y-edges (column x, between
-// rows i-1 = A and i = B, i in [0, ny]).  Writes 12 cotangents per edge.
-template <class T, int ORDER, int DIR>
-__global__ __launch_bounds__(kAThreads) void k_adj_edges(hp_grid_desc a, AdjPlanes<T> p) {
-  // DIR 0: x-edges, grid (ceil((nx+1)/256), ny); DIR 3: y-edges, grid
-  // (ceil(nx/256), ny+1).  The direction is a compile-time constant, so the
-  // frame rotations of the interior edges fold away.
+// Adjoint of one edge.  DIR 0: the x-edge between columns col - 1 (A) and
+// col (B) of row `row`; DIR 3: the y-edge between rows row - 1 (A) and row
+// (B) of column `col`.  cot(x, y, ch, cqx, cqy) yields the cotangent of a
+// valid cell's flux increments; it is asked only for A and B.  out[role]
+// receives the cotangents (h, qx, qy) of the cells in roles l, A, B, rr (the
+// cell before A, A, B, the cell after B); order 1 fills only A and B.
+template <class T, int ORDER, int DIR, class Cot>
+__device__ inline void edge_adj(const hp_grid_desc& a, bool frozen_slopes, int col, int row, Cot cot,
+                                T (&out)[4][3]) {
+  // The direction is a compile-time constant, so the frame rotations of the
+  // interior edges fold away.
   const int nx = a.nx, ny = a.ny;
-  const int col = blockIdx.x * kAThreads + threadIdx.x, row = blockIdx.y;
   constexpr bool xe = DIR == 0;
-  if (col >= (xe ? nx + 1 : nx) || row >= (xe ? ny : ny + 1)) return;
   int yA, xA, yB, xB, yl, xl, yr2, xr2;
   constexpr int dAB = DIR;
   if (xe) {
@@ -294,8 +307,6 @@ __global__ __launch_bounds__(kAThreads) void k_adj_edges(hp_grid_desc a, AdjPlan
     xA = xB = xl = xr2 = col;
     yA = row - 1; yB = row; yl = row - 2; yr2 = row + 1;
   }
-  const int64_t slot = (int64_t)row * p.cpitch + col;
-  T* out = xe ? p.xc : p.yc;
   FrameAdj<T> adj[4] = {zero_adj<T>(), zero_adj<T>(), zero_adj<T>(), zero_adj<T>()};  // l, A, B, rr
   int dir[4] = {dAB, dAB, dAB, dAB};
   const int64_t P = a.pitch;
@@ -309,10 +320,6 @@ __global__ __launch_bounds__(kAThreads) void k_adj_edges(hp_grid_desc a, AdjPlan
     const T t = T(a.ctrl->t);
     const int dBA = (dAB + 2) & 3;
     const bool axis_x = (dAB == 0 || dAB == 2);
-    auto cot = [&](int x, int y, T& ch, T& cx, T& cy) {
-      const int64_t i = (int64_t)y * P + x;
-      ch = p.ch[i]; cx = p.cqx[i]; cy = p.cqy[i];
-    };
     if (vA && vB) {
       const int64_t iA = (int64_t)yA * P + xA, iB = (int64_t)yB * P + xB;
       const T zref = Prec<T>::local_datum ? ((const T*)a.z)[iA] : T(0);
@@ -332,7 +339,7 @@ __global__ __launch_bounds__(kAThreads) void k_adj_edges(hp_grid_desc a, AdjPlan
       const T v_br = (axis_x ? cBx : cBy) * nA;
       T a_ec = T(0), a_er = T(0);
       solve_edge_vjp<T, ORDER>(c, l, r, rr, false, c.h + c.z, r.h + r.z, dx, dt, v_fh, v_fn, v_ft, v_bc, v_br,
-                               adj[1], adj[0], adj[2], adj[3], a_ec, a_er, p.frozen_slopes != 0);
+                               adj[1], adj[0], adj[2], adj[3], a_ec, a_er, frozen_slopes);
       adj[1].h += a_ec;
       adj[2].h += a_er;
       if (!vl) { adj[1].h += adj[0].h; adj[1].qn += adj[0].qn; adj[1].qt += adj[0].qt; adj[0] = zero_adj<T>(); }
@@ -380,16 +387,41 @@ __global__ __launch_bounds__(kAThreads) void k_adj_edges(hp_grid_desc a, AdjPlan
       dir[from_a ? 1 : 2] = d;
     }
   }
+#pragma unroll
+  for (int role = 0; role < 4; ++role) {
+    out[role][0] = adj[role].h;
+    to_global_adj<T>(adj[role], dir[role], out[role][1], out[role][2]);
+  }
+}
+
+// One edge per thread.  Edges [0, nxe) are x-edges (row y, between columns
+// j-1 = A and j = B, j in [0, nx]); the rest are y-edges (column x, between
+// rows i-1 = A and i = B, i in [0, ny]).  Writes 12 cotangents per edge.
+template <class T, int ORDER, int DIR>
+__global__ __launch_bounds__(kAThreads) void k_adj_edges(hp_grid_desc a, AdjPlanes<T> p) {
+  // DIR 0: x-edges, grid (ceil((nx+1)/256), ny); DIR 3: y-edges, grid
+  // (ceil(nx/256), ny+1).
+  const int col = blockIdx.x * kAThreads + threadIdx.x, row = blockIdx.y;
+  constexpr bool xe = DIR == 0;
+  if (col >= (xe ? a.nx + 1 : a.nx) || row >= (xe ? a.ny : a.ny + 1)) return;
+  const int64_t P = a.pitch;
+  T out[4][3];
+  edge_adj<T, ORDER, DIR>(a, p.frozen_slopes != 0, col, row,
+                          [&](int x, int y, T& ch, T& cx, T& cy) {
+                            const int64_t i = (int64_t)y * P + x;
+                            ch = p.ch[i]; cx = p.cqx[i]; cy = p.cqy[i];
+                          },
+                          out);
   // order 1 reads no cell beyond the edge's two (the bed reconstruction's
   // neighbours enter through the static bed only): roles l and rr carry
   // nothing and are neither written nor gathered
+  const int64_t slot = (int64_t)row * p.cpitch + col;
+  T* dst = xe ? p.xc : p.yc;
   for (int role = ORDER == 1 ? 1 : 0; role < (ORDER == 1 ? 3 : 4); ++role) {
-    T gx, gy;
-    to_global_adj<T>(adj[role], dir[role], gx, gy);
-    T* base = out + (int64_t)(role * 3) * p.cplane + slot;
-    base[0] = adj[role].h;
-    base[p.cplane] = gx;
-    base[2 * p.cplane] = gy;
+    T* base = dst + (int64_t)(role * 3) * p.cplane + slot;
+    base[0] = out[role][0];
+    base[p.cplane] = out[role][1];
+    base[2 * p.cplane] = out[role][2];
   }
 }
 
@@ -418,6 +450,225 @@ __global__ __launch_bounds__(kAThreads) void k_adj_gather(hp_grid_desc a, AdjPla
   p.lh[i] += sh;
   p.lqx[i] += sx;
   p.lqy[i] += sy;
+}
+
+// Single-pass reverse step of the order-1 scheme.  A block owns a tile of
+// kATX x TY cells and, in one pass: (1) runs the update adjoint of its cells
+// and of the ring of cells that share an edge with them, keeping the ring's
+// flux-increment cotangents in LDS beside its own; (2) solves the adjoint of
+// every edge of its cells (an edge on the tile boundary is solved by both
+// tiles that touch it): the thread that owns a cell solves the cell's west
+// and south edges, keeps their cotangents of its own cell in registers and
+// leaves those of the west and south neighbours in LDS, where the owners of
+// the tile's east column and north row also leave those of the tile's last
+// edges; (3) adds to each cell's pass-through cotangent those of its four
+// edges, in the order k_adj_gather uses.  The cotangent after the step is
+// read from `in` and the one before it written to `out`: the ring cells
+// belong to the neighbouring tiles, so the two sets of planes must differ.
+constexpr int kATX = 64;
+template <class T, int TY> struct AdjTileLds {
+  T cot[3][TY + 2][kATX + 2];  // flux-increment cotangents; row/column 0 and last are the ring
+  T east[3][TY][kATX];   // cotangent of each cell from its east edge (the cell is that edge's A)
+  T north[3][TY][kATX];  // ... from its north edge
+};
+template <class T> struct AdjLam {
+  const T* in[3];
+  T* out[3];
+};
+template <class T> constexpr int adj_tile_rows() { return sizeof(T) == 4 ? 8 : 4; }
+
+template <class T, int TY>
+__global__ __launch_bounds__(kAThreads) void k_adj_step_tile(hp_grid_desc a, AdjPlanes<T> p, AdjLam<T> lam,
+                                                             double* grads, double* gn, int ntx) {
+  static_assert(TY % (kAThreads / kATX) == 0, "a thread owns whole rows of the tile");
+  static_assert(2 * kATX + 2 * TY <= kAThreads, "one ring cell per thread");
+  constexpr int kRowsPer = TY / (kAThreads / kATX);
+  __shared__ AdjTileLds<T, TY> s;
+  __shared__ T s_man[256];
+  __shared__ T s_rb[kARateCache], s_rs[kARateCache];  // station rates: unscaled, scaled
+  __shared__ double s_r[kAThreads / 64];
+  const int tid = threadIdx.x;
+  const int tile = xcd_band(blockIdx.x, gridDim.x);
+  const int x0 = (tile % ntx) * kATX, y0 = (tile / ntx) * TY;
+  const double t_now = a.ctrl->t;
+  const T dt = T(a.ctrl->dt);
+  const T t_T = T(t_now);
+  const int mode = a.rain_mode;
+  for (int k = tid; k < a.n_classes; k += kAThreads) s_man[k] = ((const T*)a.manning)[k];
+  const RainRows rr = rain_rows(a, t_now);
+  if (mode == HP_RAIN_STATION && tid < kARateCache) {
+    s_rb[tid] = station_rate<T>(rr, tid, 1.0);
+    s_rs[tid] = station_rate<T>(rr, tid, a.rain_scale);
+  }
+  __syncthreads();
+  const int64_t P = a.pitch;
+  const int ny = a.ny;
+  const T* __restrict__ h = (const T*)a.h;
+  const T* __restrict__ qx = (const T*)a.qx;
+  const T* __restrict__ qy = (const T*)a.qy;
+  const T* __restrict__ dh = (const T*)a.dh;
+  const T* __restrict__ dqx = (const T*)a.dqx;
+  const T* __restrict__ dqy = (const T*)a.dqy;
+  const T scale = T(a.rain_scale);
+  double rsum = 0.0;
+
+  // (1) update adjoint of element (x, y): the cotangent before the step into
+  // lo, the flux-increment cotangent into LDS (zero outside the plane)
+  auto cell = [&](int x, int y, bool owned, T(&lo)[3]) {
+    T c0 = T(0), c1 = T(0), c2 = T(0);
+    lo[0] = lo[1] = lo[2] = T(0);
+    if (x >= 0 && x < P && y >= 0 && y < ny) {
+      const int64_t i = (int64_t)y * P + x;
+      T lh = lam.in[0][i], lqx = lam.in[1][i], lqy = lam.in[2][i];
+      if (p.gauge_of) {  // the loss cotangent of the depth after the step
+        const int g = p.gauge_of[i];
+        if (g >= 0) lh += T(2.0 * (p.loss_h[g] - p.loss_obs[g]));
+      }
+      T rb = T(0), rs = T(0);
+      if (mode == HP_RAIN_STATION) {
+        const int st = a.station ? a.station[i] : 0;
+        const bool cached = st >= 0 && st < kARateCache;
+        rb = cached ? s_rb[st] : station_rate<T>(rr, st, 1.0);
+        rs = cached ? s_rs[st] : station_rate<T>(rr, st, a.rain_scale);
+      } else if (mode == HP_RAIN_FIELD) {
+        const float f = a.rain_field[i];
+        rb = T(f);
+        rs = T(f) * scale;
+      }
+      double g_acc = owned ? gn[i] : 0.0, r_acc = 0.0;
+      if (adj_update_cell<T>(a.landuse[i], h[i], qx[i], qy[i], dh[i], dqx[i], dqy[i], rb, rs, dt, s_man, lh, lqx, lqy,
+                             g_acc, r_acc) &&
+          flux_listed<T>(a, h, x, y, t_T)) {
+        c0 = lh;
+        c1 = lqx;
+        c2 = lqy;
+      }
+      if (owned) {
+        gn[i] = g_acc;
+        rsum += r_acc;
+      }
+      lo[0] = lh;
+      lo[1] = lqx;
+      lo[2] = lqy;
+    }
+    const int ly = y - y0 + 1, lx = x - x0 + 1;
+    s.cot[0][ly][lx] = c0;
+    s.cot[1][ly][lx] = c1;
+    s.cot[2][ly][lx] = c2;
+  };
+  const int lx = tid % kATX, lrow = tid / kATX;
+  T lp[kRowsPer][3];
+#pragma unroll
+  for (int k = 0; k < kRowsPer; ++k) cell(x0 + lx, y0 + lrow + k * (kAThreads / kATX), true, lp[k]);
+  if (tid < 2 * kATX + 2 * TY) {  // the ring: row below, row above, column left, column right
+    int hx, hy;
+    if (tid < kATX) { hx = x0 + tid; hy = y0 - 1; }
+    else if (tid < 2 * kATX) { hx = x0 + tid - kATX; hy = y0 + TY; }
+    else if (tid < 2 * kATX + TY) { hx = x0 - 1; hy = y0 + tid - 2 * kATX; }
+    else { hx = x0 + kATX; hy = y0 + tid - 2 * kATX - TY; }
+    T unused[3];
+    cell(hx, hy, false, unused);
+  }
+  __syncthreads();
+
+  // (2) edges
+  auto lds_cot = [&](int x, int y, T& ch, T& cx, T& cy) {
+    const int ly = y - y0 + 1, lc = x - x0 + 1;
+    ch = s.cot[0][ly][lc];
+    cx = s.cot[1][ly][lc];
+    cy = s.cot[2][ly][lc];
+  };
+  T wb[kRowsPer][3], sb[kRowsPer][3];  // this thread's cells from their west and south edges
+#pragma unroll
+  for (int k = 0; k < kRowsPer; ++k) {
+    const int ly = lrow + k * (kAThreads / kATX), col = x0 + lx, row = y0 + ly;
+    T out[4][3] = {};
+    if (col <= a.nx && row < ny) edge_adj<T, 1, 0>(a, false, col, row, lds_cot, out);
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+      wb[k][v] = out[2][v];
+      if (lx > 0) s.east[v][ly][lx - 1] = out[1][v];
+    }
+    T outy[4][3] = {};
+    if (col < a.nx && row <= ny) edge_adj<T, 1, 3>(a, false, col, row, lds_cot, outy);
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+      sb[k][v] = outy[2][v];
+      if (ly > 0) s.north[v][ly - 1][lx] = outy[1][v];
+    }
+  }
+  // the tile's north edges (one wave) and east edges: their A cells are the
+  // tile's last row and column
+  if (tid < kATX) {
+    const int col = x0 + tid, row = y0 + TY;
+    T out[4][3] = {};
+    if (col < a.nx && row <= ny) edge_adj<T, 1, 3>(a, false, col, row, lds_cot, out);
+#pragma unroll
+    for (int v = 0; v < 3; ++v) s.north[v][TY - 1][tid] = out[1][v];
+  } else if (tid < kATX + TY) {
+    const int ly = tid - kATX, col = x0 + kATX, row = y0 + ly;
+    T out[4][3] = {};
+    if (col <= a.nx && row < ny) edge_adj<T, 1, 0>(a, false, col, row, lds_cot, out);
+#pragma unroll
+    for (int v = 0; v < 3; ++v) s.east[v][ly][kATX - 1] = out[1][v];
+  }
+  __syncthreads();
+
+  // (3) the tile's cells: pass-through plus the four edges (as A: east and
+  // north edge; as B: west and south edge)
+#pragma unroll
+  for (int k = 0; k < kRowsPer; ++k) {
+    const int ly = lrow + k * (kAThreads / kATX), x = x0 + lx, y = y0 + ly;
+    if (y >= ny || x >= P) continue;
+    const int64_t i = (int64_t)y * P + x;
+    T o[3] = {lp[k][0], lp[k][1], lp[k][2]};
+    if (a.type_slot[i] != 0) {
+#pragma unroll
+      for (int v = 0; v < 3; ++v) {
+        T sum = T(0);
+        sum += s.east[v][ly][lx] + s.north[v][ly][lx];
+        sum += wb[k][v] + sb[k][v];
+        o[v] += sum;
+      }
+    }
+    lam.out[0][i] = o[0];
+    lam.out[1][i] = o[1];
+    lam.out[2][i] = o[2];
+  }
+  const int lane = tid & 63, w = tid >> 6;
+  for (int off = 32; off > 0; off >>= 1) rsum += __shfl_xor(rsum, off, 64);
+  if (lane == 0) s_r[w] = rsum;
+  __syncthreads();
+  if (tid == 0) {
+    double v = 0.0;
+    for (int k = 0; k < kAThreads / 64; ++k) v += s_r[k];
+    grads[(int64_t)blockIdx.x * kGradStride + kGradStride - 1] += v;
+  }
+}
+
+template <class T>
+static int64_t adj_tile_count(const hp_grid_desc* g) {
+  constexpr int TY = adj_tile_rows<T>();
+  return ((g->pitch + kATX - 1) / kATX) * (int64_t)((g->ny + TY - 1) / TY);
+}
+
+template <class T>
+static int launch_adjoint_tile(const hp_grid_desc* g, const hp_adjoint_desc* ad, void* const* in, void* const* out,
+                               hipStream_t st) {
+  AdjPlanes<T> p{};
+  p.gauge_of = ad->gauge_of;
+  p.loss_h = ad->loss_h;
+  p.loss_obs = ad->loss_obs;
+  AdjLam<T> lam;
+  for (int v = 0; v < 3; ++v) {
+    lam.in[v] = (const T*)in[v];
+    lam.out[v] = (T*)out[v];
+  }
+  constexpr int TY = adj_tile_rows<T>();
+  const int ntx = (int)((g->pitch + kATX - 1) / kATX);
+  hipLaunchKernelGGL((k_adj_step_tile<T, TY>), dim3((unsigned)adj_tile_count<T>(g)), dim3(kAThreads), 0, st, *g, p,
+                     lam, ad->grads, ad->gn, ntx);
+  return check_launch("k_adj_step_tile");
 }
 
 template <class T>
@@ -469,10 +720,13 @@ extern "C" {
 
 int64_t hp_adjoint_grad_blocks(const hp_grid_desc* g) {
   if (!g || g->pitch <= 0 || g->ny <= 0) return 0;
-  return (g->pitch * (int64_t)g->ny + kAChunk - 1) / kAChunk;
+  // rows for the blocks of k_adj_update and for those of the tiled step
+  const int64_t nu = (g->pitch * (int64_t)g->ny + kAChunk - 1) / kAChunk;
+  const int64_t nt = g->dtype == HP_F32 ? adj_tile_count<float>(g) : adj_tile_count<double>(g);
+  return nu > nt ? nu : nt;
 }
 
-int hp_adjoint_step(const hp_grid_desc* g, const hp_adjoint_desc* ad, void* stream) {
+static int check_adjoint(const hp_grid_desc* g, const hp_adjoint_desc* ad) {
   if (!ad) return fail(HP_EINVAL, "null adjoint descriptor");
   if (!g) return fail(HP_EINVAL, "null descriptor");
   if (!ad->lam_h || !ad->lam_qx || !ad->lam_qy || !ad->cot_h || !ad->cot_qx || !ad->cot_qy || !ad->xc || !ad->yc ||
@@ -483,9 +737,15 @@ int hp_adjoint_step(const hp_grid_desc* g, const hp_adjoint_desc* ad, void* stre
   if (g->n_classes > kGradStride - 1) return fail(HP_EINVAL, "the adjoint supports at most 15 Manning classes");
   if (g->infil_ks) return fail(HP_EINVAL, "the adjoint does not cover infiltration");
   if (ad->slope_mode != 0 && ad->slope_mode != 1) return fail(HP_EINVAL, "slope_mode must be 0 or 1");
+  if (ad->separate_passes != 0 && ad->separate_passes != 1) return fail(HP_EINVAL, "separate_passes must be 0 or 1");
   if (ad->gauge_of && (!ad->loss_h || !ad->loss_obs)) return fail(HP_EINVAL, "gauge_of needs loss_h and loss_obs");
   if (g->ny + 1 > 65535) return fail(HP_EINVAL, "the adjoint launches one block row per grid row: ny < 65535");
   if (g->pitch * (int64_t)g->ny >= (int64_t(1) << 31)) return fail(HP_EINVAL, "the adjoint covers planes of fewer than 2^31 elements");
+  return HP_OK;
+}
+
+int hp_adjoint_step(const hp_grid_desc* g, const hp_adjoint_desc* ad, void* stream) {
+  if (int rc = check_adjoint(g, ad)) return rc;
   if (!ad->flux_ready) {
     int rc = grid_flux_launch(g, (hipStream_t)stream);  // recompute the step's flux on U_k
     if (rc) return rc;
@@ -625,21 +885,48 @@ int hp_calib_reverse(const hp_grid_desc* g, const hp_adjoint_desc* ad, const hp_
   if (n_gauges > 0 && (!gauge_off || !h_rec || !obs)) return fail(HP_EINVAL, "the loss needs gauges, h_rec and obs");
   hp_adjoint_desc a = *ad;
   if (n_gauges <= 0) a.gauge_of = nullptr;
+  if (a.gauge_of) {  // the rows of each step are set below
+    a.loss_h = h_rec;
+    a.loss_obs = obs;
+  }
+  if (int rc = check_adjoint(g, &a)) return rc;
+  // Order 1 runs each step as one tiled pass, which reads the cotangent from
+  // one set of planes and writes it to another: lam_* and cot_* alternate,
+  // and the result is copied back to lam_* after an odd number of steps.
+  const bool tiled = g->order == 1 && !ad->separate_passes;
+  void* lam[3] = {ad->lam_h, ad->lam_qx, ad->lam_qy};
+  void* alt[3] = {ad->cot_h, ad->cot_qx, ad->cot_qy};
+  void** cur = lam;
+  void** nxt = alt;
+  hipStream_t st = (hipStream_t)stream;
   for (int32_t k = k_hi - 1; k >= k_lo; --k) {
     int rc;
     if (a.gauge_of) {  // the loss enters in the step's own update pass
       a.loss_h = h_rec + (int64_t)(k + 1) * n_gauges;
       a.loss_obs = obs + (int64_t)(k + 1) * n_gauges;
     } else if (n_gauges > 0) {
-      rc = hp_adjoint_add_loss(g->dtype, ad->lam_h, gauge_off, h_rec + (int64_t)(k + 1) * n_gauges,
+      rc = hp_adjoint_add_loss(g->dtype, cur[0], gauge_off, h_rec + (int64_t)(k + 1) * n_gauges,
                                obs + (int64_t)(k + 1) * n_gauges, n_gauges, 1.0, stream);
       if (rc) return rc;
     }
     const int64_t off = (int64_t)(k - seg_k0) * seg_stride;
     hp_grid_desc d = with_segment(*g, (char*)seg_u + off, (char*)seg_d + off, plane_bytes);
     d.ctrl = const_cast<hp_ctrl*>(frozen) + k;  // t and dt of step k; read only
-    rc = hp_adjoint_step(&d, &a, stream);
+    if (tiled) {
+      rc = g->dtype == HP_F32 ? launch_adjoint_tile<float>(&d, &a, cur, nxt, st)
+                              : launch_adjoint_tile<double>(&d, &a, cur, nxt, st);
+      void** t = cur;
+      cur = nxt;
+      nxt = t;
+    } else {
+      rc = hp_adjoint_step(&d, &a, stream);
+    }
     if (rc) return rc;
+  }
+  if (cur != lam) {
+    for (int v = 0; v < 3; ++v)
+      if (hipMemcpyAsync(lam[v], cur[v], plane_bytes, hipMemcpyDeviceToDevice, st) != hipSuccess)
+        return fail(HP_ELAUNCH, "cotangent copy failed");
   }
   return HP_OK;
 }

@@ -47,6 +47,7 @@ __host__ __device__ inline bool solve_edge_vjp(const FrameCell<T>& c, const Fram
                                                T v_fh, T v_fn, T v_ft, T v_bc, T v_br, FrameAdj<T>& ac,
                                                FrameAdj<T>& al, FrameAdj<T>& ar, FrameAdj<T>& arr, T& a_etaC,
                                                T& a_etaR, bool frozen_slopes = false) {
+  using P = Prec<T>;
   const T hs = T(kHs), g = T(kG);
   if (c.h < hs && r.h < hs) return false;
 
@@ -82,8 +83,8 @@ __host__ __device__ inline bool solve_edge_vjp(const FrameCell<T>& c, const Fram
   }
   // ---- forward: velocities and hydrostatic reconstruction ----
   const bool dryL0 = eL.h < hs, dryR0 = eR.h < hs;
-  const T uL0 = dryL0 ? T(0) : eL.qn / eL.h, vL0 = dryL0 ? T(0) : eL.qt / eL.h;
-  const T uR0 = dryR0 ? T(0) : eR.qn / eR.h, vR0 = dryR0 ? T(0) : eR.qt / eR.h;
+  const T uL0 = dryL0 ? T(0) : P::div(eL.qn, eL.h), vL0 = dryL0 ? T(0) : P::div(eL.qt, eL.h);
+  const T uR0 = dryR0 ? T(0) : P::div(eR.qn, eR.h), vR0 = dryR0 ? T(0) : P::div(eR.qt, eR.h);
   const T wL0 = eL.h + eL.z, wR0 = eR.h + eR.z;
   const bool maxC = etaC >= etaR;  // fmax(etaC, etaR) took etaC
   const bool minC = etaC <= etaR;  // fmin(etaC, etaR) took etaC
@@ -130,16 +131,15 @@ __host__ __device__ inline bool solve_edge_vjp(const FrameCell<T>& c, const Fram
     const T qnL = uL0 * hL, qtL = vL0 * hL, qnR = uR0 * hR, qtR = vR0 * hR;
     const T aL = Prec<T>::sqrt_(g * hL), aR = Prec<T>::sqrt_(g * hR);
     const T base = (aL + aR) / T(2) + (uL0 - uR0) / T(4);
-    const T h_star = base * base / g;
     const T u_star = (uL0 + uR0) / T(2) + aL - aR;
-    const T a_star = Prec<T>::sqrt_(g * h_star);
+    const T a_star = P::star_celerity(base, g);
     const bool sLdry = hL <= hs, sRdry = hR <= hs;
     const T x1 = uL0 - aL, x2 = u_star - a_star;
     const T y1 = uR0 + aR, y2 = u_star + a_star;
     const bool sLx1 = x1 <= x2, sRy1 = y1 >= y2;
     const T sL = sLdry ? uR0 - T(2) * aR : (sLx1 ? x1 : x2);
     const T sR = sRdry ? uL0 + T(2) * aL : (sRy1 ? y1 : y2);
-    const T sM = (sL * hR * (uR0 - sR) - sR * hL * (uL0 - sL)) / (hR * (uR0 - sR) - hL * (uL0 - sL));
+    const T sM = P::div(sL * hR * (uR0 - sR) - sR * hL * (uL0 - sL), hR * (uR0 - sR) - hL * (uL0 - sL));
     const T b_fh = v_fh * s, b_fn = v_fn * s, b_ft = v_ft * s;
     T a_qnL = T(0), a_qtL = T(0), a_qnR = T(0), a_qtR = T(0), a_sL = T(0), a_sR = T(0);
     if (T(0) <= sL) {
@@ -155,11 +155,11 @@ __host__ __device__ inline bool solve_edge_vjp(const FrameCell<T>& c, const Fram
       const T qnfR = uR0 * qnR + T(0.5) * g * hR * hR;
       const T Nh = sR * qnL - sL * qnR + sL * sR * (hR - hL);
       const T Nn = sR * qnfL - sL * qnfR + sL * sR * (qnR - qnL);
-      const T fh = Nh / D, fn = Nn / D;
+      const T fh = P::div(Nh, D), fn = P::div(Nn, D);
       const T b_fh_tot = b_fh + b_ft * vstar;
       if (useL) a_vL0 += b_ft * fh; else a_vR0 += b_ft * fh;
-      T a_D = -(b_fh_tot * fh + b_fn * fn) / D;
-      const T a_Nh = b_fh_tot / D, a_Nn = b_fn / D;
+      T a_D = P::div(-(b_fh_tot * fh + b_fn * fn), D);
+      const T a_Nh = P::div(b_fh_tot, D), a_Nn = P::div(b_fn, D);
       a_sR += a_Nh * (qnL + sL * (hR - hL));
       a_sL += a_Nh * (-qnR + sR * (hR - hL));
       a_qnL += a_Nh * sR;
@@ -189,12 +189,11 @@ __host__ __device__ inline bool solve_edge_vjp(const FrameCell<T>& c, const Fram
     if (sRdry) { a_uL0 += a_sR; a_aL += T(2) * a_sR; }
     else if (sRy1) { a_uR0 += a_sR; a_aR += a_sR; }
     else { a_ustar += a_sR; a_astar += a_sR; }
-    const T a_hstar = a_star > T(0) ? a_astar * g / (T(2) * a_star) : T(0);
-    const T a_base = a_hstar * T(2) * base / g;
+    const T a_base = P::star_celerity_adj(base, g, a_star, a_astar);
     a_uL0 += a_ustar / T(2); a_uR0 += a_ustar / T(2); a_aL += a_ustar; a_aR -= a_ustar;
     a_aL += a_base / T(2); a_aR += a_base / T(2); a_uL0 += a_base / T(4); a_uR0 -= a_base / T(4);
-    if (aL > T(0)) a_hL += a_aL * g / (T(2) * aL);
-    if (aR > T(0)) a_hR += a_aR * g / (T(2) * aR);
+    if (aL > T(0)) a_hL += P::div(a_aL * g, T(2) * aL);
+    if (aR > T(0)) a_hR += P::div(a_aR * g, T(2) * aR);
     a_uL0 += a_qnL * hL; a_hL += a_qnL * uL0;
     a_vL0 += a_qtL * hL; a_hL += a_qtL * vL0;
     a_uR0 += a_qnR * hR; a_hR += a_qnR * uR0;
@@ -238,14 +237,14 @@ __host__ __device__ inline bool solve_edge_vjp(const FrameCell<T>& c, const Fram
   a_eRh += a_wR0;
   T a_eLqn = T(0), a_eLqt = T(0), a_eRqn = T(0), a_eRqt = T(0);
   if (!dryL0) {
-    a_eLqn += a_uL0 / eL.h;
-    a_eLqt += a_vL0 / eL.h;
-    a_eLh -= (a_uL0 * uL0 + a_vL0 * vL0) / eL.h;
+    a_eLqn += P::div(a_uL0, eL.h);
+    a_eLqt += P::div(a_vL0, eL.h);
+    a_eLh -= P::div(a_uL0 * uL0 + a_vL0 * vL0, eL.h);
   }
   if (!dryR0) {
-    a_eRqn += a_uR0 / eR.h;
-    a_eRqt += a_vR0 / eR.h;
-    a_eRh -= (a_uR0 * uR0 + a_vR0 * vR0) / eR.h;
+    a_eRqn += P::div(a_uR0, eR.h);
+    a_eRqt += P::div(a_vR0, eR.h);
+    a_eRh -= P::div(a_uR0 * uR0 + a_vR0 * vR0, eR.h);
   }
 
   // ---- reverse: edge states ----
@@ -326,9 +325,9 @@ __host__ __device__ inline void friction_vjp(T h, T n, T dt, T qx, T qy, T a_ox,
   const T qn2 = qx * qx + qy * qy;
   const T q_norm = P::sqrt_(qn2);
   const T lim = T(5) * h * P::sqrt_(h * g);
-  const T x = (T(4) * dt * C_f) / (h * h) * q_norm;
+  const T x = P::div(T(4) * dt * C_f, h * h) * q_norm;
   const T S = P::sqrt_(T(1) + x);
-  const T k = T(2) / (T(1) + S);
+  const T k = P::div(T(2), T(1) + S);
   T a_k = T(0);
   const T qc[2] = {qx, qy};
   const T ac_[2] = {a_ox, a_oy};
@@ -352,18 +351,18 @@ __host__ __device__ inline void friction_vjp(T h, T n, T dt, T qx, T qy, T a_ox,
     a_k += a_tmp * q;
     aq[m] += a_tmp * k;
   }
-  const T dk_dx = -T(1) / ((T(1) + S) * (T(1) + S) * S);
+  const T dk_dx = P::div(-T(1), (T(1) + S) * (T(1) + S) * S);
   const T a_x = a_k * dk_dx;
-  const T coef = (T(4) * dt) / (h * h);
+  const T coef = P::div(T(4) * dt, h * h);
   const T a_Cf = a_x * coef * q_norm;
   T a_qnorm = a_x * coef * C_f;
-  a_h += a_x * (T(-2) * x / h);
+  a_h += a_x * P::div(T(-2) * x, h);
   if (q_norm > T(0)) {
-    aq[0] += a_qnorm * qx / q_norm;
-    aq[1] += a_qnorm * qy / q_norm;
+    aq[0] += P::div(a_qnorm * qx, q_norm);
+    aq[1] += P::div(a_qnorm * qy, q_norm);
   }
-  a_n += a_Cf * T(2) * C_f / n;
-  a_h += a_Cf * (T(-1.0 / 3.0)) * C_f / h;
+  a_n += P::div(a_Cf * T(2) * C_f, n);
+  a_h += P::div(a_Cf * (T(-1.0 / 3.0)) * C_f, h);
   a_qx += aq[0];
   a_qy += aq[1];
 }

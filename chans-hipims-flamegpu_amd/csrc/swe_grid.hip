@@ -522,17 +522,6 @@ __device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by, const Pa
   if (FUSED) block_partial<T>(mn, vol, fsum, rsum, part, pidx, sums);
 }
 
-// Workgroups of a launch are handed to the 8 XCDs round-robin (block b runs
-// on XCD b % 8), and each XCD has its own L2.  Tile numbers are reassigned so
-// that every XCD walks one contiguous band of tiles: x- and y-neighbours then
-// share their halo lines in one L2 instead of fetching them from HBM twice.
-constexpr int kXcds = 8;
-__device__ inline int xcd_band(int b, int n) {
-  const int q = n / kXcds, rem = n - q * kXcds;
-  const int x = b % kXcds, local = b / kXcds;
-  return x * q + min(x, rem) + local;
-}
-
 template <class T, int ORDER, int TY>
 __global__ __launch_bounds__(kThreads) void k_flux(hp_grid_desc a, int ntx) {
   const int tile = xcd_band(blockIdx.x, gridDim.x);

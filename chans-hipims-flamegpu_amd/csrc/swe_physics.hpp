@@ -60,6 +60,12 @@ template <> struct Prec<double> {
     const double h_star = base * base / g;
     return __builtin_sqrt(g * h_star);
   }
+  // cotangent of base from that of the celerity a* = sqrt(g h*): through
+  // h* = base^2 / g, zero where a* = 0
+  __host__ __device__ static inline double star_celerity_adj(double base, double g, double a_star, double a_astar) {
+    const double a_hstar = a_star > 0.0 ? a_astar * g / (2.0 * a_star) : 0.0;
+    return a_hstar * 2.0 * base / g;
+  }
 };
 template <> struct Prec<float> {
   static constexpr bool local_datum = true;
@@ -80,6 +86,10 @@ template <> struct Prec<float> {
   __device__ static inline float scale(float x, float, float, float dt_dx) { return x * dt_dx; }
   // sqrt(g * base^2 / g) is |base| exactly; no square root needed
   __device__ static inline float star_celerity(float base, float) { return fabsf(base); }
+  // d|base| / d base = sign(base), 0 at base = 0
+  __device__ static inline float star_celerity_adj(float base, float, float, float a_astar) {
+    return base > 0.0f ? a_astar : (base < 0.0f ? -a_astar : 0.0f);
+  }
 };
 
 // A cell seen from an edge frame.

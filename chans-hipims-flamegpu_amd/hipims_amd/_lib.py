@@ -22,7 +22,7 @@ HP_OK, HP_EINVAL, HP_ELAUNCH, HP_ESPACE = 0, -1, -2, -3
 HP_F32, HP_F64 = 0, 1
 HP_RAIN_NONE, HP_RAIN_STATION, HP_RAIN_FIELD = 0, 1, 2
 HP_UPDATE_STEP, HP_UPDATE_FRICTION, HP_UPDATE_EULER, HP_UPDATE_CFL = 0, 1, 2, 3
-ABI_VERSION = 7
+ABI_VERSION = 8
 
 _P = ctypes.c_void_p
 _I64 = ctypes.c_int64
@@ -71,7 +71,8 @@ class HpEnsemble(ctypes.Structure):
 class HpAdjointDesc(ctypes.Structure):
     _fields_ = [("lam_h", _P), ("lam_qx", _P), ("lam_qy", _P), ("cot_h", _P), ("cot_qx", _P), ("cot_qy", _P),
                 ("xc", _P), ("yc", _P), ("cpitch", _I64), ("grads", _P), ("gn", _P), ("slope_mode", _I32),
-                ("flux_ready", _I32), ("gauge_of", _P), ("loss_h", _P), ("loss_obs", _P)]
+                ("flux_ready", _I32), ("gauge_of", _P), ("loss_h", _P), ("loss_obs", _P),
+                ("separate_passes", _I32)]
 
 
 GRAD_STRIDE = 16

@@ -9,12 +9,14 @@ recomputation), as the adjoint of the CFL minimum is not defined.
 
 backward(): the state before every `checkpoint`-th step is kept from the
 forward run; segments are recomputed last to first, storing the state before
-each step and the step's flux, and the hand-written adjoint kernels
-(hp_adjoint_step, with the saved flux) run the steps in reverse directly on
-the stored planes.  The per-step sequences of the forward run, of a
+each step and the step's flux, and the hand-written adjoint kernels run
+the steps in reverse directly on the stored planes (order 1: one tiled
+kernel per step; order 2: the update, edge and gather passes of
+hp_adjoint_step).  The per-step sequences of the forward run, of a
 segment's recomputation and of its reverse sweep are native loops
-(hp_calib_forward, hp_calib_reverse): one call per segment.  checkpoint=1 keeps every state and flux from the forward
-run and recomputes nothing (K x 6 planes of device memory).  Parameter
+(hp_calib_forward, hp_calib_reverse): one call per segment.  checkpoint=1
+keeps every state and flux from the forward run and recomputes nothing
+(K x 6 planes of device memory).  Parameter
 gradients are accumulated in FP64 (per cell for Manning, per block for the
 rain multiplier) and folded at the end.  ``CalibrationLoss.apply(theta,
 calib)`` exposes this as a torch.autograd.Function.
@@ -38,15 +40,19 @@ from .grid import GridSolver
 
 class Calibration:
     def __init__(self, solver: GridSolver, gauges, steps: int, checkpoint: int = 100, obs=None,
-                 slopes: str = "exact"):
+                 slopes: str = "exact", reverse: str = "tiled"):
         """gauges: (row, col) raster cells; obs: [steps + 1, G] observed depths
         (row 0 unused) or None (set later).  slopes: "exact" differentiates
         the limited MUSCL slopes (the exact derivative of the discrete step;
         its linearisation grows over long horizons, as that of MUSCL with
         forward Euler does), "frozen" holds them fixed (stable over long
-        horizons; an approximation of the derivative)."""
+        horizons; an approximation of the derivative).  reverse: "tiled" runs
+        each order-1 reverse step as one tiled kernel, "separate" as the
+        update, edge and gather passes (order 2 always does)."""
         if slopes not in ("exact", "frozen"):
             raise ValueError("slopes must be 'exact' or 'frozen'")
+        if reverse not in ("tiled", "separate"):
+            raise ValueError("reverse must be 'tiled' or 'separate'")
         self.s = s = solver
         s.ensure_out()  # the calibration runs use the single-pass step
         self.K = int(steps)
@@ -97,6 +103,7 @@ class Calibration:
         ad.gn = self.gn.data_ptr()
         ad.slope_mode = 1 if slopes == "frozen" else 0
         ad.flux_ready = 1
+        ad.separate_passes = 1 if reverse == "separate" else 0
         # gauge index per cell: the reverse sweep adds the loss cotangent in
         # the adjoint's own update pass (one gauge per cell; otherwise a
         # separate small kernel per step does it)

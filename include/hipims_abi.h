@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define HP_ABI_VERSION 7
+#define HP_ABI_VERSION 8
 
 enum hp_status {
   HP_OK = 0,
@@ -268,7 +268,11 @@ int hp_ensemble_step(const hp_ensemble* e, int32_t nsteps, void* stream);
  * the gauge's index g (-1 elsewhere; each cell at most one gauge); with it
  * the step first adds the squared-misfit cotangent 2 (loss_h[g] -
  * loss_obs[g]) (FP64 rows of the depths after the step and the observations)
- * to lam_h at the gauge cells, in the same pass. */
+ * to lam_h at the gauge cells, in the same pass.
+ * separate_passes (hp_calib_reverse only): 0 runs each order-1 step as one
+ * tiled kernel (update, edge and gather adjoints in one pass, with cot_*
+ * and lam_* taking turns as the cotangent planes; xc / yc unused); 1 runs the
+ * separate passes of hp_adjoint_step.  Order 2 always uses the latter. */
 typedef struct hp_adjoint_desc {
   void *lam_h, *lam_qx, *lam_qy;
   void *cot_h, *cot_qx, *cot_qy;
@@ -281,6 +285,7 @@ typedef struct hp_adjoint_desc {
   const int32_t* gauge_of;
   const double* loss_h;
   const double* loss_obs;
+  int32_t separate_passes;
 } hp_adjoint_desc;
 
 int64_t hp_adjoint_grad_blocks(const hp_grid_desc* g);
@@ -314,9 +319,12 @@ int hp_calib_forward(const hp_grid_desc* g, double* sched, const hp_ctrl* frozen
 /* hp_calib_reverse runs the reverse sweep over steps k_hi-1 down to k_lo:
  * per step the loss cotangent of the depths after it (row k+1 of h_rec and
  * obs, FP64 [K+1][n_gauges]; inside the step's update pass when a->gauge_of
- * is set, else through hp_adjoint_add_loss) and hp_adjoint_step with the
- * control block frozen[k] on the state and flux saved at
- * seg_u / seg_d + (k - seg_k0)*seg_stride bytes (flux_ready must be 1). */
+ * is set, else through hp_adjoint_add_loss) and the step's adjoint (one
+ * tiled pass for order 1 unless a->separate_passes, else hp_adjoint_step)
+ * with the control block frozen[k] on the state and flux saved at
+ * seg_u / seg_d + (k - seg_k0)*seg_stride bytes (flux_ready must be 1).  On
+ * return lam_* hold the cotangent of the state before step k_lo; cot_* are
+ * overwritten. */
 int hp_calib_reverse(const hp_grid_desc* g, const hp_adjoint_desc* a, const hp_ctrl* frozen, int32_t k_lo,
                      int32_t k_hi, int32_t seg_k0, void* seg_u, void* seg_d, int64_t seg_stride,
                      const int64_t* gauge_off, int32_t n_gauges, const double* h_rec, const double* obs,
