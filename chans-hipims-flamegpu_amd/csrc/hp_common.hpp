@@ -24,6 +24,19 @@ inline int check_launch(const char* what) {
   return HP_OK;
 }
 
+// What a calibration step records once its time control has run: the depths
+// at n plane offsets into `row` (grid dtype) and, when sched is set, (t, dt)
+// of the next step into sched[k_next].
+struct StepRecord {
+  const void* h;
+  const int64_t* off;
+  int n;
+  void* row;
+  double* sched;
+  int k_next;
+  int f32;
+};
+
 // Workgroups of a launch are handed to the 8 XCDs round-robin (block b runs
 // on XCD b % 8), and each XCD has its own L2.  Tile numbers are reassigned so
 // that every XCD walks one contiguous band of tiles: x- and y-neighbours then

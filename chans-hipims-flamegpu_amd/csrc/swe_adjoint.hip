@@ -43,7 +43,8 @@ constexpr int kGradStride = 16;  // doubles per block: classes 0..14, rain at 15
 constexpr int kARateCache = 64;  // station rates kept in LDS per block
 
 int grid_flux_launch(const hp_grid_desc* g, hipStream_t st);                   // swe_grid.hip
-int grid_fused_launch(const hp_grid_desc* g, bool save_flux, bool finalize, hipStream_t st);  // swe_grid.hip
+int grid_fused_launch(const hp_grid_desc* g, bool save_flux, bool finalize, hipStream_t st,
+                      const StepRecord* rec);  // swe_grid.hip
 
 // Records (t, dt) of the control block as step k of a schedule [K+1][2].
 __global__ void k_schedule_save(const hp_ctrl* ctrl, double* sched, int k) {
@@ -854,12 +855,16 @@ int hp_calib_forward(const hp_grid_desc* g, double* sched, const hp_ctrl* frozen
     }
     d.h = in[0]; d.qx = in[1]; d.qy = in[2];
     d.h_out = out[0]; d.qx_out = out[1]; d.qy_out = out[2];
-    if ((rc = grid_fused_launch(&d, seg_u != nullptr, frozen == nullptr, st))) return rc;
     double* save = (!frozen && sched && j + 1 < nsteps) ? sched : nullptr;
-    if (gauges || save) {
-      const int n = gauges ? n_gauges : 0;
+    const int n = gauges ? n_gauges : 0;
+    char* row = gauges ? (char*)hist + (int64_t)(k + 1) * n_gauges * esz : nullptr;
+    // with time control the record rides on its kernel; a frozen schedule
+    // has none, so the gauges then take a small kernel of their own
+    const StepRecord rec{out[0], gauge_off, n, row, save, k + 1, g->dtype == HP_F32};
+    const bool ride = !frozen && (gauges || save);
+    if ((rc = grid_fused_launch(&d, seg_u != nullptr, frozen == nullptr, st, ride ? &rec : nullptr))) return rc;
+    if (frozen && gauges) {
       const unsigned nb = (unsigned)((n > 0 ? n : 1) + 255) / 256;
-      char* row = gauges ? (char*)hist + (int64_t)(k + 1) * n_gauges * esz : nullptr;
       if (g->dtype == HP_F32)
         hipLaunchKernelGGL(k_step_record<float>, dim3(nb), dim3(256), 0, st, (const float*)out[0], gauge_off, n,
                            (float*)row, g->ctrl, save, k + 1);

@@ -808,6 +808,22 @@ __global__ __launch_bounds__(1024) void k_finalize(hp_ctrl* ctrl, void* work, in
   finalize_domain(ctrl, P.mins, P.sums, n, dx, sums != 0);
 }
 
+// k_finalize followed by a calibration step's record (thread 0 wrote the
+// control block, so it also reads the next step's t and dt).
+__global__ __launch_bounds__(1024) void k_finalize_record(hp_ctrl* ctrl, void* work, int n, double dx, int sums,
+                                                          StepRecord rec) {
+  const Partials P(work, n);
+  finalize_domain(ctrl, P.mins, P.sums, n, dx, sums != 0);
+  for (int i = threadIdx.x; i < rec.n; i += blockDim.x) {
+    if (rec.f32) ((float*)rec.row)[i] = ((const float*)rec.h)[rec.off[i]];
+    else ((double*)rec.row)[i] = ((const double*)rec.h)[rec.off[i]];
+  }
+  if (rec.sched && threadIdx.x == 0) {
+    rec.sched[2 * (int64_t)rec.k_next] = ctrl->t;
+    rec.sched[2 * (int64_t)rec.k_next + 1] = ctrl->dt;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_finalize_multi(const hp_grid_desc* __restrict__ doms,
                                                         const int2* __restrict__ ranges, void* parts,
                                                         int n_total) {
@@ -918,7 +934,12 @@ static int launch_step_fused(const hp_grid_desc* g, hipStream_t st, int save_flu
   return launch_fused_t<double, 1, 4>(g, st, save_flux);
 }
 
-static int launch_finalize_fused(const hp_grid_desc* g, hipStream_t st) {
+static int launch_finalize_fused(const hp_grid_desc* g, hipStream_t st, const StepRecord* rec = nullptr) {
+  if (rec) {
+    hipLaunchKernelGGL(k_finalize_record, dim3(1), dim3(1024), 0, st, g->ctrl, g->work, (int)flux_tiles(g), g->dx,
+                       g->mass_sums, *rec);
+    return check_launch("k_finalize_record");
+  }
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, st, g->ctrl, g->work, (int)flux_tiles(g), g->dx,
                      g->mass_sums);
   return check_launch("k_finalize");
@@ -1029,9 +1050,9 @@ __global__ void k_scatter(T* __restrict__ plane, const int64_t* __restrict__ off
 }
 
 // Single-pass step (h, qx, qy -> h_out, qx_out, qy_out; with save_flux the
-// step's flux also into dh, dqx, dqy) and, with finalize, its time control;
-// shared with the calibration runs.
-int grid_fused_launch(const hp_grid_desc* g, bool save_flux, bool finalize, hipStream_t st) {
+// step's flux also into dh, dqx, dqy) and, with finalize, its time control,
+// followed by rec when given; shared with the calibration runs.
+int grid_fused_launch(const hp_grid_desc* g, bool save_flux, bool finalize, hipStream_t st, const StepRecord* rec) {
   int rc = check_desc(g);
   if (rc) return rc;
   if (!g->h_out || !g->qx_out || !g->qy_out) return fail(HP_EINVAL, "the fused step needs h_out, qx_out, qy_out");
@@ -1040,7 +1061,7 @@ int grid_fused_launch(const hp_grid_desc* g, bool save_flux, bool finalize, hipS
     return fail(HP_ESPACE, "workspace smaller than hp_grid_workspace_bytes()");
   rc = launch_step_fused(g, st, save_flux ? 1 : 0);
   if (rc || !finalize) return rc;
-  return launch_finalize_fused(g, st);
+  return launch_finalize_fused(g, st, rec);
 }
 
 // Flux pass launch shared with the adjoint (which recomputes it).
