@@ -477,17 +477,18 @@ template <class T> struct AdjLam {
   T* out[3];
 };
 template <class T> constexpr int adj_tile_rows() { return sizeof(T) == 4 ? 8 : 4; }
+constexpr int kATileThreads = 256;
 
-template <class T, int TY>
-__global__ __launch_bounds__(kAThreads) void k_adj_step_tile(hp_grid_desc a, AdjPlanes<T> p, AdjLam<T> lam,
+template <class T, int TY, int NT>
+__global__ __launch_bounds__(NT) void k_adj_step_tile(hp_grid_desc a, AdjPlanes<T> p, AdjLam<T> lam,
                                                              double* grads, double* gn, int ntx) {
-  static_assert(TY % (kAThreads / kATX) == 0, "a thread owns whole rows of the tile");
-  static_assert(2 * kATX + 2 * TY <= kAThreads, "one ring cell per thread");
-  constexpr int kRowsPer = TY / (kAThreads / kATX);
+  static_assert(TY % (NT / kATX) == 0, "a thread owns whole rows of the tile");
+  static_assert(2 * kATX + 2 * TY <= NT, "one ring cell per thread");
+  constexpr int kRowsPer = TY / (NT / kATX);
   __shared__ AdjTileLds<T, TY> s;
   __shared__ T s_man[256];
   __shared__ T s_rb[kARateCache], s_rs[kARateCache];  // station rates: unscaled, scaled
-  __shared__ double s_r[kAThreads / 64];
+  __shared__ double s_r[NT / 64];
   const int tid = threadIdx.x;
   const int tile = xcd_band(blockIdx.x, gridDim.x);
   const int x0 = (tile % ntx) * kATX, y0 = (tile / ntx) * TY;
@@ -495,7 +496,7 @@ __global__ __launch_bounds__(kAThreads) void k_adj_step_tile(hp_grid_desc a, Adj
   const T dt = T(a.ctrl->dt);
   const T t_T = T(t_now);
   const int mode = a.rain_mode;
-  for (int k = tid; k < a.n_classes; k += kAThreads) s_man[k] = ((const T*)a.manning)[k];
+  for (int k = tid; k < a.n_classes; k += NT) s_man[k] = ((const T*)a.manning)[k];
   const RainRows rr = rain_rows(a, t_now);
   if (mode == HP_RAIN_STATION && tid < kARateCache) {
     s_rb[tid] = station_rate<T>(rr, tid, 1.0);
@@ -560,7 +561,7 @@ __global__ __launch_bounds__(kAThreads) void k_adj_step_tile(hp_grid_desc a, Adj
   const int lx = tid % kATX, lrow = tid / kATX;
   T lp[kRowsPer][3];
 #pragma unroll
-  for (int k = 0; k < kRowsPer; ++k) cell(x0 + lx, y0 + lrow + k * (kAThreads / kATX), true, lp[k]);
+  for (int k = 0; k < kRowsPer; ++k) cell(x0 + lx, y0 + lrow + k * (NT / kATX), true, lp[k]);
   if (tid < 2 * kATX + 2 * TY) {  // the ring: row below, row above, column left, column right
     int hx, hy;
     if (tid < kATX) { hx = x0 + tid; hy = y0 - 1; }
@@ -582,7 +583,7 @@ __global__ __launch_bounds__(kAThreads) void k_adj_step_tile(hp_grid_desc a, Adj
   T wb[kRowsPer][3], sb[kRowsPer][3];  // this thread's cells from their west and south edges
 #pragma unroll
   for (int k = 0; k < kRowsPer; ++k) {
-    const int ly = lrow + k * (kAThreads / kATX), col = x0 + lx, row = y0 + ly;
+    const int ly = lrow + k * (NT / kATX), col = x0 + lx, row = y0 + ly;
     T out[4][3] = {};
     if (col <= a.nx && row < ny) edge_adj<T, 1, 0>(a, false, col, row, lds_cot, out);
 #pragma unroll
@@ -619,7 +620,7 @@ __global__ __launch_bounds__(kAThreads) void k_adj_step_tile(hp_grid_desc a, Adj
   // north edge; as B: west and south edge)
 #pragma unroll
   for (int k = 0; k < kRowsPer; ++k) {
-    const int ly = lrow + k * (kAThreads / kATX), x = x0 + lx, y = y0 + ly;
+    const int ly = lrow + k * (NT / kATX), x = x0 + lx, y = y0 + ly;
     if (y >= ny || x >= P) continue;
     const int64_t i = (int64_t)y * P + x;
     T o[3] = {lp[k][0], lp[k][1], lp[k][2]};
@@ -642,7 +643,7 @@ __global__ __launch_bounds__(kAThreads) void k_adj_step_tile(hp_grid_desc a, Adj
   __syncthreads();
   if (tid == 0) {
     double v = 0.0;
-    for (int k = 0; k < kAThreads / 64; ++k) v += s_r[k];
+    for (int k = 0; k < NT / 64; ++k) v += s_r[k];
     grads[(int64_t)blockIdx.x * kGradStride + kGradStride - 1] += v;
   }
 }
@@ -667,8 +668,8 @@ static int launch_adjoint_tile(const hp_grid_desc* g, const hp_adjoint_desc* ad,
   }
   constexpr int TY = adj_tile_rows<T>();
   const int ntx = (int)((g->pitch + kATX - 1) / kATX);
-  hipLaunchKernelGGL((k_adj_step_tile<T, TY>), dim3((unsigned)adj_tile_count<T>(g)), dim3(kAThreads), 0, st, *g, p,
-                     lam, ad->grads, ad->gn, ntx);
+  hipLaunchKernelGGL((k_adj_step_tile<T, TY, kATileThreads>), dim3((unsigned)adj_tile_count<T>(g)),
+                     dim3(kATileThreads), 0, st, *g, p, lam, ad->grads, ad->gn, ntx);
   return check_launch("k_adj_step_tile");
 }
 
