@@ -40,7 +40,7 @@ from .grid import GridSolver
 
 class Calibration:
     def __init__(self, solver: GridSolver, gauges, steps: int, checkpoint: int = 100, obs=None,
-                 slopes: str = "exact", reverse: str = "tiled"):
+                 slopes: str = "exact", reverse: str = "tiled", overlap: bool = True):
         """gauges: (row, col) raster cells; obs: [steps + 1, G] observed depths
         (row 0 unused) or None (set later).  slopes: "exact" differentiates
         the limited MUSCL slopes (the exact derivative of the discrete step;
@@ -48,7 +48,10 @@ class Calibration:
         forward Euler does), "frozen" holds them fixed (stable over long
         horizons; an approximation of the derivative).  reverse: "tiled" runs
         each order-1 reverse step as one tiled kernel, "separate" as the
-        update, edge and gather passes (order 2 always does)."""
+        update, edge and gather passes (order 2 always does).  overlap (with
+        checkpoint > 1): the recomputation of the next segment runs on a
+        second stream, into a second set of segment planes, while the
+        current segment is reversed."""
         if slopes not in ("exact", "frozen"):
             raise ValueError("slopes must be 'exact' or 'frozen'")
         if reverse not in ("tiled", "separate"):
@@ -94,6 +97,11 @@ class Calibration:
         n_keep = self.K if self.ck == 1 else self.ck
         self.segU = torch.zeros((n_keep,) + tuple(s.U.shape), dtype=s.dtype, device=dev)
         self.segD = torch.zeros_like(self.segU)
+        self.overlap = bool(overlap) and self.ck > 1 and self.K > self.ck
+        self.seg_bufs = [(self.segU, self.segD)]
+        if self.overlap:
+            self.seg_bufs.append((torch.zeros_like(self.segU), torch.zeros_like(self.segU)))
+            self.side = torch.cuda.Stream(device=dev)  # default priority
         ad = HpAdjointDesc()
         ad.lam_h, ad.lam_qx, ad.lam_qy = (t.data_ptr() for t in self.lam)
         ad.cot_h, ad.cot_qx, ad.cot_qy = (t.data_ptr() for t in self.cot)
@@ -140,19 +148,22 @@ class Calibration:
     def _seg_stride(self) -> int:
         return self.segU[0].numel() * self.segU.element_size()
 
-    def _forward(self, mode, k0, n, seg=False, gauges=True):
+    def _forward(self, mode, k0, n, seg=False, gauges=True, bufs=None, stream=None):
         """Steps k0 .. k0+n-1 in one native call (hp_calib_forward) of the
-        single-pass step, with the states and fluxes kept in segU / segD when
-        seg is set.  mode: "record" (own CFL steps, schedule saved), "free"
-        (own CFL steps, nothing saved) or "frozen" (the recorded schedule)."""
+        single-pass step, with the states and fluxes kept in segU / segD (or
+        the pair bufs) when seg is set.  mode: "record" (own CFL steps,
+        schedule saved), "free" (own CFL steps, nothing saved) or "frozen"
+        (the recorded schedule)."""
         s = self.s
+        U, D = bufs if bufs is not None else (self.segU, self.segD)
         check(lib().hp_calib_forward(
             ctypes.byref(s.desc), self.sched.data_ptr() if mode == "record" else None,
             self.frozen.data_ptr() if mode == "frozen" else None, int(k0), int(n),
-            self.segU.data_ptr() if seg else None, self.segD.data_ptr() if seg else None,
+            U.data_ptr() if seg else None, D.data_ptr() if seg else None,
             self._seg_stride() if seg else 0,
             self.gauge_off.data_ptr() if gauges else None, self.G if gauges else 0,
-            self.hist.data_ptr() if gauges else None, _lib.stream_ptr(s.device)), "hp_calib_forward")
+            self.hist.data_ptr() if gauges else None,
+            _lib.stream_ptr(s.device) if stream is None else stream.cuda_stream), "hp_calib_forward")
 
     def run(self, theta, keep_checkpoints=True) -> torch.Tensor:
         """Forward run; returns the FP64 loss (needs obs).  The first run
@@ -210,22 +221,50 @@ class Calibration:
         stream = _lib.stream_ptr(s.device)
         stride = self._seg_stride()
 
-        def reverse(k_lo, k_hi, seg_k0):
+        def reverse(k_lo, k_hi, seg_k0, bufs):
             check(lib().hp_calib_reverse(
                 ctypes.byref(s.desc), ctypes.byref(self.adesc), self.frozen.data_ptr(), int(k_lo), int(k_hi),
-                int(seg_k0), self.segU.data_ptr(), self.segD.data_ptr(), stride, self.gauge_off.data_ptr(), self.G,
+                int(seg_k0), bufs[0].data_ptr(), bufs[1].data_ptr(), stride, self.gauge_off.data_ptr(), self.G,
                 h_rec.data_ptr(), obs.data_ptr(), stream), "hp_calib_reverse")
 
         if self.ck == 1:
-            reverse(0, self.K, 0)
-        else:
+            reverse(0, self.K, 0, self.seg_bufs[0])
+        elif not self.overlap:
             for a in sorted(self.ckpts, reverse=True):
                 b = min(a + self.ck, self.K)
                 # recompute the segment on the frozen schedule: the state
-                # before each step and its flux (the last step's flux only)
+                # before each step and its flux
                 s.U.copy_(self.ckpts[a])
                 self._forward("frozen", a, b - a, seg=True, gauges=False)
-                reverse(a, b, a)
+                reverse(a, b, a, self.seg_bufs[0])
+        else:
+            # segment i is recomputed on the side stream into buffer pair
+            # i % 2 while segment i - 1 is reversed on the main stream
+            main, side = torch.cuda.current_stream(s.device), self.side
+            segs = sorted(self.ckpts, reverse=True)
+            ready = [torch.cuda.Event(), torch.cuda.Event()]  # recomputed into pair j
+            done = [torch.cuda.Event(), torch.cuda.Event()]   # pair j reversed
+
+            def recompute(i):
+                a = segs[i]
+                j = i % 2
+                if i >= 2:
+                    side.wait_event(done[j])
+                with torch.cuda.stream(side):
+                    s.U.copy_(self.ckpts[a])
+                    self._forward("frozen", a, min(a + self.ck, self.K) - a, seg=True, gauges=False,
+                                  bufs=self.seg_bufs[j], stream=side)
+                ready[j].record(side)
+
+            side.wait_stream(main)
+            recompute(0)
+            for i, a in enumerate(segs):
+                if i + 1 < len(segs):
+                    recompute(i + 1)
+                main.wait_event(ready[i % 2])
+                reverse(a, min(a + self.ck, self.K), a, self.seg_bufs[i % 2])
+                done[i % 2].record(main)
+            main.wait_stream(side)
         check(lib().hp_adjoint_finish(ctypes.byref(s.desc), ctypes.byref(self.adesc), stream), "hp_adjoint_finish")
         g = self.grads.sum(dim=0).cpu()
         return torch.cat([g[: self.C], g[_lib.GRAD_STRIDE - 1: _lib.GRAD_STRIDE]])
