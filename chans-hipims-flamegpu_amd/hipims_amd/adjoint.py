@@ -76,6 +76,7 @@ class Calibration:
         self.hist = torch.zeros(self.K + 1, self.G, dtype=s.dtype, device=dev)
         self.obs = None if obs is None else torch.as_tensor(obs, dtype=torch.float64, device=dev)
         self.ckpts: dict[int, torch.Tensor] = {}
+        self.tail = None  # first step of a last segment kept by run() in seg_bufs[0]
         # checkpoint states, allocated once (no allocation inside a run)
         self.ckpt_store = None if self.ck == 1 else torch.zeros(
             ((self.K + self.ck - 1) // self.ck,) + tuple(s.U.shape), dtype=s.dtype, device=dev)
@@ -173,17 +174,25 @@ class Calibration:
         self.ckpts.clear()
         self._record_gauges(0)
         mode = "frozen" if self.sched_ready else "record"
+        self.tail = None
         if keep_checkpoints and self.ck == 1:
             # every state and flux kept: nothing is recomputed in reverse
             self._forward(mode, 0, self.K, seg=True)
             self.ckpts = {0: self.segU[0]}
         else:
+            last = ((self.K - 1) // self.ck) * self.ck
             for a in range(0, self.K, self.ck):
                 if keep_checkpoints:
                     slot = self.ckpt_store[a // self.ck]
                     slot.copy_(self.s.U)
                     self.ckpts[a] = slot
-                self._forward(mode, a, min(self.ck, self.K - a))
+                if keep_checkpoints and a == last:
+                    # the last segment keeps its states and fluxes: the
+                    # reverse sweep starts on it without recomputing it
+                    self._forward(mode, a, self.K - a, seg=True, bufs=self.seg_bufs[0])
+                    self.tail = a
+                else:
+                    self._forward(mode, a, min(self.ck, self.K - a))
         if not self.sched_ready:
             self.frozen[:, 0:2].copy_(self.sched)
             self.sched_ready = True
@@ -220,6 +229,7 @@ class Calibration:
         obs = self.obs.contiguous()
         stream = _lib.stream_ptr(s.device)
         stride = self._seg_stride()
+        tail, self.tail = self.tail, None  # its planes are reused below
 
         def reverse(k_lo, k_hi, seg_k0, bufs):
             check(lib().hp_calib_reverse(
@@ -232,10 +242,11 @@ class Calibration:
         elif not self.overlap:
             for a in sorted(self.ckpts, reverse=True):
                 b = min(a + self.ck, self.K)
-                # recompute the segment on the frozen schedule: the state
-                # before each step and its flux
-                s.U.copy_(self.ckpts[a])
-                self._forward("frozen", a, b - a, seg=True, gauges=False)
+                if a != tail:
+                    # recompute the segment on the frozen schedule: the state
+                    # before each step and its flux
+                    s.U.copy_(self.ckpts[a])
+                    self._forward("frozen", a, b - a, seg=True, gauges=False)
                 reverse(a, b, a, self.seg_bufs[0])
         else:
             # segment i is recomputed on the side stream into buffer pair
@@ -248,6 +259,8 @@ class Calibration:
             def recompute(i):
                 a = segs[i]
                 j = i % 2
+                if a == tail:  # kept by the forward run (in pair 0)
+                    return
                 if i >= 2:
                     side.wait_event(done[j])
                 with torch.cuda.stream(side):
@@ -261,7 +274,8 @@ class Calibration:
             for i, a in enumerate(segs):
                 if i + 1 < len(segs):
                     recompute(i + 1)
-                main.wait_event(ready[i % 2])
+                if a != tail:
+                    main.wait_event(ready[i % 2])
                 reverse(a, min(a + self.ck, self.K), a, self.seg_bufs[i % 2])
                 done[i % 2].record(main)
             main.wait_stream(side)
