@@ -226,6 +226,8 @@ def parse():
                         "horizons; the frozen-branch linearisation of MUSCL with forward Euler grows)")
     p.add_argument("--adj-slopes", choices=["exact", "frozen"], default="exact")
     p.add_argument("--no-fd", action="store_true", help="adjoint: skip the finite-difference check")
+    p.add_argument("--adj-serial", action="store_true",
+                   help="adjoint: recompute segments between the reverse ones instead of beside them")
     return p.parse_args()
 
 
@@ -448,7 +450,7 @@ def adjoint_mode(a):
     torch.cuda.synchronize()
     fwd_s = time.perf_counter() - t0
 
-    cal = Calibration(solver(), gauges, a.adj_steps, a.adj_checkpoint, slopes=a.adj_slopes)
+    cal = Calibration(solver(), gauges, a.adj_steps, a.adj_checkpoint, slopes=a.adj_slopes, overlap=not a.adj_serial)
     th0 = cal.theta0()
     pert = torch.tensor(np.random.Generator(np.random.PCG64(7)).choice([0.8, 1.2], size=th0.numel() - 1))
     th_true = th0.clone()
@@ -456,7 +458,8 @@ def adjoint_mode(a):
     cal.obs = cal.simulate(th_true)
     # untimed warm-up of the same calibration (first launches of the record,
     # recompute and reverse kernels), then the timed one on fresh state
-    warm = Calibration(solver(), gauges, a.adj_steps, a.adj_checkpoint, obs=cal.obs, slopes=a.adj_slopes)
+    warm = Calibration(solver(), gauges, a.adj_steps, a.adj_checkpoint, obs=cal.obs, slopes=a.adj_slopes,
+                       overlap=not a.adj_serial)
     warm.run(th0)
     warm.gradient()
     del warm
@@ -490,6 +493,7 @@ def adjoint_mode(a):
     line = {"mode": "adjoint", "size": a.adj_size, "order": a.adj_order, "slopes": a.adj_slopes, "tile_rows": rows,
             "plain_path": "single-pass step (k_step_fused + k_finalize)",
             "steps": a.adj_steps, "checkpoint": a.adj_checkpoint,
+            "recompute": "serial" if a.adj_serial else "overlapped on a second stream",
             "gauges": int(len(gauges)), "dtype": "f32 state, f64 gradient sums", "loss": float(loss),
             "grad": [float(v) for v in grad], "plain_forward_s": fwd_s, "recorded_forward_s": run_s,
             "reverse_s": rev_s, "cost_ratio": (run_s + rev_s) / fwd_s, "store_all": store_all}
