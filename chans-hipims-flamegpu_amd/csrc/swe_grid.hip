@@ -127,21 +127,32 @@ __device__ inline double wave_sum(double v) {
 // LDS tile of one flux block.
 template <class T, int TY> struct FluxTile {
   static constexpr int LW = kTX + 4, LH = TY + 4;
-  static constexpr int NX = TY * (kTX + 1);  // x-edges
-  static constexpr int NY = (TY + 1) * kTX;  // y-edges
   T h[LH][LW], qx[LH][LW], qy[LH][LW], z[LH][LW];
   uint8_t ty[LH][LW];
-  // Edge records, A = west / north cell, B = east / south cell, fluxes from
-  // A's view (subtract at A, add at B); x-edge (ly, lx) is record
-  // ly*(TX+1)+lx, y-edge (ly, lx) is NX + ly*TX+lx.
+  // Edge record (solve_tile_edge), A = west / north cell, B = east / south
+  // cell, fluxes from A's view (subtract at A, add at B):
   //   double (kRec 5): 0 mass, 1 qx, 2 qy, 3 bed term of A, 4 bed term of B
   //     (the reference's accumulation order: flux terms, then the bed term);
   //   float (kRec 4): 0 mass, 1 momentum across the edge axis, 2 momentum
-  //     along the axis + A's bed term, 3 the same - B's bed term (one fewer
-  //     LDS word per edge, so one more block fits a CU).
+  //     along the axis + A's bed term, 3 the same - B's bed term.
   static constexpr int kRec = sizeof(T) == 4 ? 4 : 5;
-  T e[NX + NY][kRec];
+  // A record splits into the part its A cell reads and the part its B cell
+  // reads (kSide words each: mass, the two momentum terms and that side's
+  // bed term, folded as above).  The thread that owns a cell solves the
+  // cell's west x-edge and north y-edge, where the cell is B: it keeps the B
+  // part in registers and leaves the A part here, under the A cell - the
+  // west neighbour's east-edge part (a[0]) and the north neighbour's
+  // south-edge part (a[1]).
+  static constexpr int kSide = kRec - 1;
+  T a[2][kSide][TY][kTX];
 };
+
+// The part of record e that cell A (side 0) or cell B (side 1) reads.
+template <class T, int kRec>
+__device__ inline T side_word(const T* e, int side, int k) {
+  if (kRec == 5) return k < 3 ? e[k] : e[3 + side];
+  return k < 2 ? e[k] : e[2 + side];
+}
 
 template <class T, int TY>
 __device__ inline FrameCell<T> frame_cell(const FluxTile<T, TY>& s, int r, int c, int d, T zref) {
@@ -188,7 +199,7 @@ __device__ inline void store_record(T* out, T fh, T gx, T gy, T bedA, T bedB) {
 // Solve the edge between LDS cells A and B, where B lies in direction D from
 // A (D = 0: x-edge, B east of A; D = 3: y-edge, B south of A).  A is
 // (ra, ca); L is A's neighbour opposite to B, RR is B's neighbour beyond B.
-// Writes the edge record out[0..4].
+// Writes the edge record out[0 .. kRec-1].
 template <class T, int ORDER, int TY, int D>
 __device__ inline void solve_tile_edge(const FluxTile<T, TY>& s, const hp_grid_desc& a, const SeriesRows& rows,
                                        int ra, int ca, T dx, T dt, T dt_dx, T* out) {
@@ -413,23 +424,46 @@ __device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by, const Pa
 
   // Edges touching the tile: x-edges (row ly, between local columns lx-1 = A
   // and lx = B, lx = 0..64) and y-edges (column lx, between local rows ly-1 =
-  // A and ly = B, ly = 0..TY).  The first TY x 64 of each kind fill whole
-  // passes of the block; the last x-edge column runs in wave 0 and the last
-  // y-edge row in wave 1, so every wave solves edges of one direction only.
-  for (int e = tid; e < TY * kTX; e += kThreads) {
-    const int ly = e >> 6, lx = e & (kTX - 1);
-    solve_tile_edge<T, ORDER, TY, 0>(s, a, rows, ly + 2, lx + 1, dx, dt, dt_dx, s.e[ly * (kTX + 1) + lx]);
+  // A and ly = B, ly = 0..TY).  The owner of cell (ly, lx) solves its west
+  // and north edges, in the same passes for every wave (one direction per
+  // pass); the last x-edge column runs in wave 0 and the last y-edge row in
+  // wave 1.  B parts stay in registers, A parts go to LDS.
+  constexpr int kRec = Tile::kRec, kSide = Tile::kSide;
+  T bw[CPT][kSide], bn[CPT][kSide];  // the own cells' parts of their west and north edges
+#pragma unroll
+  for (int k = 0; k < CPT; ++k) {
+    const int ly = wv + 4 * k, lx = lane;
+    T rec[kRec];
+    solve_tile_edge<T, ORDER, TY, 0>(s, a, rows, ly + 2, lx + 1, dx, dt, dt_dx, rec);
+#pragma unroll
+    for (int w = 0; w < kSide; ++w) {
+      bw[k][w] = side_word<T, kRec>(rec, 1, w);
+      if (lx > 0) s.a[0][w][ly][lx - 1] = side_word<T, kRec>(rec, 0, w);
+    }
   }
-  for (int e = tid; e < TY * kTX; e += kThreads) {
-    const int ly = e >> 6, lx = e & (kTX - 1);
-    solve_tile_edge<T, ORDER, TY, 3>(s, a, rows, ly + 1, lx + 2, dx, dt, dt_dx, s.e[Tile::NX + ly * kTX + lx]);
+#pragma unroll
+  for (int k = 0; k < CPT; ++k) {
+    const int ly = wv + 4 * k, lx = lane;
+    T rec[kRec];
+    solve_tile_edge<T, ORDER, TY, 3>(s, a, rows, ly + 1, lx + 2, dx, dt, dt_dx, rec);
+#pragma unroll
+    for (int w = 0; w < kSide; ++w) {
+      bn[k][w] = side_word<T, kRec>(rec, 1, w);
+      if (ly > 0) s.a[1][w][ly - 1][lx] = side_word<T, kRec>(rec, 0, w);
+    }
   }
   static_assert(TY <= kTX && kThreads >= 2 * kTX, "edge tail layout");
   if (tid < TY) {
-    solve_tile_edge<T, ORDER, TY, 0>(s, a, rows, tid + 2, kTX + 1, dx, dt, dt_dx, s.e[tid * (kTX + 1) + kTX]);
+    T rec[kRec];
+    solve_tile_edge<T, ORDER, TY, 0>(s, a, rows, tid + 2, kTX + 1, dx, dt, dt_dx, rec);
+#pragma unroll
+    for (int w = 0; w < kSide; ++w) s.a[0][w][tid][kTX - 1] = side_word<T, kRec>(rec, 0, w);
   } else if (tid >= kTX && tid < 2 * kTX) {
     const int lx = tid - kTX;
-    solve_tile_edge<T, ORDER, TY, 3>(s, a, rows, TY + 1, lx + 2, dx, dt, dt_dx, s.e[Tile::NX + TY * kTX + lx]);
+    T rec[kRec];
+    solve_tile_edge<T, ORDER, TY, 3>(s, a, rows, TY + 1, lx + 2, dx, dt, dt_dx, rec);
+#pragma unroll
+    for (int w = 0; w < kSide; ++w) s.a[1][w][TY - 1][lx] = side_word<T, kRec>(rec, 0, w);
   }
   __syncthreads();
 
@@ -464,20 +498,24 @@ __device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by, const Pa
         listed = w_e | w_n | w_w | w_s;
       }
       if (listed) {
-        const T* eE = s.e[ly * (kTX + 1) + lx + 1];         // this cell is A
-        const T* eN = s.e[Tile::NX + ly * kTX + lx];        // this cell is B
-        const T* eW = s.e[ly * (kTX + 1) + lx];             // this cell is B
-        const T* eS = s.e[Tile::NX + (ly + 1) * kTX + lx];  // this cell is A
+        T eE[kSide], eS[kSide];  // this cell is A of its east and south edges
+#pragma unroll
+        for (int w = 0; w < kSide; ++w) {
+          eE[w] = s.a[0][w][ly][lx];
+          eS[w] = s.a[1][w][ly][lx];
+        }
+        const T* eN = bn[k];  // B of its north edge
+        const T* eW = bw[k];  // B of its west edge
         // the reference's order: E, N, W, S; per edge HLLC terms then bed term
-        if (Tile::kRec == 5) {
+        if (kRec == 5) {
           ah -= eE[0]; aqx -= eE[1]; aqx -= eE[3]; aqy -= eE[2];
-          ah -= -eN[0]; aqx -= -eN[1]; aqy -= -eN[2]; aqy -= eN[4];
-          ah -= -eW[0]; aqx -= -eW[1]; aqx -= eW[4]; aqy -= -eW[2];
+          ah -= -eN[0]; aqx -= -eN[1]; aqy -= -eN[2]; aqy -= eN[3];
+          ah -= -eW[0]; aqx -= -eW[1]; aqx -= eW[3]; aqy -= -eW[2];
           ah -= eS[0]; aqx -= eS[1]; aqy -= eS[2]; aqy -= eS[3];
         } else {
           ah -= eE[0]; aqx -= eE[2]; aqy -= eE[1];
-          ah += eN[0]; aqx += eN[1]; aqy += eN[3];
-          ah += eW[0]; aqx += eW[3]; aqy += eW[1];
+          ah += eN[0]; aqx += eN[1]; aqy += eN[2];
+          ah += eW[0]; aqx += eW[2]; aqy += eW[1];
           ah -= eS[0]; aqx -= eS[1]; aqy -= eS[2];
         }
       }
