@@ -334,7 +334,6 @@ template <class T, int ORDER, int TY, bool FUSED>
 __device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by, const Partials& part, int64_t pidx,
                                  bool save_flux = false) {
   __shared__ FluxTile<T, TY> s;
-  __shared__ T s_man[FUSED ? 256 : 1];
   __shared__ T s_rate[FUSED ? kRateCache : 1];
   using Tile = FluxTile<T, TY>;
   constexpr int LH = Tile::LH;
@@ -359,7 +358,6 @@ __device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by, const Pa
   CellIn<T> cin[FUSED ? CPT : 1];
   RainRows rr{};
   if (FUSED) {
-    for (int k = tid; k < a.n_classes; k += kThreads) s_man[k] = ((const T*)a.manning)[k];
     rr = rain_rows(a, a.ctrl->t);
     if (a.rain_mode == HP_RAIN_STATION && tid < kRateCache) s_rate[tid] = station_rate<T>(rr, tid, a.rain_scale);
     const int gx = x0 + lane;
@@ -372,6 +370,9 @@ __device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by, const Pa
         const int cls = a.landuse[gi];
         cin[k].on = cls != 255;
         cin[k].cls = cls;
+        // the class's Manning value from the (cached) table: no LDS copy, so
+        // that six blocks fit a CU
+        if (cin[k].on) cin[k].n = ((const T*)a.manning)[cls];
         if (a.rain_mode == HP_RAIN_STATION) cin[k].st = a.station ? (int)a.station[gi] : 0;
         else if (a.rain_mode == HP_RAIN_FIELD) cin[k].rain = T(a.rain_field[gi]) * T(a.rain_scale);
         cin[k].hmax = ((const T*)a.h_max)[gi];
@@ -532,7 +533,6 @@ __device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by, const Pa
         dqy[gi] = aqy;
       }
       CellIn<T> in = cin[k];
-      in.n = s_man[in.cls];
       if (a.rain_mode == HP_RAIN_STATION)
         in.rain = (in.st >= 0 && in.st < kRateCache) ? s_rate[in.st] : station_rate<T>(rr, in.st, a.rain_scale);
       T hn = s.h[r][c], qxn = s.qx[r][c], qyn = s.qy[r][c], hm = in.hmax;
