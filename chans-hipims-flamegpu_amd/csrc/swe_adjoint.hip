@@ -480,7 +480,7 @@ template <class T> constexpr int adj_tile_rows() { return sizeof(T) == 4 ? 8 : 4
 constexpr int kATileThreads = 256;
 
 template <class T, int TY, int NT>
-__global__ __launch_bounds__(NT) void k_adj_step_tile(hp_grid_desc a, AdjPlanes<T> p, AdjLam<T> lam,
+__global__ __launch_bounds__(NT, sizeof(T) == 4 ? 6 : 1) void k_adj_step_tile(hp_grid_desc a, AdjPlanes<T> p, AdjLam<T> lam,
                                                              double* grads, double* gn, int ntx) {
   static_assert(TY % (NT / kATX) == 0, "a thread owns whole rows of the tile");
   static_assert(2 * kATX + 2 * TY <= NT, "one ring cell per thread");
@@ -514,32 +514,55 @@ __global__ __launch_bounds__(NT) void k_adj_step_tile(hp_grid_desc a, AdjPlanes<
   const T scale = T(a.rain_scale);
   double rsum = 0.0;
 
-  // (1) update adjoint of element (x, y): the cotangent before the step into
-  // lo, the flux-increment cotangent into LDS (zero outside the plane)
-  auto cell = [&](int x, int y, bool owned, T(&lo)[3]) {
+  // (1) update adjoint of the tile's cells and of its ring: first every
+  // load of the thread's cells (so their memory latencies overlap), then per
+  // cell the cotangent before the step into lo and the flux-increment
+  // cotangent into LDS (zero outside the plane)
+  struct CellLd {
+    T lh, lqx, lqy, h0, qx, qy, dh, dqx, dqy;
+    float f;
+    int cls, st, g;
+    double gn;
+    bool in;
+  };
+  auto load = [&](int x, int y, bool owned, CellLd& L) {
+    L.in = x >= 0 && x < P && y >= 0 && y < ny;
+    if (!L.in) return;
+    const int64_t i = (int64_t)y * P + x;
+    L.lh = lam.in[0][i];
+    L.lqx = lam.in[1][i];
+    L.lqy = lam.in[2][i];
+    L.g = p.gauge_of ? p.gauge_of[i] : -1;
+    L.cls = a.landuse[i];
+    L.h0 = h[i];
+    L.qx = qx[i];
+    L.qy = qy[i];
+    L.dh = dh[i];
+    L.dqx = dqx[i];
+    L.dqy = dqy[i];
+    L.st = (mode == HP_RAIN_STATION && a.station) ? (int)a.station[i] : 0;
+    L.f = mode == HP_RAIN_FIELD ? a.rain_field[i] : 0.0f;
+    L.gn = owned ? gn[i] : 0.0;
+  };
+  auto finish = [&](int x, int y, bool owned, const CellLd& L, T(&lo)[3]) {
     T c0 = T(0), c1 = T(0), c2 = T(0);
     lo[0] = lo[1] = lo[2] = T(0);
-    if (x >= 0 && x < P && y >= 0 && y < ny) {
+    if (L.in) {
       const int64_t i = (int64_t)y * P + x;
-      T lh = lam.in[0][i], lqx = lam.in[1][i], lqy = lam.in[2][i];
-      if (p.gauge_of) {  // the loss cotangent of the depth after the step
-        const int g = p.gauge_of[i];
-        if (g >= 0) lh += T(2.0 * (p.loss_h[g] - p.loss_obs[g]));
-      }
+      T lh = L.lh, lqx = L.lqx, lqy = L.lqy;
+      if (L.g >= 0) lh += T(2.0 * (p.loss_h[L.g] - p.loss_obs[L.g]));  // the loss cotangent of the depth after the step
       T rb = T(0), rs = T(0);
       if (mode == HP_RAIN_STATION) {
-        const int st = a.station ? a.station[i] : 0;
-        const bool cached = st >= 0 && st < kARateCache;
-        rb = cached ? s_rb[st] : station_rate<T>(rr, st, 1.0);
-        rs = cached ? s_rs[st] : station_rate<T>(rr, st, a.rain_scale);
+        const bool cached = L.st >= 0 && L.st < kARateCache;
+        rb = cached ? s_rb[L.st] : station_rate<T>(rr, L.st, 1.0);
+        rs = cached ? s_rs[L.st] : station_rate<T>(rr, L.st, a.rain_scale);
       } else if (mode == HP_RAIN_FIELD) {
-        const float f = a.rain_field[i];
-        rb = T(f);
-        rs = T(f) * scale;
+        rb = T(L.f);
+        rs = T(L.f) * scale;
       }
-      double g_acc = owned ? gn[i] : 0.0, r_acc = 0.0;
-      if (adj_update_cell<T>(a.landuse[i], h[i], qx[i], qy[i], dh[i], dqx[i], dqy[i], rb, rs, dt, s_man, lh, lqx, lqy,
-                             g_acc, r_acc) &&
+      double g_acc = L.gn, r_acc = 0.0;
+      if (adj_update_cell<T>(L.cls, L.h0, L.qx, L.qy, L.dh, L.dqx, L.dqy, rb, rs, dt, s_man, lh, lqx, lqy, g_acc,
+                             r_acc) &&
           flux_listed<T>(a, h, x, y, t_T)) {
         c0 = lh;
         c1 = lqx;
@@ -559,17 +582,24 @@ __global__ __launch_bounds__(NT) void k_adj_step_tile(hp_grid_desc a, AdjPlanes<
     s.cot[2][ly][lx] = c2;
   };
   const int lx = tid % kATX, lrow = tid / kATX;
+  // the ring: row below, row above, column left, column right
+  const bool ring = tid < 2 * kATX + 2 * TY;
+  int hx = 0, hy = 0;
+  if (tid < kATX) { hx = x0 + tid; hy = y0 - 1; }
+  else if (tid < 2 * kATX) { hx = x0 + tid - kATX; hy = y0 + TY; }
+  else if (tid < 2 * kATX + TY) { hx = x0 - 1; hy = y0 + tid - 2 * kATX; }
+  else { hx = x0 + kATX; hy = y0 + tid - 2 * kATX - TY; }
+  CellLd own[kRowsPer], rc;
+#pragma unroll
+  for (int k = 0; k < kRowsPer; ++k) load(x0 + lx, y0 + lrow + k * (NT / kATX), true, own[k]);
+  rc.in = false;
+  if (ring) load(hx, hy, false, rc);
   T lp[kRowsPer][3];
 #pragma unroll
-  for (int k = 0; k < kRowsPer; ++k) cell(x0 + lx, y0 + lrow + k * (NT / kATX), true, lp[k]);
-  if (tid < 2 * kATX + 2 * TY) {  // the ring: row below, row above, column left, column right
-    int hx, hy;
-    if (tid < kATX) { hx = x0 + tid; hy = y0 - 1; }
-    else if (tid < 2 * kATX) { hx = x0 + tid - kATX; hy = y0 + TY; }
-    else if (tid < 2 * kATX + TY) { hx = x0 - 1; hy = y0 + tid - 2 * kATX; }
-    else { hx = x0 + kATX; hy = y0 + tid - 2 * kATX - TY; }
+  for (int k = 0; k < kRowsPer; ++k) finish(x0 + lx, y0 + lrow + k * (NT / kATX), true, own[k], lp[k]);
+  if (ring) {
     T unused[3];
-    cell(hx, hy, false, unused);
+    finish(hx, hy, false, rc, unused);
   }
   __syncthreads();
 
