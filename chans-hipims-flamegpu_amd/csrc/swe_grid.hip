@@ -847,12 +847,21 @@ __global__ __launch_bounds__(1024) void k_finalize(hp_ctrl* ctrl, void* work, in
 }
 
 // k_finalize followed by a calibration step's record (thread 0 wrote the
-// control block, so it also reads the next step's t and dt).
+// control block, so it also reads the next step's t and dt).  The gauge
+// depths (one per thread; more are read after the fold) are loaded before
+// the fold so that their latency hides behind it.
 __global__ __launch_bounds__(1024) void k_finalize_record(hp_ctrl* ctrl, void* work, int n, double dx, int sums,
                                                           StepRecord rec) {
   const Partials P(work, n);
+  const int i0 = threadIdx.x;
+  double v0 = 0.0;
+  if (i0 < rec.n) v0 = rec.f32 ? (double)((const float*)rec.h)[rec.off[i0]] : ((const double*)rec.h)[rec.off[i0]];
   finalize_domain(ctrl, P.mins, P.sums, n, dx, sums != 0);
-  for (int i = threadIdx.x; i < rec.n; i += blockDim.x) {
+  if (i0 < rec.n) {
+    if (rec.f32) ((float*)rec.row)[i0] = (float)v0;
+    else ((double*)rec.row)[i0] = v0;
+  }
+  for (int i = i0 + blockDim.x; i < rec.n; i += blockDim.x) {
     if (rec.f32) ((float*)rec.row)[i] = ((const float*)rec.h)[rec.off[i]];
     else ((double*)rec.row)[i] = ((const double*)rec.h)[rec.off[i]];
   }
