@@ -340,7 +340,22 @@ __device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by, const Pa
   constexpr int CPT = TY * kTX / kThreads;  // own cells per thread
   static_assert(TY * kTX % kThreads == 0, "tile cells per thread");
   if (stopped(a.ctrl)) {
-    if (FUSED && threadIdx.x == 0) part.put(pidx, INFINITY, 0.0, 0.0, 0.0, a.mass_sums != 0);
+    if (FUSED) {
+      if (threadIdx.x == 0) part.put(pidx, INFINITY, 0.0, 0.0, 0.0, a.mass_sums != 0);
+      // the state passes through to the output planes as well, so that a
+      // caller alternating the two sets of planes keeps the final state
+      const int gx = bx * kTX + (int)(threadIdx.x & (kTX - 1));
+#pragma unroll
+      for (int k = 0; k < CPT; ++k) {
+        const int gy = by * TY + (int)(threadIdx.x / kTX) + (kThreads / kTX) * k;
+        if (gx < a.nx && gy < a.ny) {
+          const int64_t gi = (int64_t)gy * a.pitch + gx;
+          ((T*)a.h_out)[gi] = ((const T*)a.h)[gi];
+          ((T*)a.qx_out)[gi] = ((const T*)a.qx)[gi];
+          ((T*)a.qy_out)[gi] = ((const T*)a.qy)[gi];
+        }
+      }
+    }
     return;
   }
   const int tid = threadIdx.x;

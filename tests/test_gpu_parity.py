@@ -383,6 +383,38 @@ def test_fused_step_matches_two_pass_bitwise(cuda, dtype, order):
     assert abs(ca["volume"] - cb["volume"]) <= 1e-12 * ca["volume"]
 
 
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_fused_step_past_stop_time_keeps_final_state(cuda, dtype):
+    """Steps past a domain's stop time leave it unchanged on both paths: the
+    single-pass step, which alternates two sets of planes, passes the stopped
+    state through to the other set (31 steps in calls of 14 and 17, the stop
+    reached after about a third of them)."""
+    say(f"[fused step past the stop time, {dtype}]")
+    c = gen.rainfall_runoff(100, 5.0, rain_mmh=120.0, relief=6.0)
+
+    def mk():
+        g = GridSolver(c.mask, c.h, c.qx, c.qy, c.z, c.dx, cfl=0.5, order=2, dtype=dtype, first_dt=0.5,
+                       manning=c.manning, landuse=c.landuse)
+        g.set_station_rain(c.rain_table)
+        return g
+    probe = mk()
+    probe.step(10)
+    t_stop = probe.ctrl_dict()["t"]
+    a, b = mk(), mk()
+    a.set_t_stop(t_stop)
+    b.set_t_stop(t_stop)
+    a.step(31)
+    b.step_fused(14)
+    b.step_fused(17)
+    torch.cuda.synchronize()
+    ca, cb = a.ctrl_dict(), b.ctrl_dict()
+    say(f"  t_stop {t_stop:.3f} s, steps {ca['steps']:.0f} / {cb['steps']:.0f}")
+    assert ca["steps"] == cb["steps"] == 10 and ca["t"] == cb["t"] == t_stop
+    for name in ("h", "qx", "qy", "h_max"):
+        assert torch.equal(getattr(a, name), getattr(b, name)), name
+        assert torch.equal(getattr(a, name), getattr(probe, name)), name
+
+
 @pytest.mark.parametrize("path", ["flux", "fused"])
 def test_autotune_restores_state_and_tile_height_is_neutral(cuda, path):
     """The tuner leaves the state as it found it, and every legal tile height
