@@ -216,7 +216,8 @@ def parse():
     p.add_argument("--copy-mib", type=int, default=1024)
     p.add_argument("--scenarios", type=int, default=64)
     p.add_argument("--sim-seconds", type=float, default=900.0, help="ensemble: simulated time of every scenario")
-    p.add_argument("--replay-steps", type=int, default=50, help="ensemble: steps per graph replay")
+    p.add_argument("--replay-steps", type=int, default=50, help="ensemble: steps per graph replay (even)")
+    p.add_argument("--ens-two-pass", action="store_true", help="ensemble: flux and update passes per step")
     p.add_argument("--adj-size", type=int, default=1024)
     p.add_argument("--adj-steps", type=int, default=2000)
     p.add_argument("--adj-checkpoint", type=int, default=100)
@@ -324,7 +325,7 @@ def ensemble_mode(a):
         s.set_station_rain(c.rain_table)
         return s
 
-    ens = Ensemble([solver(c) for c in cases], t_stop=a.sim_seconds)
+    ens = Ensemble([solver(c) for c in cases], t_stop=a.sim_seconds, single_pass=not a.ens_two_pass)
     ens.capture(a.replay_steps)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -354,6 +355,7 @@ def ensemble_mode(a):
     seq_s, seq_steps = seq["godunov"]
     line = {"mode": "ensemble", "scenarios": len(cases), "sizes": [c.mask.shape[0] - 2 for c in cases],
             "sim_seconds": a.sim_seconds, "dtype": "f32", "order": 2, "steps_per_replay": a.replay_steps,
+            "step": "two-pass" if a.ens_two_pass else "single-pass",
             "cell_updates": work, "ensemble_s": ens_s, "ensemble_cell_updates_per_s": work / ens_s,
             "steps_min": min(steps), "steps_max": max(steps), "generation_s": gen_s,
             "baselines": {name: {"seconds": v[0], "cell_updates_per_s": work / v[0], "speedup": v[0] / ens_s,

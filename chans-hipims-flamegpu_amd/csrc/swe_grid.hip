@@ -590,11 +590,27 @@ __global__ __launch_bounds__(kThreads) void k_flux_multi(const hp_grid_desc* __r
   flux_tile<T, ORDER, TY, false>(a, m.y % ntx, m.y / ntx, Partials(nullptr, 0), 0);
 }
 
+// Single-pass step of every tile of every domain of an ensemble; map[i] =
+// (domain, tile), listed domain by domain, so the partial records of a domain
+// are one contiguous range.
+template <class T, int ORDER, int TY>
+__global__ __launch_bounds__(kThreads) void k_step_fused_multi(const hp_grid_desc* __restrict__ doms,
+                                                               const int2* __restrict__ map, void* parts,
+                                                               int n_total) {
+  const int slot = xcd_band(blockIdx.x, gridDim.x);
+  const int2 m = map[slot];
+  const hp_grid_desc& a = doms[m.x];
+  const int ntx = (a.nx + kTX - 1) / kTX;
+  flux_tile<T, ORDER, TY, true>(a, m.y % ntx, m.y / ntx, Partials(parts, n_total), slot);
+}
+
 // Single-pass step: reads (h, qx, qy), writes (h_out, qx_out, qy_out).
 template <class T, int ORDER, int TY>
 __global__ __launch_bounds__(kThreads) void k_step_fused(hp_grid_desc a, int ntx, int save_flux) {
+  // the partial record goes to the tile's own slot (tile order, as in an
+  // ensemble), not the block's
   const int tile = xcd_band(blockIdx.x, gridDim.x);
-  flux_tile<T, ORDER, TY, true>(a, tile % ntx, tile / ntx, Partials(a.work, gridDim.x), blockIdx.x, save_flux != 0);
+  flux_tile<T, ORDER, TY, true>(a, tile % ntx, tile / ntx, Partials(a.work, gridDim.x), tile, save_flux != 0);
 }
 
 // Infiltration pass over plane elements: a valid cell listed by the flux
@@ -886,7 +902,9 @@ __global__ __launch_bounds__(1024) void k_finalize_record(hp_ctrl* ctrl, void* w
   }
 }
 
-__global__ __launch_bounds__(256) void k_finalize_multi(const hp_grid_desc* __restrict__ doms,
+// One block per domain, with k_finalize's block size, so that a domain's
+// partials are folded in the same order as in a standalone run.
+__global__ __launch_bounds__(1024) void k_finalize_multi(const hp_grid_desc* __restrict__ doms,
                                                         const int2* __restrict__ ranges, void* parts,
                                                         int n_total) {
   const int2 r = ranges[blockIdx.x];
@@ -1043,8 +1061,12 @@ static int launch_update(const hp_grid_desc* g, int mode, hipStream_t st) {
 }
 
 // ---- ensemble ---------------------------------------------------------------
+// Workspace of an ensemble: the descriptors twice (the second copy with the
+// two sets of state planes exchanged, for the odd steps of the single-pass
+// step), the (domain, tile) and (domain, chunk) maps, the per-domain ranges of
+// partial records for the two-pass and the single-pass step, the records.
 struct EnsembleLayout {
-  int64_t off_doms, off_fmap, off_umap, off_fin, off_parts, total;
+  int64_t off_doms, off_fmap, off_umap, off_fin, off_fin1, off_parts, total;
   int64_t n_ftiles, n_ublocks;
 };
 
@@ -1056,11 +1078,12 @@ static EnsembleLayout ensemble_layout(int32_t n, const hp_grid_desc* d) {
     L.n_ublocks += update_blocks(&d[k]);
   }
   L.off_doms = 0;
-  L.off_fmap = align256((int64_t)n * (int64_t)sizeof(hp_grid_desc));
+  L.off_fmap = align256(2 * (int64_t)n * (int64_t)sizeof(hp_grid_desc));
   L.off_umap = L.off_fmap + align256(L.n_ftiles * 8);
   L.off_fin = L.off_umap + align256(L.n_ublocks * 8);
-  L.off_parts = L.off_fin + align256((int64_t)n * 8);
-  L.total = L.off_parts + align256(partials_bytes(L.n_ublocks));
+  L.off_fin1 = L.off_fin + align256((int64_t)n * 8);
+  L.off_parts = L.off_fin1 + align256((int64_t)n * 8);
+  L.total = L.off_parts + align256(partials_bytes(L.n_ublocks > L.n_ftiles ? L.n_ublocks : L.n_ftiles));
   return L;
 }
 
@@ -1074,6 +1097,34 @@ template <class T>
 static void launch_update_multi(const hp_ensemble* e, hipStream_t st) {
   hipLaunchKernelGGL((k_update_multi<T, HP_UPDATE_STEP>), dim3((unsigned)e->n_update_blocks), dim3(kThreads), 0, st,
                      (const hp_grid_desc*)e->doms, (const int2*)e->update_map, e->partials);
+}
+
+template <class T, int ORDER, int TY>
+static void launch_fused_multi(const hp_ensemble* e, int parity, hipStream_t st) {
+  hipLaunchKernelGGL((k_step_fused_multi<T, ORDER, TY>), dim3((unsigned)e->n_flux_tiles), dim3(kThreads), 0, st,
+                     (const hp_grid_desc*)e->doms + (parity ? e->n_domains : 0), (const int2*)e->flux_map,
+                     e->partials, (int)e->n_flux_tiles);
+}
+
+// One single-pass step of every domain: even steps read the domains' own
+// planes and write their second set, odd steps the reverse.
+static int ensemble_fused_step_once(const hp_ensemble* e, int parity, hipStream_t st) {
+  const int ty = e->tile_rows;
+  if (e->dtype == HP_F32) {
+    if (e->order == 2)
+      (ty == 4 ? launch_fused_multi<float, 2, 4> : ty == 16 ? launch_fused_multi<float, 2, 16> : launch_fused_multi<float, 2, 8>)(e, parity, st);
+    else
+      (ty == 4 ? launch_fused_multi<float, 1, 4> : ty == 16 ? launch_fused_multi<float, 1, 16> : launch_fused_multi<float, 1, 8>)(e, parity, st);
+  } else {
+    if (e->order == 2) launch_fused_multi<double, 2, 4>(e, parity, st);
+    else launch_fused_multi<double, 1, 4>(e, parity, st);
+  }
+  int rc = check_launch("k_step_fused_multi");
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_finalize_multi, dim3((unsigned)e->n_domains), dim3(1024), 0, st,
+                     (const hp_grid_desc*)e->doms, (const int2*)e->fused_finalize_map, e->partials,
+                     (int)e->n_flux_tiles);
+  return check_launch("k_finalize_multi");
 }
 
 static int ensemble_step_once(const hp_ensemble* e, hipStream_t st) {
@@ -1094,7 +1145,7 @@ static int ensemble_step_once(const hp_ensemble* e, hipStream_t st) {
   else launch_update_multi<double>(e, st);
   rc = check_launch("k_update_multi");
   if (rc) return rc;
-  hipLaunchKernelGGL(k_finalize_multi, dim3((unsigned)e->n_domains), dim3(256), 0, st,
+  hipLaunchKernelGGL(k_finalize_multi, dim3((unsigned)e->n_domains), dim3(1024), 0, st,
                      (const hp_grid_desc*)e->doms, (const int2*)e->finalize_map, e->partials,
                      (int)e->n_update_blocks);
   return check_launch("k_finalize_multi");
@@ -1220,13 +1271,16 @@ int64_t hp_ensemble_workspace_bytes(int32_t n, const hp_grid_desc* descs) {
   return ensemble_layout(n, descs).total;
 }
 
-int hp_ensemble_init(int32_t n, const hp_grid_desc* descs, void* work, int64_t work_bytes, hp_ensemble* out,
-                     void* stream) {
+int hp_ensemble_init(int32_t n, const hp_grid_desc* descs, int32_t single_pass, void* work, int64_t work_bytes,
+                     hp_ensemble* out, void* stream) {
   if (n <= 0 || !descs || !out) return fail(HP_EINVAL, "need n > 0 descriptors and an output handle");
+  if (single_pass != 0 && single_pass != 1) return fail(HP_EINVAL, "single_pass must be 0 or 1");
   for (int k = 0; k < n; ++k) {
     if (descs[k].infil_ks) return fail(HP_EINVAL, "infiltration is not supported in an ensemble");
     int rc = check_desc(&descs[k]);
     if (rc) return fail(rc, "domain " + std::to_string(k) + ": " + last_error_slot());
+    if (single_pass && (!descs[k].h_out || !descs[k].qx_out || !descs[k].qy_out))
+      return fail(HP_EINVAL, "domain " + std::to_string(k) + ": the single-pass step needs h_out, qx_out, qy_out");
     if (descs[k].dtype != descs[0].dtype || descs[k].order != descs[0].order ||
         flux_rows(&descs[k]) != flux_rows(&descs[0]))
       return fail(HP_EINVAL, "every domain of an ensemble needs the same dtype, order and tile_rows");
@@ -1236,12 +1290,22 @@ int hp_ensemble_init(int32_t n, const hp_grid_desc* descs, void* work, int64_t w
   if (L.n_ftiles > 0x7fffffffLL || L.n_ublocks > 0x7fffffffLL) return fail(HP_EINVAL, "ensemble too large");
   std::vector<uint8_t> host((size_t)L.off_parts, 0);
   std::memcpy(host.data() + L.off_doms, descs, sizeof(hp_grid_desc) * (size_t)n);
+  hp_grid_desc* swapped = (hp_grid_desc*)(host.data() + L.off_doms) + n;
+  for (int k = 0; k < n; ++k) {
+    swapped[k] = descs[k];
+    std::swap(swapped[k].h, swapped[k].h_out);
+    std::swap(swapped[k].qx, swapped[k].qx_out);
+    std::swap(swapped[k].qy, swapped[k].qy_out);
+  }
   int32_t* fm = (int32_t*)(host.data() + L.off_fmap);
   int32_t* um = (int32_t*)(host.data() + L.off_umap);
   int32_t* fin = (int32_t*)(host.data() + L.off_fin);
+  int32_t* fin1 = (int32_t*)(host.data() + L.off_fin1);
   int64_t fi = 0, ui = 0;
   for (int k = 0; k < n; ++k) {
     const int64_t nt = flux_tiles(&descs[k]), nu = update_blocks(&descs[k]);
+    fin1[2 * k] = (int32_t)fi;
+    fin1[2 * k + 1] = (int32_t)nt;
     for (int64_t j = 0; j < nt; ++j, ++fi) {
       fm[2 * fi] = k;
       fm[2 * fi + 1] = (int32_t)j;
@@ -1268,12 +1332,24 @@ int hp_ensemble_init(int32_t n, const hp_grid_desc* descs, void* work, int64_t w
   out->update_map = w + L.off_umap;
   out->finalize_map = w + L.off_fin;
   out->partials = w + L.off_parts;
+  out->single_pass = single_pass;
+  out->fused_finalize_map = w + L.off_fin1;
   return HP_OK;
 }
 
 int hp_ensemble_step(const hp_ensemble* e, int32_t nsteps, void* stream) {
   if (!e || e->n_domains <= 0 || !e->doms) return fail(HP_EINVAL, "ensemble handle not initialised");
   if (nsteps < 0) return fail(HP_EINVAL, "nsteps must be >= 0");
+  if (e->single_pass) {
+    if (nsteps & 1)
+      return fail(HP_EINVAL, "a single-pass ensemble advances an even number of steps per call (the state then "
+                             "ends in the domains' own planes)");
+    for (int32_t k = 0; k < nsteps; ++k) {
+      int rc = ensemble_fused_step_once(e, k & 1, (hipStream_t)stream);
+      if (rc) return rc;
+    }
+    return HP_OK;
+  }
   for (int32_t k = 0; k < nsteps; ++k) {
     int rc = ensemble_step_once(e, (hipStream_t)stream);
     if (rc) return rc;

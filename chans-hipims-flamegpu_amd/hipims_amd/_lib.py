@@ -22,7 +22,7 @@ HP_OK, HP_EINVAL, HP_ELAUNCH, HP_ESPACE = 0, -1, -2, -3
 HP_F32, HP_F64 = 0, 1
 HP_RAIN_NONE, HP_RAIN_STATION, HP_RAIN_FIELD = 0, 1, 2
 HP_UPDATE_STEP, HP_UPDATE_FRICTION, HP_UPDATE_EULER, HP_UPDATE_CFL = 0, 1, 2, 3
-ABI_VERSION = 8
+ABI_VERSION = 9
 
 _P = ctypes.c_void_p
 _I64 = ctypes.c_int64
@@ -65,7 +65,8 @@ class HpGridDesc(ctypes.Structure):
 class HpEnsemble(ctypes.Structure):
     _fields_ = [("n_domains", _I32), ("dtype", _I32), ("order", _I32), ("tile_rows", _I32),
                 ("n_flux_tiles", _I32), ("n_update_blocks", _I32),
-                ("doms", _P), ("flux_map", _P), ("update_map", _P), ("finalize_map", _P), ("partials", _P)]
+                ("doms", _P), ("flux_map", _P), ("update_map", _P), ("finalize_map", _P), ("partials", _P),
+                ("single_pass", _I32), ("fused_finalize_map", _P)]
 
 
 class HpAdjointDesc(ctypes.Structure):
@@ -99,7 +100,7 @@ SIGNATURES = {
     "hp_grid_step": (_I, [ctypes.POINTER(HpGridDesc), _I32, _P]),
     "hp_grid_step_fused": (_I, [ctypes.POINTER(HpGridDesc), _I32, _P]),
     "hp_ensemble_workspace_bytes": (_I64, [_I32, ctypes.POINTER(HpGridDesc)]),
-    "hp_ensemble_init": (_I, [_I32, ctypes.POINTER(HpGridDesc), _P, _I64, ctypes.POINTER(HpEnsemble), _P]),
+    "hp_ensemble_init": (_I, [_I32, ctypes.POINTER(HpGridDesc), _I32, _P, _I64, ctypes.POINTER(HpEnsemble), _P]),
     "hp_ensemble_step": (_I, [ctypes.POINTER(HpEnsemble), _I32, _P]),
     "hp_adjoint_grad_blocks": (_I64, [ctypes.POINTER(HpGridDesc)]),
     "hp_adjoint_step": (_I, [ctypes.POINTER(HpGridDesc), ctypes.POINTER(HpAdjointDesc), _P]),

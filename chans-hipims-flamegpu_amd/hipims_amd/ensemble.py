@@ -3,9 +3,11 @@
 The reference runs one domain per process and steps it from Python with
 several host synchronisations per step (SWE_CUDA.py:780-831).  Here every
 scenario keeps its own planes and device control block (its own dt, t, export
-clamp and stop time), and one set of three launches - flux of every tile of
-every scenario, update of every chunk of every scenario, time control of every
-scenario - advances all of them by one step each.  K such steps are captured
+clamp and stop time), and one set of launches advances all of them by one
+step each: by default two - the single-pass step of every tile of every
+scenario (alternating each scenario's two sets of state planes), time
+control of every scenario - or, with single_pass=False, three (flux of every
+tile, update of every chunk, time control).  K such steps are captured
 once in a HIP graph and replayed; a scenario that has reached its stop time is
 skipped by every pass, so the ensemble runs until the slowest scenario is done
 with one host check per replay.
@@ -22,7 +24,10 @@ from .grid import GridSolver
 
 
 class Ensemble:
-    def __init__(self, solvers: list[GridSolver], t_stop: float | None = None):
+    def __init__(self, solvers: list[GridSolver], t_stop: float | None = None, single_pass: bool = True):
+        """single_pass: advance with the single-pass step (every step count
+        given to step(), capture() and run() must then be even, so that the
+        state ends in the scenarios' own planes)."""
         if not solvers:
             raise ValueError("empty ensemble")
         dev = solvers[0].device
@@ -30,6 +35,10 @@ class Ensemble:
             raise ValueError("all domains of an ensemble must live on one device")
         self.solvers = solvers
         self.device = dev
+        self.single_pass = bool(single_pass)
+        if self.single_pass:
+            for s in solvers:
+                s.ensure_out()
         if t_stop is not None:
             for s in solvers:
                 s.set_t_stop(t_stop)
@@ -43,7 +52,7 @@ class Ensemble:
         need = int(lib().hp_ensemble_workspace_bytes(n, self._descs))
         self.work = torch.empty(max(need, 256), dtype=torch.uint8, device=self.device)
         self.handle = HpEnsemble()
-        check(lib().hp_ensemble_init(n, self._descs, self.work.data_ptr(), self.work.numel(),
+        check(lib().hp_ensemble_init(n, self._descs, int(self.single_pass), self.work.data_ptr(), self.work.numel(),
                                      ctypes.byref(self.handle), _lib.stream_ptr(self.device)), "hp_ensemble_init")
         self.graph = None
         self.graph_steps = 0

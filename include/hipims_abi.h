@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define HP_ABI_VERSION 8
+#define HP_ABI_VERSION 9
 
 enum hp_status {
   HP_OK = 0,
@@ -233,17 +233,23 @@ typedef struct hp_ensemble {
   const void* update_map;   /* device int32[2 * n_update_blocks]       */
   const void* finalize_map; /* device int32[2 * n_domains]             */
   void* partials;           /* device reduction records                */
+  int32_t single_pass;      /* 1: one single-pass launch per step      */
+  const void* fused_finalize_map; /* device int32[2 * n_domains]       */
 } hp_ensemble;
 
 /* Device workspace bytes hp_ensemble_init needs for these domains. */
 int64_t hp_ensemble_workspace_bytes(int32_t n, const hp_grid_desc* descs);
 /* Copy the descriptors and block maps into `work` (a one-off upload that
  * synchronises `stream`) and fill the handle.  The domains' planes must stay
- * allocated while the handle is used. */
-int hp_ensemble_init(int32_t n, const hp_grid_desc* descs, void* work, int64_t work_bytes, hp_ensemble* out,
-                     void* stream);
+ * allocated while the handle is used.  single_pass 1: every domain needs
+ * h_out, qx_out, qy_out, and each step is one single-pass launch over every
+ * tile of every domain, alternating the two sets of planes. */
+int hp_ensemble_init(int32_t n, const hp_grid_desc* descs, int32_t single_pass, void* work, int64_t work_bytes,
+                     hp_ensemble* out, void* stream);
 /* nsteps x (flux of every domain, update of every domain, per-domain time
- * control), three launches per step, graph-capturable. */
+ * control): three launches per step; single-pass: two (step of every domain,
+ * per-domain time control), and nsteps must be even so that the state ends
+ * in the domains' own planes.  Graph-capturable. */
 int hp_ensemble_step(const hp_ensemble* e, int32_t nsteps, void* stream);
 
 /* ---- Adjoint of one step with a frozen time-step schedule (no reference

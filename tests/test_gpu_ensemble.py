@@ -36,15 +36,21 @@ def state(s):
     return [s.h.clone(), s.qx.clone(), s.qy.clone(), s.h_max.clone(), s.ctrl.clone()]
 
 
+@pytest.mark.parametrize("single_pass", [True, False])
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
-def test_ensemble_matches_standalone_bitwise(cuda, dtype):
-    say(f"[ensemble vs standalone {dtype}]")
+def test_ensemble_matches_standalone_bitwise(cuda, dtype, single_pass):
+    """The single-pass ensemble against standalone single-pass steps, the
+    two-pass ensemble against standalone two-pass steps."""
+    say(f"[ensemble vs standalone {dtype}, single_pass {single_pass}]")
     ens_s = [scenario(*sp, dtype) for sp in SPECS]
     alone = [scenario(*sp, dtype) for sp in SPECS]
-    e = Ensemble(ens_s)
+    e = Ensemble(ens_s, single_pass=single_pass)
     e.step(40)
     for s in alone:
-        s.step(40)
+        if single_pass:
+            s.step_fused(40)
+        else:
+            s.step(40)
     torch.cuda.synchronize()
     for k, (a, b) in enumerate(zip(ens_s, alone)):
         for x, y in zip(state(a), state(b)):
@@ -52,12 +58,13 @@ def test_ensemble_matches_standalone_bitwise(cuda, dtype):
         assert a.ctrl_dict()["steps"] == 40
 
 
-def test_graph_replay_matches_eager(cuda):
-    say("[graph replay]")
+@pytest.mark.parametrize("single_pass", [True, False])
+def test_graph_replay_matches_eager(cuda, single_pass):
+    say(f"[graph replay, single_pass {single_pass}]")
     eager = [scenario(*sp, torch.float32) for sp in SPECS]
     graphed = [scenario(*sp, torch.float32) for sp in SPECS]
-    Ensemble(eager).step(30)
-    e = Ensemble(graphed)
+    Ensemble(eager, single_pass=single_pass).step(30)
+    e = Ensemble(graphed, single_pass=single_pass)
     e.capture(10)
     for r in range(3):
         e.replay()
@@ -69,12 +76,13 @@ def test_graph_replay_matches_eager(cuda):
             assert torch.equal(x, y), f"domain {k} differs after replays"
 
 
-def test_stop_time_freezes_each_domain(cuda):
-    say("[stop time]")
+@pytest.mark.parametrize("single_pass", [True, False])
+def test_stop_time_freezes_each_domain(cuda, single_pass):
+    say(f"[stop time, single_pass {single_pass}]")
     ss = [scenario(*sp, torch.float32) for sp in SPECS]
     for s in ss:
         s.ctrl[_lib.CTRL_EXPORT_T] = 120.0  # the export clamp lands t exactly on 120 s
-    e = Ensemble(ss, t_stop=120.0)
+    e = Ensemble(ss, t_stop=120.0, single_pass=single_pass)
     replays = e.run(steps_per_replay=20)
     torch.cuda.synchronize()
     snap = [state(s) for s in ss]
@@ -86,3 +94,9 @@ def test_stop_time_freezes_each_domain(cuda):
         for x, y in zip(state(a), b):
             assert torch.equal(x, y)
     say(f"  replays {replays}, steps {[s.ctrl_dict()['steps'] for s in ss]}")
+
+
+def test_single_pass_ensemble_rejects_odd_step_counts(cuda):
+    ss = [scenario(*sp, torch.float32) for sp in SPECS[:2]]
+    with pytest.raises(_lib.HipimsError, match="even number of steps"):
+        Ensemble(ss).step(3)
