@@ -389,11 +389,15 @@ __host__ __device__ inline T green_ampt(T h, T F0, T ks, T phi, T dtheta, T dt) 
 
 // CFL time-step candidate of one updated cell (timeControl_Kernel.cu:186-189):
 // the square root is single precision in both instantiations, as in the
-// reference.
+// reference - correctly rounded for FP64 (parity with the reference's FP64
+// time step), the hardware square root for FP32 (one instruction; the
+// reference's build uses --use_fast_math, cuda/setup.py:78, whose sqrtf is not
+// correctly rounded either).
 template <class T>
 __device__ inline T cfl_candidate(T h, T qx, T qy, T cfl_dx) {
   if (!(h > T(kHs))) return T(60.0);
-  const float a = __builtin_sqrtf((float)(T(kG) * h));
+  const float gh = (float)(T(kG) * h);
+  const float a = sizeof(T) == 4 ? __builtin_amdgcn_sqrtf(gh) : __builtin_sqrtf(gh);
   return Prec<T>::div(cfl_dx, T(a) + Prec<T>::div(fmax(fabs(qx), fabs(qy)), h));
 }
 
