@@ -263,7 +263,7 @@ __global__ __launch_bounds__(kAThreads) void k_adj_finish(hp_grid_desc a, const 
 }
 
 template <class T>
-__device__ inline FrameCell<T> gframe(const hp_grid_desc& a, int64_t i, int d, T zref) {
+__device__ inline FrameCell<T> gframe(const hp_grid_desc& a, int i, int d, T zref) {
   const T* h = (const T*)a.h;
   const T* qx = (const T*)a.qx;
   const T* qy = (const T*)a.qy;
@@ -310,11 +310,18 @@ __device__ inline void edge_adj(const hp_grid_desc& a, bool frozen_slopes, int c
   }
   FrameAdj<T> adj[4] = {zero_adj<T>(), zero_adj<T>(), zero_adj<T>(), zero_adj<T>()};  // l, A, B, rr
   int dir[4] = {dAB, dAB, dAB, dAB};
-  const int64_t P = a.pitch;
-  auto valid = [&](int x, int y) -> bool {
-    return x >= 0 && x < nx && y >= 0 && y < ny && a.type_slot[(int64_t)y * P + x] != 0;
+  // Plane offsets fit an int (the entry points check pitch * ny < 2^31); the
+  // four cells lie on one line, one step apart.  The callers launch x-edges
+  // only in rows inside the plane and y-edges only in columns inside it, so
+  // only the coordinate across the edge needs a bounds test; an offset is
+  // read only for a cell that passes it.
+  const int P = (int)a.pitch;
+  const int step = xe ? 1 : P;
+  const int iA = yA * P + xA, iB = iA + step, il = iA - step, ir2 = iB + step;
+  auto valid = [&](int x, int y, int i) -> bool {
+    return (xe ? (x >= 0 && x < nx) : (y >= 0 && y < ny)) && a.type_slot[i] != 0;
   };
-  const bool vA = valid(xA, yA), vB = valid(xB, yB);
+  const bool vA = valid(xA, yA, iA), vB = valid(xB, yB, iB);
   if (vA || vB) {
     const T dt = T(a.ctrl->dt);
     const T dx = T(a.dx);
@@ -322,13 +329,12 @@ __device__ inline void edge_adj(const hp_grid_desc& a, bool frozen_slopes, int c
     const int dBA = (dAB + 2) & 3;
     const bool axis_x = (dAB == 0 || dAB == 2);
     if (vA && vB) {
-      const int64_t iA = (int64_t)yA * P + xA, iB = (int64_t)yB * P + xB;
       const T zref = Prec<T>::local_datum ? ((const T*)a.z)[iA] : T(0);
       const FrameCell<T> c = gframe<T>(a, iA, dAB, zref);
       const FrameCell<T> r = gframe<T>(a, iB, dAB, zref);
-      const bool vl = valid(xl, yl), vr = valid(xr2, yr2);
-      const FrameCell<T> l = vl ? gframe<T>(a, (int64_t)yl * P + xl, dAB, zref) : c;
-      const FrameCell<T> rr = vr ? gframe<T>(a, (int64_t)yr2 * P + xr2, dAB, zref) : r;
+      const bool vl = valid(xl, yl, il), vr = valid(xr2, yr2, ir2);
+      const FrameCell<T> l = vl ? gframe<T>(a, il, dAB, zref) : c;
+      const FrameCell<T> rr = vr ? gframe<T>(a, ir2, dAB, zref) : r;
       T cAh, cAx, cAy, cBh, cBx, cBy;
       cot(xA, yA, cAh, cAx, cAy);
       cot(xB, yB, cBh, cBx, cBy);
@@ -348,7 +354,7 @@ __device__ inline void edge_adj(const hp_grid_desc& a, bool frozen_slopes, int c
     } else {
       const bool from_a = vA;
       const int xc = from_a ? xA : xB, yc = from_a ? yA : yB, d = from_a ? dAB : dBA;
-      const int64_t ic = (int64_t)yc * P + xc;
+      const int ic = from_a ? iA : iB;
       const T zref = Prec<T>::local_datum ? ((const T*)a.z)[ic] : T(0);
       const FrameCell<T> c = gframe<T>(a, ic, d, zref);
       // boundary values at this time
