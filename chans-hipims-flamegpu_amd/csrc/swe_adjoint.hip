@@ -82,15 +82,15 @@ template <class T> struct AdjPlanes {
 template <class T>
 __device__ inline bool flux_listed(const hp_grid_desc& a, const T* h, int x, int y, T t) {
   const T hs = T(kHs);
-  const int64_t P = a.pitch;
-  const int64_t i = (int64_t)y * P + x;
+  const int P = (int)a.pitch;  // planes of fewer than 2^31 elements
+  const int i = y * P + x;
   if (t == T(0) || h[i] > hs) return true;
   const int nx[4] = {x + 1, x, x - 1, x};
   const int ny[4] = {y, y - 1, y, y + 1};
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     if (nx[k] < 0 || nx[k] >= a.nx || ny[k] < 0 || ny[k] >= a.ny) continue;
-    const int64_t j = (int64_t)ny[k] * P + nx[k];
+    const int j = ny[k] * P + nx[k];
     if (a.type_slot[j] == 0) continue;
     if (nx[k] == a.cell0_x && ny[k] == a.cell0_y) continue;
     if (h[j] > hs) return true;
@@ -509,7 +509,7 @@ __global__ __launch_bounds__(NT, sizeof(T) == 4 ? 6 : 1) void k_adj_step_tile(hp
     s_rs[tid] = station_rate<T>(rr, tid, a.rain_scale);
   }
   __syncthreads();
-  const int64_t P = a.pitch;
+  const int P = (int)a.pitch;  // planes of fewer than 2^31 elements (checked at launch)
   const int ny = a.ny;
   const T* __restrict__ h = (const T*)a.h;
   const T* __restrict__ qx = (const T*)a.qx;
@@ -534,7 +534,7 @@ __global__ __launch_bounds__(NT, sizeof(T) == 4 ? 6 : 1) void k_adj_step_tile(hp
   auto load = [&](int x, int y, bool owned, CellLd& L) {
     L.in = x >= 0 && x < P && y >= 0 && y < ny;
     if (!L.in) return;
-    const int64_t i = (int64_t)y * P + x;
+    const int i = y * P + x;
     L.lh = lam.in[0][i];
     L.lqx = lam.in[1][i];
     L.lqy = lam.in[2][i];
@@ -554,7 +554,7 @@ __global__ __launch_bounds__(NT, sizeof(T) == 4 ? 6 : 1) void k_adj_step_tile(hp
     T c0 = T(0), c1 = T(0), c2 = T(0);
     lo[0] = lo[1] = lo[2] = T(0);
     if (L.in) {
-      const int64_t i = (int64_t)y * P + x;
+      const int i = y * P + x;
       T lh = L.lh, lqx = L.lqx, lqy = L.lqy;
       if (L.g >= 0) lh += T(2.0 * (p.loss_h[L.g] - p.loss_obs[L.g]));  // the loss cotangent of the depth after the step
       T rb = T(0), rs = T(0);
@@ -658,7 +658,7 @@ __global__ __launch_bounds__(NT, sizeof(T) == 4 ? 6 : 1) void k_adj_step_tile(hp
   for (int k = 0; k < kRowsPer; ++k) {
     const int ly = lrow + k * (NT / kATX), x = x0 + lx, y = y0 + ly;
     if (y >= ny || x >= P) continue;
-    const int64_t i = (int64_t)y * P + x;
+    const int i = y * P + x;
     T o[3] = {lp[k][0], lp[k][1], lp[k][2]};
     if (a.type_slot[i] != 0) {
 #pragma unroll
