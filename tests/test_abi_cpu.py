@@ -87,6 +87,53 @@ def test_calibration_loop_argument_checks():
                               None, None) == _lib.HP_EINVAL
 
 
+def _fake_desc(dtype=_lib.HP_F32, order=1, nx=100, ny=40, pitch=128):
+    """A descriptor whose plane pointers are aligned placeholders: it passes the
+    host-side checks, and nothing here reaches a launch."""
+    d = HpGridDesc()
+    d.dtype, d.order, d.nx, d.ny, d.pitch, d.dx, d.n_classes = dtype, order, nx, ny, pitch, 1.0, 6
+    for k, name in enumerate(("h", "qx", "qy", "z", "h_max", "dh", "dqx", "dqy", "type_slot", "landuse",
+                              "slot_code", "manning", "ctrl")):
+        setattr(d, name, (k + 1) << 12)
+    return d
+
+
+def test_adjoint_and_ensemble_argument_checks():
+    L = _lib.lib()
+    d = _fake_desc()
+    # grad rows cover the update chunks and the tiles of the single-pass
+    # reverse step (64 x 8 in FP32, 64 x 4 in FP64)
+    assert L.hp_adjoint_grad_blocks(ctypes.byref(d)) >= max((128 * 40 + 1023) // 1024, 2 * 5)
+    d.dtype = _lib.HP_F64
+    assert L.hp_adjoint_grad_blocks(ctypes.byref(d)) >= 2 * 10
+    d.dtype = _lib.HP_F32
+    ad = _lib.HpAdjointDesc()
+    for k, name in enumerate(("lam_h", "lam_qx", "lam_qy", "cot_h", "cot_qx", "cot_qy", "xc", "yc", "grads", "gn")):
+        setattr(ad, name, (k + 64) << 12)
+    ad.cpitch, ad.flux_ready = 128, 1
+    ad.separate_passes = 2
+    assert L.hp_adjoint_step(ctypes.byref(d), ctypes.byref(ad), None) == _lib.HP_EINVAL
+    assert b"separate_passes" in L.hp_last_error()
+    ad.separate_passes = 0
+    ad.gauge_of = 1 << 20  # a gauge plane without its loss rows
+    assert L.hp_adjoint_step(ctypes.byref(d), ctypes.byref(ad), None) == _lib.HP_EINVAL
+    assert b"loss_h" in L.hp_last_error()
+    # the single-pass ensemble needs every domain's second set of planes
+    descs = (HpGridDesc * 2)(_fake_desc(), _fake_desc(nx=60, pitch=64))
+    e = _lib.HpEnsemble()
+    need = L.hp_ensemble_workspace_bytes(2, descs)
+    assert need > 0
+    assert L.hp_ensemble_init(2, descs, 2, 1 << 24, need, ctypes.byref(e), None) == _lib.HP_EINVAL
+    assert b"single_pass" in L.hp_last_error()
+    assert L.hp_ensemble_init(2, descs, 1, 1 << 24, need, ctypes.byref(e), None) == _lib.HP_EINVAL
+    assert b"h_out" in L.hp_last_error()
+    assert L.hp_ensemble_init(2, descs, 0, 1 << 24, need - 1, ctypes.byref(e), None) == _lib.HP_ESPACE
+    # an even step count is asked of a single-pass handle before anything runs
+    e.n_domains, e.doms, e.single_pass = 2, 1 << 24, 1
+    assert L.hp_ensemble_step(ctypes.byref(e), 3, None) == _lib.HP_EINVAL
+    assert b"even" in L.hp_last_error()
+
+
 def test_ops_reject_host_tensors_like_the_reference():
     h = torch.zeros(4, dtype=torch.float64)
     with pytest.raises(RuntimeError, match="must be a CUDA tensor"):
