@@ -7,9 +7,12 @@
 //
 // Per step:
 //   flux    tile of 64 x TY cells + 2-cell cross halo staged in LDS; every
-//           edge touching the tile is solved once (swe_physics.hpp) and both
-//           cells take their share; listed cells (the reference's fluxMask
-//           rule) write dh/dqx/dqy.
+//           edge touching the tile is solved once (swe_physics.hpp), by the
+//           thread that owns its east/south cell, and both cells take their
+//           share (the owner's in registers, the other through LDS); listed
+//           cells (the reference's fluxMask rule) write dh/dqx/dqy.
+//   single-pass step: the flux tile finishes its own cells (the update
+//           below) into a second set of state planes.
 //   update  a contiguous chunk of plane elements per block: rain, implicit
 //           friction + update, h_max, the CFL minimum (wave64 shuffle + LDS),
 //           FP64 volume / flux / rain sums; one partial record per block.
@@ -21,7 +24,8 @@
 // launches take the domain descriptor as a kernel argument; ensemble launches
 // take a device array of descriptors plus a (domain, local index) map per
 // block, so one launch advances every domain of an ensemble by its own dt.
-// A domain whose control block has t_stop > 0 and t >= t_stop is skipped.
+// A domain whose control block has t_stop > 0 and t >= t_stop is skipped
+// (the single-pass step copies its state through to the other set).
 
 #include <hip/hip_runtime.h>
 #include <math.h>
