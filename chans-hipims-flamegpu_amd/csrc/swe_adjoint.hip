@@ -19,6 +19,10 @@
 //   k_adj_gather per cell: lambda_k += the contributions of the 8 edges that
 //               read it
 // Every write goes to the writing thread's own cell or edge slot: no atomics.
+// For order 1, k_adj_step_tile does the three in one pass over 64 x 8 tiles
+// (the update adjoint of the tile and of its edge-sharing ring, the edges of
+// the tile's cells, the per-cell sum), reading the cotangent from one set of
+// planes and writing it to another.
 //
 // hp_calib_forward / hp_calib_reverse run the step loops of a calibration
 // natively: the forward steps on the single-pass kernel (recording the time
