@@ -390,8 +390,9 @@ __device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by, const Pa
         cin[k].on = cls != 255;
         cin[k].cls = cls;
         // the class's Manning value from the (cached) table: no LDS copy, so
-        // that six blocks fit a CU
-        if (cin[k].on) cin[k].n = ((const T*)a.manning)[cls];
+        // that six blocks fit a CU; a class beyond the table (undefined in
+        // the reference) reads its last entry, never past it
+        if (cin[k].on) cin[k].n = ((const T*)a.manning)[cls < a.n_classes ? cls : a.n_classes - 1];
         if (a.rain_mode == HP_RAIN_STATION) cin[k].st = a.station ? (int)a.station[gi] : 0;
         else if (a.rain_mode == HP_RAIN_FIELD) cin[k].rain = T(a.rain_field[gi]) * T(a.rain_scale);
         cin[k].hmax = ((const T*)a.h_max)[gi];

@@ -161,6 +161,8 @@ class GridSolver:
         m = np.atleast_1d(np.asarray(_to_np(manning), dtype=np.float64))
         if m.size < 1 or m.size > 255:
             raise ValueError("manning must have 1..255 classes")
+        if getattr(self, "_max_class", -1) >= m.size:
+            raise ValueError("landuse class outside the manning table")
         self.manning = torch.tensor(m, dtype=self.dtype, device=self.device)
         self._desc = None
 
@@ -173,6 +175,7 @@ class GridSolver:
             cls = np.asarray(_to_np(landuse)).astype(np.int64)
         if (cls[L.valid] < 0).any() or (cls[L.valid] >= self.manning.numel()).any():
             raise ValueError("landuse class outside the manning table")
+        self._max_class = int(cls[L.valid].max()) if L.valid.any() else -1
         lu[:, : L.nx] = np.where(L.valid, cls, NODATA_CLASS).astype(np.uint8)
         self.landuse = torch.from_numpy(lu).to(self.device)
         self._desc = None

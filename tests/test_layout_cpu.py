@@ -65,3 +65,18 @@ def test_generators_are_deterministic_and_shaped():
     assert f.shape == (128, 96) and f.min() >= 0.0 and abs(f.mean() - 1.0) < 0.05
     specs = gen.ensemble_specs(64)
     assert len(specs) == 64 and {s[0] for s in specs} <= set(gen.ENSEMBLE_SIZES)
+
+
+def test_manning_table_must_cover_every_landuse_class():
+    """A valid cell's class indexes the Manning table on the GPU: a table
+    shorter than the classes in use is refused, in either order of setting."""
+    import torch
+    from hipims_amd.grid import GridSolver
+    c = gen.rainfall_runoff(24, 5.0)
+    s = GridSolver(c.mask, c.h, c.qx, c.qy, c.z, c.dx, manning=c.manning, landuse=c.landuse,
+                   device=torch.device("cpu"))
+    with pytest.raises(ValueError, match="manning table"):
+        s.set_manning([0.03])
+    s.set_manning(list(c.manning) + [0.1])
+    with pytest.raises(ValueError, match="manning table"):
+        s.set_landuse(np.full_like(c.landuse, 9))
