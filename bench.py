@@ -218,6 +218,8 @@ def parse():
     p.add_argument("--sim-seconds", type=float, default=900.0, help="ensemble: simulated time of every scenario")
     p.add_argument("--replay-steps", type=int, default=50, help="ensemble: steps per graph replay (even)")
     p.add_argument("--ens-two-pass", action="store_true", help="ensemble: flux and update passes per step")
+    p.add_argument("--no-baselines", action="store_true",
+                   help="ensemble: skip the sequential baseline runs (for long simulated times)")
     p.add_argument("--adj-size", type=int, default=1024)
     p.add_argument("--adj-steps", type=int, default=2000)
     p.add_argument("--adj-checkpoint", type=int, default=100)
@@ -351,8 +353,8 @@ def ensemble_mode(a):
 
     seq = {}
     for name, cls in (("godunov", Godunov), ("compact", CompactGodunov)):
-        seq[name] = sequential(cls, cases, a.sim_seconds, dev, name)
-    seq_s, seq_steps = seq["godunov"]
+        if not a.no_baselines:
+            seq[name] = sequential(cls, cases, a.sim_seconds, dev, name)
     line = {"mode": "ensemble", "scenarios": len(cases), "sizes": [c.mask.shape[0] - 2 for c in cases],
             "sim_seconds": a.sim_seconds, "dtype": "f32", "order": 2, "steps_per_replay": a.replay_steps,
             "step": "two-pass" if a.ens_two_pass else "single-pass",
