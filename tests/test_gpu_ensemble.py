@@ -58,6 +58,19 @@ def test_ensemble_matches_standalone_bitwise(cuda, dtype, single_pass):
         assert a.ctrl_dict()["steps"] == 40
 
 
+def test_single_pass_ensemble_order1_matches_standalone_bitwise(cuda):
+    say("[single-pass ensemble vs standalone, order 1]")
+    ens_s = [scenario(*sp, torch.float32, order=1) for sp in SPECS]
+    alone = [scenario(*sp, torch.float32, order=1) for sp in SPECS]
+    Ensemble(ens_s).step(24)
+    for s in alone:
+        s.step_fused(24)
+    torch.cuda.synchronize()
+    for k, (a, b) in enumerate(zip(ens_s, alone)):
+        for x, y in zip(state(a), state(b)):
+            assert torch.equal(x, y), f"domain {k} differs"
+
+
 @pytest.mark.parametrize("single_pass", [True, False])
 def test_graph_replay_matches_eager(cuda, single_pass):
     say(f"[graph replay, single_pass {single_pass}]")
