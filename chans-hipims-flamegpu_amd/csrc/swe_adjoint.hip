@@ -469,18 +469,19 @@ __global__ __launch_bounds__(kAThreads) void k_adj_gather(hp_grid_desc a, AdjPla
 // flux-increment cotangents in LDS beside its own; (2) solves the adjoint of
 // every edge of its cells (an edge on the tile boundary is solved by both
 // tiles that touch it): the thread that owns a cell solves the cell's west
-// and south edges, keeps their cotangents of its own cell in registers and
-// leaves those of the west and south neighbours in LDS, where the owners of
-// the tile's east column and north row also leave those of the tile's last
-// edges; (3) adds to each cell's pass-through cotangent those of its four
-// edges, in the order k_adj_gather uses.  The cotangent after the step is
+// and north edges (the cell is B of both; rows grow southwards), keeps their
+// cotangents of its own cell in registers and leaves those of the west and
+// north neighbours in LDS, where the threads that solve the tile's east
+// column and south row of edges also leave theirs; (3) adds to each cell's
+// pass-through cotangent those of its four edges, in the order k_adj_gather
+// uses.  The cotangent after the step is
 // read from `in` and the one before it written to `out`: the ring cells
 // belong to the neighbouring tiles, so the two sets of planes must differ.
 constexpr int kATX = 64;
 template <class T, int TY> struct AdjTileLds {
   T cot[3][TY + 2][kATX + 2];  // flux-increment cotangents; row/column 0 and last are the ring
   T east[3][TY][kATX];   // cotangent of each cell from its east edge (the cell is that edge's A)
-  T north[3][TY][kATX];  // ... from its north edge
+  T south[3][TY][kATX];  // ... from its south edge (the cell is that edge's A)
 };
 template <class T> struct AdjLam {
   const T* in[3];
@@ -620,7 +621,7 @@ __global__ __launch_bounds__(NT, sizeof(T) == 4 ? 6 : 1) void k_adj_step_tile(hp
     cx = s.cot[1][ly][lc];
     cy = s.cot[2][ly][lc];
   };
-  T wb[kRowsPer][3], sb[kRowsPer][3];  // this thread's cells from their west and south edges
+  T wb[kRowsPer][3], nb[kRowsPer][3];  // this thread's cells from their west and north edges
 #pragma unroll
   for (int k = 0; k < kRowsPer; ++k) {
     const int ly = lrow + k * (NT / kATX), col = x0 + lx, row = y0 + ly;
@@ -635,18 +636,18 @@ __global__ __launch_bounds__(NT, sizeof(T) == 4 ? 6 : 1) void k_adj_step_tile(hp
     if (col < a.nx && row <= ny) edge_adj<T, 1, 3>(a, false, col, row, lds_cot, outy);
 #pragma unroll
     for (int v = 0; v < 3; ++v) {
-      sb[k][v] = outy[2][v];
-      if (ly > 0) s.north[v][ly - 1][lx] = outy[1][v];
+      nb[k][v] = outy[2][v];
+      if (ly > 0) s.south[v][ly - 1][lx] = outy[1][v];
     }
   }
-  // the tile's north edges (one wave) and east edges: their A cells are the
-  // tile's last row and column
+  // the tile's south row of y-edges (one wave) and east column of x-edges:
+  // their A cells are the tile's last row and column
   if (tid < kATX) {
     const int col = x0 + tid, row = y0 + TY;
     T out[4][3] = {};
     if (col < a.nx && row <= ny) edge_adj<T, 1, 3>(a, false, col, row, lds_cot, out);
 #pragma unroll
-    for (int v = 0; v < 3; ++v) s.north[v][TY - 1][tid] = out[1][v];
+    for (int v = 0; v < 3; ++v) s.south[v][TY - 1][tid] = out[1][v];
   } else if (tid < kATX + TY) {
     const int ly = tid - kATX, col = x0 + kATX, row = y0 + ly;
     T out[4][3] = {};
@@ -657,7 +658,7 @@ __global__ __launch_bounds__(NT, sizeof(T) == 4 ? 6 : 1) void k_adj_step_tile(hp
   __syncthreads();
 
   // (3) the tile's cells: pass-through plus the four edges (as A: east and
-  // north edge; as B: west and south edge)
+  // south edge; as B: west and north edge)
 #pragma unroll
   for (int k = 0; k < kRowsPer; ++k) {
     const int ly = lrow + k * (NT / kATX), x = x0 + lx, y = y0 + ly;
@@ -668,8 +669,8 @@ __global__ __launch_bounds__(NT, sizeof(T) == 4 ? 6 : 1) void k_adj_step_tile(hp
 #pragma unroll
       for (int v = 0; v < 3; ++v) {
         T sum = T(0);
-        sum += s.east[v][ly][lx] + s.north[v][ly][lx];
-        sum += wb[k][v] + sb[k][v];
+        sum += s.east[v][ly][lx] + s.south[v][ly][lx];
+        sum += wb[k][v] + nb[k][v];
         o[v] += sum;
       }
     }
