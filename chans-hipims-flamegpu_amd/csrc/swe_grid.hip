@@ -347,7 +347,9 @@ __device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by, const Pa
     if (FUSED) {
       if (threadIdx.x == 0) part.put(pidx, INFINITY, 0.0, 0.0, 0.0, a.mass_sums != 0);
       // the state passes through to the output planes as well, so that a
-      // caller alternating the two sets of planes keeps the final state
+      // caller alternating the two sets of planes keeps the final state;
+      // once the time control has marked it done, both sets hold it
+      if (a.ctrl->stopped_synced != 0.0) return;
       const int gx = bx * kTX + (int)(threadIdx.x & (kTX - 1));
 #pragma unroll
       for (int k = 0; k < CPT; ++k) {
@@ -803,12 +805,19 @@ __global__ __launch_bounds__(kThreads) void k_update_multi(const hp_grid_desc* _
 // Fold a domain's partials in a fixed order and advance its time control.
 // The minima are read eight per thread before any is used, so one block
 // covers many thousands of partials in a few memory round trips.
+// single_pass: the step just run was the single-pass one.  On a stopped
+// domain the time control records whether that step left the second set of
+// planes holding the state (the single-pass kernel copies it there on the
+// first step past the stop, and no more once this is set).
 __device__ inline void finalize_domain(hp_ctrl* ctrl, const double* __restrict__ mins, const Sums* __restrict__ part,
-                                       int n, double dx, bool sums) {
+                                       int n, double dx, bool sums, bool single_pass) {
   constexpr int kU = 8;
   __shared__ double s_red[4][16];
   const int tid = threadIdx.x;
-  if (stopped(ctrl)) return;
+  if (stopped(ctrl)) {
+    if (threadIdx.x == 0) ctrl->stopped_synced = single_pass ? 1.0 : 0.0;
+    return;
+  }
   double mn = INFINITY, vol = 0.0, fs = 0.0, rs = 0.0;
   for (int base = tid; base < n; base += blockDim.x * kU) {
     double v[kU];
@@ -874,12 +883,14 @@ __device__ inline void finalize_domain(hp_ctrl* ctrl, const double* __restrict__
       ctrl->rain_volume += rs * area;
     }
     ctrl->steps += 1.0;
+    ctrl->stopped_synced = 0.0;
   }
 }
 
-__global__ __launch_bounds__(1024) void k_finalize(hp_ctrl* ctrl, void* work, int n, double dx, int sums) {
+__global__ __launch_bounds__(1024) void k_finalize(hp_ctrl* ctrl, void* work, int n, double dx, int sums,
+                                                   int single_pass) {
   const Partials P(work, n);
-  finalize_domain(ctrl, P.mins, P.sums, n, dx, sums != 0);
+  finalize_domain(ctrl, P.mins, P.sums, n, dx, sums != 0, single_pass != 0);
 }
 
 // k_finalize followed by a calibration step's record (thread 0 wrote the
@@ -892,7 +903,7 @@ __global__ __launch_bounds__(1024) void k_finalize_record(hp_ctrl* ctrl, void* w
   const int i0 = threadIdx.x;
   double v0 = 0.0;
   if (i0 < rec.n) v0 = rec.f32 ? (double)((const float*)rec.h)[rec.off[i0]] : ((const double*)rec.h)[rec.off[i0]];
-  finalize_domain(ctrl, P.mins, P.sums, n, dx, sums != 0);
+  finalize_domain(ctrl, P.mins, P.sums, n, dx, sums != 0, true);
   if (i0 < rec.n) {
     if (rec.f32) ((float*)rec.row)[i0] = (float)v0;
     else ((double*)rec.row)[i0] = v0;
@@ -911,11 +922,11 @@ __global__ __launch_bounds__(1024) void k_finalize_record(hp_ctrl* ctrl, void* w
 // partials are folded in the same order as in a standalone run.
 __global__ __launch_bounds__(1024) void k_finalize_multi(const hp_grid_desc* __restrict__ doms,
                                                         const int2* __restrict__ ranges, void* parts,
-                                                        int n_total) {
+                                                        int n_total, int single_pass) {
   const int2 r = ranges[blockIdx.x];
   const hp_grid_desc& a = doms[blockIdx.x];
   const Partials P(parts, n_total);
-  finalize_domain(a.ctrl, P.mins + r.x, P.sums + r.x, r.y, a.dx, a.mass_sums != 0);
+  finalize_domain(a.ctrl, P.mins + r.x, P.sums + r.x, r.y, a.dx, a.mass_sums != 0, single_pass != 0);
 }
 
 // --- host side -------------------------------------------------------------
@@ -1026,7 +1037,7 @@ static int launch_finalize_fused(const hp_grid_desc* g, hipStream_t st, const St
     return check_launch("k_finalize_record");
   }
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, st, g->ctrl, g->work, (int)flux_tiles(g), g->dx,
-                     g->mass_sums);
+                     g->mass_sums, 1);
   return check_launch("k_finalize");
 }
 
@@ -1059,7 +1070,7 @@ static int launch_update(const hp_grid_desc* g, int mode, hipStream_t st) {
   if (rc) return rc;
   if (mode == HP_UPDATE_STEP || mode == HP_UPDATE_CFL) {
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, st, g->ctrl, g->work, (int)nb, g->dx,
-                       g->mass_sums);
+                       g->mass_sums, 0);
     return check_launch("k_finalize");
   }
   return HP_OK;
@@ -1128,7 +1139,7 @@ static int ensemble_fused_step_once(const hp_ensemble* e, int parity, hipStream_
   if (rc) return rc;
   hipLaunchKernelGGL(k_finalize_multi, dim3((unsigned)e->n_domains), dim3(1024), 0, st,
                      (const hp_grid_desc*)e->doms, (const int2*)e->fused_finalize_map, e->partials,
-                     (int)e->n_flux_tiles);
+                     (int)e->n_flux_tiles, 1);
   return check_launch("k_finalize_multi");
 }
 
@@ -1152,7 +1163,7 @@ static int ensemble_step_once(const hp_ensemble* e, hipStream_t st) {
   if (rc) return rc;
   hipLaunchKernelGGL(k_finalize_multi, dim3((unsigned)e->n_domains), dim3(1024), 0, st,
                      (const hp_grid_desc*)e->doms, (const int2*)e->finalize_map, e->partials,
-                     (int)e->n_update_blocks);
+                     (int)e->n_update_blocks, 0);
   return check_launch("k_finalize_multi");
 }
 

@@ -34,7 +34,8 @@ class HpCtrl(ctypes.Structure):
     _fields_ = [("t", ctypes.c_double), ("dt", ctypes.c_double), ("export_T", ctypes.c_double),
                 ("export_n", ctypes.c_double), ("export_pending", ctypes.c_double), ("volume", ctypes.c_double),
                 ("rain_volume", ctypes.c_double), ("flux_volume", ctypes.c_double), ("steps", ctypes.c_double),
-                ("t_stop", ctypes.c_double), ("reserved", ctypes.c_double * 6)]
+                ("t_stop", ctypes.c_double), ("stopped_synced", ctypes.c_double),
+                ("reserved", ctypes.c_double * 5)]
 
 
 CTRL_FIELDS = [f for f, _ in HpCtrl._fields_[:10]]

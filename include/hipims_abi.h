@@ -117,7 +117,9 @@ typedef struct hp_ctrl {
   double flux_volume;    /* cumulative sum of flux dh (net boundary inflow)  */
   double steps;          /* completed steps                                  */
   double t_stop;         /* > 0: the domain's passes do nothing once t >= t_stop */
-  double reserved[6];
+  double stopped_synced; /* 1: stopped, and the single-pass step's second set of
+                            planes holds the state (set by the time control)   */
+  double reserved[5];
 } hp_ctrl;
 
 /* Rain source of the update pass. */
