@@ -48,7 +48,7 @@ constexpr int kARateCache = 64;  // station rates kept in LDS per block
 
 int grid_flux_launch(const hp_grid_desc* g, hipStream_t st);                   // swe_grid.hip
 int grid_fused_launch(const hp_grid_desc* g, bool save_flux, bool finalize, hipStream_t st,
-                      const StepRecord* rec);  // swe_grid.hip
+                      const StepRecord* rec, bool time_control);  // swe_grid.hip
 
 // Records (t, dt) of the control block as step k of a schedule [K+1][2].
 __global__ void k_schedule_save(const hp_ctrl* ctrl, double* sched, int k) {
@@ -904,7 +904,11 @@ int hp_calib_forward(const hp_grid_desc* g, double* sched, const hp_ctrl* frozen
     // has none, so the gauges then take a small kernel of their own
     const StepRecord rec{out[0], gauge_off, n, row, save, k + 1, g->dtype == HP_F32};
     const bool ride = !frozen && (gauges || save);
-    if ((rc = grid_fused_launch(&d, seg_u != nullptr, frozen == nullptr, st, ride ? &rec : nullptr))) return rc;
+    // a frozen segment (the reverse sweep's recomputation, or a frozen run's
+    // last segment) needs no CFL candidates: no time control follows
+    const bool time_control = !(frozen && seg_u);
+    if ((rc = grid_fused_launch(&d, seg_u != nullptr, frozen == nullptr, st, ride ? &rec : nullptr, time_control)))
+      return rc;
     if (frozen && gauges) {
       const unsigned nb = (unsigned)((n > 0 ? n : 1) + 255) / 256;
       if (g->dtype == HP_F32)

@@ -301,7 +301,8 @@ template <class T> struct CellIn {
 // them in unconditionally and the accumulators stay in registers.
 template <class T>
 __device__ inline void update_state(const CellIn<T>& in, T& h, T& qx, T& qy, T dhf, T dqx, T dqy, T dF, T dt,
-                                    T cfl_dx, T& hmax, T& cand, T& vol_h, T& flux_h, T& rain_h) {
+                                    T cfl_dx, T& hmax, T& cand, T& vol_h, T& flux_h, T& rain_h,
+                                    bool time_control = true) {
   cand = T(INFINITY);
   vol_h = T(0);
   flux_h = T(0);
@@ -323,7 +324,7 @@ __device__ inline void update_state(const CellIn<T>& in, T& h, T& qx, T& qy, T d
     implicit_friction<T>(h, in.n, dt, qx, qy);
   }
   hmax = fmax(h, hmax);
-  cand = cfl_candidate<T>(h, qx, qy, cfl_dx);
+  if (time_control) cand = cfl_candidate<T>(h, qx, qy, cfl_dx);
   vol_h = h;
   flux_h = dhf;
   rain_h = rdt;
@@ -334,9 +335,11 @@ __device__ inline void update_state(const CellIn<T>& in, T& h, T& qx, T& qy, T d
 // and update of the tile's own cells into h_out/qx_out/qy_out (the input
 // planes stay untouched for the neighbouring tiles' halos), h_max, and one
 // partial record (CFL minimum, FP64 sums) per tile.
+// time_control false (single-pass step of a frozen schedule, whose time
+// control never runs): no CFL candidates, sums or partial record.
 template <class T, int ORDER, int TY, bool FUSED>
 __device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by, const Partials& part, int64_t pidx,
-                                 bool save_flux = false) {
+                                 bool save_flux = false, bool time_control = true) {
   __shared__ FluxTile<T, TY> s;
   __shared__ T s_rate[FUSED ? kRateCache : 1];
   using Tile = FluxTile<T, TY>;
@@ -566,12 +569,14 @@ __device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by, const Pa
         ((T*)a.infil_F)[gi] = in.F + dF;
       }
       T cand, vh, fh, rh;
-      update_state<T>(in, hn, qxn, qyn, ah, aqx, aqy, dF, dt, cfl_dx, hm, cand, vh, fh, rh);
-      mn = fmin(mn, cand);
-      if (sums) {
-        vol += (double)vh;
-        fsum += (double)fh;
-        rsum += (double)rh;
+      update_state<T>(in, hn, qxn, qyn, ah, aqx, aqy, dF, dt, cfl_dx, hm, cand, vh, fh, rh, time_control);
+      if (time_control) {
+        mn = fmin(mn, cand);
+        if (sums) {
+          vol += (double)vh;
+          fsum += (double)fh;
+          rsum += (double)rh;
+        }
       }
       ((T*)a.h_out)[gi] = hn;
       ((T*)a.qx_out)[gi] = qxn;
@@ -579,7 +584,7 @@ __device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by, const Pa
       if (hm != in.hmax) ((T*)a.h_max)[gi] = hm;
     }
   }
-  if (FUSED) block_partial<T>(mn, vol, fsum, rsum, part, pidx, sums);
+  if (FUSED && time_control) block_partial<T>(mn, vol, fsum, rsum, part, pidx, sums);
 }
 
 template <class T, int ORDER, int TY>
@@ -613,11 +618,14 @@ __global__ __launch_bounds__(kThreads) void k_step_fused_multi(const hp_grid_des
 
 // Single-pass step: reads (h, qx, qy), writes (h_out, qx_out, qy_out).
 template <class T, int ORDER, int TY>
-__global__ __launch_bounds__(kThreads) void k_step_fused(hp_grid_desc a, int ntx, int save_flux) {
+// flags: 1 also save the step's flux in dh, dqx, dqy; 2 no time-control
+// outputs (a frozen schedule).
+__global__ __launch_bounds__(kThreads) void k_step_fused(hp_grid_desc a, int ntx, int flags) {
   // the partial record goes to the tile's own slot (tile order, as in an
   // ensemble), not the block's
   const int tile = xcd_band(blockIdx.x, gridDim.x);
-  flux_tile<T, ORDER, TY, true>(a, tile % ntx, tile / ntx, Partials(a.work, gridDim.x), tile, save_flux != 0);
+  flux_tile<T, ORDER, TY, true>(a, tile % ntx, tile / ntx, Partials(a.work, gridDim.x), tile, (flags & 1) != 0,
+                                (flags & 2) == 0);
 }
 
 // Infiltration pass over plane elements: a valid cell listed by the flux
@@ -1010,24 +1018,24 @@ static int launch_flux(const hp_grid_desc* g, hipStream_t st) {
 }
 
 template <class T, int ORDER, int TY>
-static int launch_fused_t(const hp_grid_desc* g, hipStream_t st, int save_flux) {
+static int launch_fused_t(const hp_grid_desc* g, hipStream_t st, int flags) {
   const int ntx = (g->nx + kTX - 1) / kTX, nty = (g->ny + TY - 1) / TY;
-  hipLaunchKernelGGL((k_step_fused<T, ORDER, TY>), dim3(ntx * nty), dim3(kThreads), 0, st, *g, ntx, save_flux);
+  hipLaunchKernelGGL((k_step_fused<T, ORDER, TY>), dim3(ntx * nty), dim3(kThreads), 0, st, *g, ntx, flags);
   return check_launch("k_step_fused");
 }
 
-static int launch_step_fused(const hp_grid_desc* g, hipStream_t st, int save_flux = 0) {
+static int launch_step_fused(const hp_grid_desc* g, hipStream_t st, int flags = 0) {
   const int ty = flux_rows(g);
   if (g->dtype == HP_F32) {
     if (g->order == 2) {
-      if (ty == 4) return launch_fused_t<float, 2, 4>(g, st, save_flux);
-      return ty == 16 ? launch_fused_t<float, 2, 16>(g, st, save_flux) : launch_fused_t<float, 2, 8>(g, st, save_flux);
+      if (ty == 4) return launch_fused_t<float, 2, 4>(g, st, flags);
+      return ty == 16 ? launch_fused_t<float, 2, 16>(g, st, flags) : launch_fused_t<float, 2, 8>(g, st, flags);
     }
-    if (ty == 4) return launch_fused_t<float, 1, 4>(g, st, save_flux);
-    return ty == 16 ? launch_fused_t<float, 1, 16>(g, st, save_flux) : launch_fused_t<float, 1, 8>(g, st, save_flux);
+    if (ty == 4) return launch_fused_t<float, 1, 4>(g, st, flags);
+    return ty == 16 ? launch_fused_t<float, 1, 16>(g, st, flags) : launch_fused_t<float, 1, 8>(g, st, flags);
   }
-  if (g->order == 2) return launch_fused_t<double, 2, 4>(g, st, save_flux);
-  return launch_fused_t<double, 1, 4>(g, st, save_flux);
+  if (g->order == 2) return launch_fused_t<double, 2, 4>(g, st, flags);
+  return launch_fused_t<double, 1, 4>(g, st, flags);
 }
 
 static int launch_finalize_fused(const hp_grid_desc* g, hipStream_t st, const StepRecord* rec = nullptr) {
@@ -1180,15 +1188,19 @@ __global__ void k_scatter(T* __restrict__ plane, const int64_t* __restrict__ off
 
 // Single-pass step (h, qx, qy -> h_out, qx_out, qy_out; with save_flux the
 // step's flux also into dh, dqx, dqy) and, with finalize, its time control,
-// followed by rec when given; shared with the calibration runs.
-int grid_fused_launch(const hp_grid_desc* g, bool save_flux, bool finalize, hipStream_t st, const StepRecord* rec) {
+// followed by rec when given; without time_control the step computes no
+// CFL candidates, sums or partial records (no time control may follow).
+// Shared with the calibration runs.
+int grid_fused_launch(const hp_grid_desc* g, bool save_flux, bool finalize, hipStream_t st, const StepRecord* rec,
+                      bool time_control) {
   int rc = check_desc(g);
   if (rc) return rc;
   if (!g->h_out || !g->qx_out || !g->qy_out) return fail(HP_EINVAL, "the fused step needs h_out, qx_out, qy_out");
   if (save_flux && (!g->dh || !g->dqx || !g->dqy)) return fail(HP_EINVAL, "saving the flux needs dh, dqx, dqy");
   if (!g->work || g->work_bytes < partials_bytes(flux_tiles(g)))
     return fail(HP_ESPACE, "workspace smaller than hp_grid_workspace_bytes()");
-  rc = launch_step_fused(g, st, save_flux ? 1 : 0);
+  if (finalize && !time_control) return fail(HP_EINVAL, "a time control needs the step's CFL candidates");
+  rc = launch_step_fused(g, st, (save_flux ? 1 : 0) | (time_control ? 0 : 2));
   if (rc || !finalize) return rc;
   return launch_finalize_fused(g, st, rec);
 }

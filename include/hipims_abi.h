@@ -318,8 +318,10 @@ int hp_adjoint_add_loss(int dtype, void* lam_h, const int64_t* offsets, const do
  * planes of pitch*ny elements one after another; the first is copied from
  * h, qx, qy, the others are written by the steps) and the step's flux at
  * seg_d + (k-k0)*seg_stride; the state after the last step goes to h, qx,
- * qy.  Without segments the steps alternate between h, qx, qy and h_out,
- * qx_out, qy_out and end in h, qx, qy.  hist (may be null): the depths at
+ * qy; with a frozen schedule the saved steps skip the CFL candidates and
+ * mass sums, which no time control reads.  Without segments the steps
+ * alternate between h, qx, qy and h_out, qx_out, qy_out and end in h, qx,
+ * qy.  hist (may be null): the depths at
  * the n_gauges plane offsets after step k go to row k+1 (grid dtype). */
 int hp_calib_forward(const hp_grid_desc* g, double* sched, const hp_ctrl* frozen, int32_t k0, int32_t nsteps,
                      void* seg_u, void* seg_d, int64_t seg_stride, const int64_t* gauge_off, int32_t n_gauges,
