@@ -143,51 +143,28 @@ def time_two_pass(g, steps):
 
 
 def time_fused(g, steps):
-    """Single-pass step (fused kernel + finalize), two steps per call so the
-    state returns to its own buffers; events around each call."""
-    import torch
-    stream = torch.cuda.current_stream(g.device)
-    pairs = max(1, steps // 2)
-    g.step_fused(2)  # allocate the second buffers outside the timed region
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(pairs + 1)]
-    torch.cuda.synchronize()
-    wall = time.perf_counter()
-    ev[0].record(stream)
-    for k in range(pairs):
-        g.step_fused(2)
-        ev[k + 1].record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - wall
-    total = ev[0].elapsed_time(ev[-1])
-    return {"path": "fused", "steps": 2 * pairs, "total_ms": total, "wall_s": wall,
-            "kernels_ms": {"k_step_fused": total / (2 * pairs)}}
-
-
-def time_fused_kernel(g, steps):
-    """The single-pass kernel alone: `steps` launches back to back on the
-    current dt (frozen control blocks, no time-control pass, the buffers
-    alternating), one call, HIP events around it; the state is restored
-    afterwards."""
+    """Single-pass step (fused kernel + time control): `steps` steps in one
+    call of hp_grid_step_fused_timed, which records HIP events on the launch
+    stream before each single-pass kernel, between it and its time control
+    and after the last step, so the kernel's own duration and the steps'
+    total come from the same launches."""
     import ctypes
 
     import torch
 
     from hipims_amd import _lib
-    steps += steps & 1
     g.ensure_out()
-    saved = [g.U.clone(), g.h_max.clone()]
-    frozen = g.ctrl.reshape(1, -1).repeat(steps, 1).contiguous()
-    stream = torch.cuda.current_stream(g.device)
-    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    g.step_fused(2)  # untimed: first launches on the second set of planes
     torch.cuda.synchronize()
-    a.record(stream)
-    _lib.check(_lib.lib().hp_calib_forward(ctypes.byref(g.desc), None, frozen.data_ptr(), 0, steps, None, None, 0,
-                                           None, 0, None, _lib.stream_ptr(g.device)), "hp_calib_forward")
-    b.record(stream)
-    torch.cuda.synchronize()
-    g.U.copy_(saved[0])
-    g.h_max.copy_(saved[1])
-    return a.elapsed_time(b) / steps
+    k_ms, tot_ms = ctypes.c_float(), ctypes.c_float()
+    wall = time.perf_counter()
+    _lib.check(_lib.lib().hp_grid_step_fused_timed(ctypes.byref(g.desc), int(steps), ctypes.byref(k_ms),
+                                                    ctypes.byref(tot_ms), _lib.stream_ptr(g.device)),
+               "hp_grid_step_fused_timed")
+    wall = time.perf_counter() - wall
+    kern = k_ms.value / steps
+    return {"path": "fused", "steps": steps, "total_ms": tot_ms.value, "wall_s": wall,
+            "kernels_ms": {"k_step_fused": kern, "k_finalize+gaps": tot_ms.value / steps - kern}}
 
 
 def parse():
@@ -206,7 +183,9 @@ def parse():
                    help="also keep the FP64 volume / rain / boundary-flux sums each step (diagnostics the "
                         "reference step does not compute)")
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
-    p.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, available cores)")
+    p.add_argument("--cpu-threads", type=int, default=0,
+                   help="0: the physical cores (lscpu) among the CPUs this process may run on, at most the "
+                        "host's per-GPU share (OMP_NUM_THREADS when set)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--mode", choices=["step", "copy", "valu", "ensemble", "adjoint"], default="step",
                    help="copy: a plain device copy of known size, the reference point for HBM byte counters; "
@@ -592,7 +571,6 @@ def main():
     two = time_two_pass(g, a.steps)
     g.set_tile_rows(rows["fused"])
     fused = time_fused(g, a.steps)
-    fused_kernel_ms = time_fused_kernel(g, min(a.steps, 400))
     ctrl = g.ctrl_dict()
     assert np.isfinite(ctrl["t"]) and ctrl["dt"] > 0
 
@@ -601,14 +579,13 @@ def main():
     ms_step = best["total_ms"] / best["steps"]
     value = cells * best["steps"] / (best["total_ms"] / 1e3)
     per_kernel = dict(best["kernels_ms"])
-    if best["path"] == "fused":
-        # the kernel's own duration (the step adds the time-control pass)
-        per_kernel = {"k_step_fused": fused_kernel_ms, "k_finalize+gaps": ms_step - fused_kernel_ms}
-    dom = max(per_kernel, key=per_kernel.get)
+    dom = max((k for k in per_kernel if k in B), key=per_kernel.get)
+    assert per_kernel[dom] <= ms_step, "the dominant kernel's time exceeds the step it belongs to"
     achieved = B[dom] * cells / (per_kernel[dom] / 1e3) / 1e9
     two_pass = (B["k_flux"] + B["k_update"]) * value / 1e9
     valu = profiled_valu(dom, workload_tag(a))
     vpeak, vsrc = valu_peak()
+    valu_frac = None if valu is None else valu * cells / (per_kernel[dom] / 1e3) / vpeak
     t_wall = best["wall_s"]
 
     cpu = None
@@ -636,12 +613,18 @@ def main():
                    "tile_rows_tuning_ms": {p: {str(k): v for k, v in t.items()} for p, t in tuned.items()},
                    "mass_sums": a.mass_sums,
                    "sim_time_s": ctrl["t"]},
+        # priced on the HBM roofline (no MFMA work on this path); `limiter`
+        # names the resource the kernel is closer to saturating: the HBM
+        # fraction above or the VALU issue fraction from the counted
+        # instructions per cell
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": profiled_traffic(dom, workload_tag(a)),
                      "alg_bytes_per_cell": B[dom], "kernel_ms": per_kernel[dom],
+                     "timed_with": "HIP events around each launch of the timed steps",
                      "valu_lane_instr_per_cell": valu,
-                     "valu_issue_frac": None if valu is None else valu * cells / (per_kernel[dom] / 1e3) / vpeak,
+                     "valu_issue_frac": valu_frac,
+                     "limiter": "valu" if valu_frac is not None and valu_frac > achieved / HBM_PEAK_GBS else "hbm",
                      "valu_peak_lane_instr_per_s": vpeak, "valu_peak_source": vsrc},
         "kernels_ms": per_kernel,
         "path": best["path"],
@@ -654,6 +637,27 @@ def main():
     print(json.dumps(line), flush=True)
 
 
+def physical_cores():
+    """(physical cores among the CPUs this process may run on, physical cores
+    of the host) from `lscpu -p=CPU,CORE,SOCKET`; logical CPU counts when
+    lscpu is unavailable."""
+    import subprocess
+    allowed = os.sched_getaffinity(0)
+    try:
+        out = subprocess.run(["lscpu", "-p=CPU,CORE,SOCKET"], capture_output=True, text=True, check=True).stdout
+    except (OSError, subprocess.CalledProcessError):
+        return len(allowed), os.cpu_count() or len(allowed)
+    mine, every = set(), set()
+    for line in out.splitlines():
+        if not line or line.startswith("#"):
+            continue
+        cpu, core, sock = (int(v) if v else 0 for v in line.split(",")[:3])
+        every.add((sock, core))
+        if cpu in allowed:
+            mine.add((sock, core))
+    return len(mine) or len(allowed), len(every) or os.cpu_count() or len(allowed)
+
+
 def cpu_baseline(a, g, case, dtype):
     """Oracle (C++/OpenMP, the reference's compact layout and op sequence) on
     host cores, from the GPU's current state, for a bounded step count."""
@@ -663,7 +667,9 @@ def cpu_baseline(a, g, case, dtype):
     import oracle as orc
 
     orc.build()
-    threads = a.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+    mine, host = physical_cores()
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = a.cpu_threads or (min(mine, share) if share > 0 else mine)
     orc.set_threads(threads)
     np_dt = np.float32 if dtype == torch.float32 else np.float64
     sim = orc.OracleSim(case.mask, case.h, case.qx, case.qy, case.z, case.dx, cfl=case.cfl, order=a.order,
@@ -680,6 +686,9 @@ def cpu_baseline(a, g, case, dtype):
     sim.run(n)
     secs = sim.last_seconds
     return {"value": sim.N * n / secs, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "cores_note": f"{threads} OpenMP threads, one per physical core: lscpu counts {mine} physical cores "
+                          f"on the CPUs this process may use and {host} on the host"
+                          + (f"; the host's per-GPU share is OMP_NUM_THREADS={share}" if share > 0 else ""),
             "sample": f"{n} steps of the same {a.size}x{a.size} workload from the GPU's warmed state "
                       f"({secs:.1f} s, oracle C++/OpenMP, {np_dt.__name__}"
                       + (")" if case.rain_table is not None else "; the oracle driver has no rain-field source, "

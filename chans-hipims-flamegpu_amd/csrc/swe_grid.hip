@@ -339,7 +339,7 @@ __device__ inline void update_state(const CellIn<T>& in, T& h, T& qx, T& qy, T d
 // control never runs): no CFL candidates, sums or partial record.
 template <class T, int ORDER, int TY, bool FUSED>
 __device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by, const Partials& part, int64_t pidx,
-                                 bool save_flux = false, bool time_control = true) {
+                                 bool save_flux = false, bool time_control = true, bool force_copy = false) {
   __shared__ FluxTile<T, TY> s;
   __shared__ T s_rate[FUSED ? kRateCache : 1];
   using Tile = FluxTile<T, TY>;
@@ -351,8 +351,10 @@ __device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by, const Pa
       if (threadIdx.x == 0) part.put(pidx, INFINITY, 0.0, 0.0, 0.0, a.mass_sums != 0);
       // the state passes through to the output planes as well, so that a
       // caller alternating the two sets of planes keeps the final state;
-      // once the time control has marked it done, both sets hold it
-      if (a.ctrl->stopped_synced != 0.0) return;
+      // once the time control has marked it done, both sets hold it (unless
+      // the caller asks for the copy: the first step of a call, after which
+      // the host may have written the state)
+      if (a.ctrl->stopped_synced != 0.0 && !force_copy) return;
       const int gx = bx * kTX + (int)(threadIdx.x & (kTX - 1));
 #pragma unroll
       for (int k = 0; k < CPT; ++k) {
@@ -619,13 +621,14 @@ __global__ __launch_bounds__(kThreads) void k_step_fused_multi(const hp_grid_des
 // Single-pass step: reads (h, qx, qy), writes (h_out, qx_out, qy_out).
 template <class T, int ORDER, int TY>
 // flags: 1 also save the step's flux in dh, dqx, dqy; 2 no time-control
-// outputs (a frozen schedule).
+// outputs (a frozen schedule); 4 a stopped domain copies its state to the
+// output planes even when the time control marked both sets in step.
 __global__ __launch_bounds__(kThreads) void k_step_fused(hp_grid_desc a, int ntx, int flags) {
   // the partial record goes to the tile's own slot (tile order, as in an
   // ensemble), not the block's
   const int tile = xcd_band(blockIdx.x, gridDim.x);
   flux_tile<T, ORDER, TY, true>(a, tile % ntx, tile / ntx, Partials(a.work, gridDim.x), tile, (flags & 1) != 0,
-                                (flags & 2) == 0);
+                                (flags & 2) == 0, (flags & 4) != 0);
 }
 
 // Infiltration pass over plane elements: a valid cell listed by the flux
@@ -1212,6 +1215,17 @@ int grid_flux_launch(const hp_grid_desc* g, hipStream_t st) {
   return launch_flux(g, st);
 }
 
+// After an odd number of single-pass steps the state sits in the second set
+// of planes: bring it back to the caller's (h, qx, qy).
+static int copy_state_back(const hp_grid_desc* g, hipStream_t st) {
+  const size_t bytes = (size_t)(g->pitch * (int64_t)g->ny) * (g->dtype == HP_F32 ? 4 : 8);
+  hipError_t e = hipMemcpyAsync(g->h, g->h_out, bytes, hipMemcpyDeviceToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(g->qx, g->qx_out, bytes, hipMemcpyDeviceToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(g->qy, g->qy_out, bytes, hipMemcpyDeviceToDevice, st);
+  if (e != hipSuccess) return fail(HP_ELAUNCH, std::string("state copy: ") + hipGetErrorString(e));
+  return HP_OK;
+}
+
 }  // namespace hp
 
 using namespace hp;
@@ -1235,7 +1249,10 @@ int hp_grid_step_fused(const hp_grid_desc* g, int32_t nsteps, void* stream) {
   hipStream_t st = (hipStream_t)stream;
   hp_grid_desc d = *g;
   for (int32_t k = 0; k < nsteps; ++k) {
-    rc = launch_step_fused(&d, st);
+    // the first step of a call copies a stopped domain's state through even
+    // if an earlier call left both sets in step: the host may have written
+    // the state in between
+    rc = launch_step_fused(&d, st, k == 0 ? 4 : 0);
     if (rc) return rc;
     rc = launch_finalize_fused(&d, st);
     if (rc) return rc;
@@ -1243,14 +1260,63 @@ int hp_grid_step_fused(const hp_grid_desc* g, int32_t nsteps, void* stream) {
     std::swap(d.qx, d.qx_out);
     std::swap(d.qy, d.qy_out);
   }
-  if (nsteps & 1) {  // bring the result back to the caller's state planes
-    const size_t bytes = (size_t)(g->pitch * (int64_t)g->ny) * (g->dtype == HP_F32 ? 4 : 8);
-    hipError_t e = hipMemcpyAsync(g->h, g->h_out, bytes, hipMemcpyDeviceToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(g->qx, g->qx_out, bytes, hipMemcpyDeviceToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(g->qy, g->qy_out, bytes, hipMemcpyDeviceToDevice, st);
-    if (e != hipSuccess) return fail(HP_ELAUNCH, std::string("state copy: ") + hipGetErrorString(e));
-  }
+  if (nsteps & 1) return copy_state_back(g, st);
   return HP_OK;
+}
+
+int hp_grid_step_fused_timed(const hp_grid_desc* g, int32_t nsteps, float* kernel_ms, float* step_ms,
+                             void* stream) {
+  int rc = check_desc(g);
+  if (rc) return rc;
+  if (nsteps <= 0) return fail(HP_EINVAL, "nsteps must be positive");
+  if (!kernel_ms || !step_ms) return fail(HP_EINVAL, "kernel_ms and step_ms are required");
+  if (!g->h_out || !g->qx_out || !g->qy_out) return fail(HP_EINVAL, "the fused step needs h_out, qx_out, qy_out");
+  if (!g->work || g->work_bytes < partials_bytes(flux_tiles(g)))
+    return fail(HP_ESPACE, "workspace smaller than hp_grid_workspace_bytes()");
+  hipStream_t st = (hipStream_t)stream;
+  // ev[2k]: before step k's single-pass kernel, ev[2k+1]: between it and its
+  // time control, ev[2 nsteps]: after the last time control
+  std::vector<hipEvent_t> ev(2 * (size_t)nsteps + 1, nullptr);
+  auto release = [&]() {
+    for (hipEvent_t e : ev)
+      if (e) (void)hipEventDestroy(e);
+  };
+  for (auto& e : ev) {
+    if (hipEventCreate(&e) != hipSuccess) {
+      release();
+      return fail(HP_ELAUNCH, "hipEventCreate failed");
+    }
+  }
+  hp_grid_desc d = *g;
+  for (int32_t k = 0; k < nsteps && rc == HP_OK; ++k) {
+    (void)hipEventRecord(ev[2 * k], st);
+    rc = launch_step_fused(&d, st, k == 0 ? 4 : 0);
+    if (rc == HP_OK) {
+      (void)hipEventRecord(ev[2 * k + 1], st);
+      rc = launch_finalize_fused(&d, st);
+    }
+    std::swap(d.h, d.h_out);
+    std::swap(d.qx, d.qx_out);
+    std::swap(d.qy, d.qy_out);
+  }
+  if (rc == HP_OK) {
+    (void)hipEventRecord(ev[2 * (size_t)nsteps], st);
+    if (nsteps & 1) rc = copy_state_back(g, st);  // after the timed steps
+    if (rc == HP_OK && hipEventSynchronize(ev[2 * (size_t)nsteps]) != hipSuccess)
+      rc = fail(HP_ELAUNCH, "hipEventSynchronize failed");
+  }
+  if (rc == HP_OK) {
+    float sum = 0.0f, ms = 0.0f;
+    for (int32_t k = 0; k < nsteps; ++k) {
+      (void)hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]);
+      sum += ms;
+    }
+    (void)hipEventElapsedTime(&ms, ev[0], ev[2 * (size_t)nsteps]);
+    *kernel_ms = sum;
+    *step_ms = ms;
+  }
+  release();
+  return rc;
 }
 
 int hp_grid_flux(const hp_grid_desc* g, void* stream) {

@@ -40,7 +40,7 @@ class HpCtrl(ctypes.Structure):
 
 CTRL_FIELDS = [f for f, _ in HpCtrl._fields_[:10]]
 CTRL_T, CTRL_DT, CTRL_EXPORT_T, CTRL_EXPORT_N, CTRL_PENDING, CTRL_VOLUME = 0, 1, 2, 3, 4, 5
-CTRL_RAIN, CTRL_FLUX, CTRL_STEPS, CTRL_T_STOP = 6, 7, 8, 9
+CTRL_RAIN, CTRL_FLUX, CTRL_STEPS, CTRL_T_STOP, CTRL_STOPPED_SYNCED = 6, 7, 8, 9, 10
 CTRL_DOUBLES = 16
 
 
@@ -100,6 +100,8 @@ SIGNATURES = {
     "hp_grid_update_mode": (_I, [ctypes.POINTER(HpGridDesc), _I, _P]),
     "hp_grid_step": (_I, [ctypes.POINTER(HpGridDesc), _I32, _P]),
     "hp_grid_step_fused": (_I, [ctypes.POINTER(HpGridDesc), _I32, _P]),
+    "hp_grid_step_fused_timed": (_I, [ctypes.POINTER(HpGridDesc), _I32, ctypes.POINTER(ctypes.c_float),
+                                      ctypes.POINTER(ctypes.c_float), _P]),
     "hp_ensemble_workspace_bytes": (_I64, [_I32, ctypes.POINTER(HpGridDesc)]),
     "hp_ensemble_init": (_I, [_I32, ctypes.POINTER(HpGridDesc), _I32, _P, _I64, ctypes.POINTER(HpEnsemble), _P]),
     "hp_ensemble_step": (_I, [ctypes.POINTER(HpEnsemble), _I32, _P]),

@@ -378,8 +378,14 @@ class GridSolver:
                               out.data_ptr(), self._stream()), "hp_gather")
         return out
 
+    def state_written(self) -> None:
+        """Record that the host wrote the state planes: a stopped domain's
+        second set of planes no longer holds its state."""
+        self.ctrl[_lib.CTRL_STOPPED_SYNCED] = 0.0
+
     def load_compact(self, plane: torch.Tensor, values: torch.Tensor) -> None:
         """Write compact-order values into `plane` at the valid cells."""
+        self.state_written()
         values = values.to(device=self.device, dtype=plane.dtype).contiguous()
         check(lib().hp_scatter(_lib.dtype_code(plane.dtype), plane.data_ptr(), self.offsets.data_ptr(),
                                values.numel(), values.data_ptr(), self._stream()), "hp_scatter")

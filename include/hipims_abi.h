@@ -3,9 +3,10 @@
  *
  * Every entry point takes caller-owned device pointers, plain sizes and an
  * explicit hipStream_t (passed as void*), launches asynchronously on that
- * stream, never synchronises and never allocates.  Each returns HP_OK (0) or
- * a negative HP_E* code; hp_last_error() then holds a message for the calling
- * thread.
+ * stream, never synchronises and never allocates device memory (the one
+ * measurement entry, hp_grid_step_fused_timed, waits for its own events).
+ * Each returns HP_OK (0) or a negative HP_E* code; hp_last_error() then holds
+ * a message for the calling thread.
  *
  * Two families:
  *  1. The reference's torch-extension op surface, on the reference's compact
@@ -214,6 +215,17 @@ int hp_grid_update_mode(const hp_grid_desc* g, int mode, void* stream);
  * every step and the result is left in (h, qx, qy).  Same results as
  * hp_grid_flux + hp_grid_update (HP_UPDATE_STEP); dh/dqx/dqy are not written. */
 int hp_grid_step_fused(const hp_grid_desc* g, int32_t nsteps, void* stream);
+
+/* Measurement form of hp_grid_step_fused (no reference counterpart; the
+ * reference times nothing on this path): nsteps single-pass steps exactly
+ * as hp_grid_step_fused runs them, with HIP events recorded on
+ * `stream` before each single-pass kernel, between it and its time control,
+ * and after the last time control.  Synchronises on the last event, then
+ * writes the summed single-pass kernel durations to *kernel_ms and the
+ * duration of all nsteps steps to *step_ms, so both come from the same
+ * launches (the state copy after an odd count falls outside both). */
+int hp_grid_step_fused_timed(const hp_grid_desc* g, int32_t nsteps, float* kernel_ms, float* step_ms,
+                             void* stream);
 
 /* Infiltration pass: cells listed by the flux rule of the current state
  * (the wet list of the flux pass) take the Green-Ampt increment; infil_dF

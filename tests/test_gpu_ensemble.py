@@ -33,7 +33,9 @@ SPECS = [(40, 2001, 80.0, 1800.0), (72, 2002, 120.0, 900.0), (100, 2003, 30.0, 3
 
 
 def state(s):
-    return [s.h.clone(), s.qx.clone(), s.qy.clone(), s.h_max.clone(), s.ctrl.clone()]
+    """State planes, h_max and the control fields a step defines (not
+    stopped_synced, which records the copy state of a stopped domain's planes)."""
+    return [s.h.clone(), s.qx.clone(), s.qy.clone(), s.h_max.clone(), s.ctrl[: len(_lib.CTRL_FIELDS)].clone()]
 
 
 @pytest.mark.parametrize("single_pass", [True, False])
@@ -113,3 +115,25 @@ def test_single_pass_ensemble_rejects_odd_step_counts(cuda):
     ss = [scenario(*sp, torch.float32) for sp in SPECS[:2]]
     with pytest.raises(_lib.HipimsError, match="even number of steps"):
         Ensemble(ss).step(3)
+
+
+def test_host_write_to_a_stopped_domain_survives_the_next_step(cuda):
+    """A stopped domain whose state the host edits between calls keeps the
+    edit through single-pass steps (odd and even counts)."""
+    say("[stopped domain, host edit]")
+    s = scenario(*SPECS[0], torch.float32)
+    s.set_t_stop(60.0)
+    s.ctrl[_lib.CTRL_EXPORT_T] = 60.0
+    while s.t < 60.0:
+        s.step_fused(2)
+    s.step_fused(2)  # both sets of planes now hold the stopped state
+    torch.cuda.synchronize()
+    assert s.ctrl[_lib.CTRL_STOPPED_SYNCED].item() == 1.0
+    for n in (1, 2, 3):
+        edit = s.h.clone()
+        edit[s.layout.ny // 2, : s.layout.nx] += 0.125
+        s.h.copy_(edit)
+        s.step_fused(n)
+        torch.cuda.synchronize()
+        assert torch.equal(s.h, edit), f"host edit lost after step_fused({n})"
+
