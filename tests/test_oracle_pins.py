@@ -1,0 +1,385 @@
+"""More pins of the CPU oracle (oracle/swe_oracle.cpp) by closed forms: the
+branches that the lake-at-rest, Stoker and rotation checks of
+test_oracle_closed_forms.py leave open.  Every expected value below is either
+an exact solution of the shallow-water equations or a number derived by hand
+for the given input (the derivation is in the docstring), so a slip in the
+oracle - a sign, a factor, the wrong side of a branch, the wrong table row -
+fails a check.
+
+Which oracle function / branch each test pins:
+  hllc, dry side (s_L = u_R - 2 a_R, s_R = u_L + 2 a_L) .. Ritter dry-bed dam
+        break (depth, velocity, wet front); closed-form one-dry-side fluxes
+        of the fall, H-given and WL-given edges
+  bc_ghost type 7 (Q given) ........ supercritical inflow: flux = the ghost's
+                                     physical flux
+  bc_ghost type 8 (WL given) ....... lake at rest on a sloped bed (h = wl - z,
+                                     not wl); level below the bed drains like
+                                     a dry ghost; equals type 6 with
+                                     h = wl - z
+  bc_ghost type 9 (fall) ........... a still cell drains at 2/3 h sqrt(g h)
+  series_row ....................... first row whose time >= t, no
+                                     interpolation, last row after the table
+  hydro_o1 / hydro_o2 wet/dry ...... lake at rest with emerging islands: no
+                                     mass flux (both orders), exact momentum
+                                     balance in order 1 and where the
+                                     reconstruction sees no thin wet cell in
+                                     order 2
+  MUSCL minmod (order 2) ........... supercritical flux = the left face of
+                                     q, on linear, monotone-convex and
+                                     extremum data
+  euler_update ..................... equals the frictionless update on cells
+                                     that stay wet; listed cells only;
+                                     wl = z + h
+"""
+import math
+
+import numpy as np
+import pytest
+
+from helpers import box, compact
+from hipims_amd import generators as gen
+from test_oracle_closed_forms import one_flux
+
+G = 9.81
+
+
+# --------------------------------------------------------------------------
+# Ritter: dam break onto a dry bed (the dry-side wave speeds)
+# --------------------------------------------------------------------------
+def ritter(x, t, h0, x0):
+    """Exact dry-bed dam break (Ritter), frictionless, flat bed: h0 at rest
+    for x < x0 at t = 0, dry beyond.  Depth and velocity at time t."""
+    c0 = math.sqrt(G * h0)
+    xi = (x - x0) / t
+    h = np.where(xi <= -c0, h0, np.where(xi < 2 * c0, (2 * c0 - xi) ** 2 / (9 * G), 0.0))
+    u = np.where(xi <= -c0, 0.0, np.where(xi < 2 * c0, 2.0 / 3.0 * (c0 + xi), 0.0))
+    return h, u
+
+
+def ritter_run(orc, order, nx=600, x0=200, h0=1.0, t_end=12.0):
+    """The oracle on a 3-row flume with slip walls, no friction, until t_end.
+    Returns (L1 depth error, velocity error in the rarefaction / 2 c0, wet
+    front position minus the exact position of the same depth)."""
+    ny = 3
+    h_in = np.where(np.arange(nx)[None, :] < x0, h0, 0.0) * np.ones((ny, 1))
+    mask, h, qx, qy, z = box(ny, nx, wall=40, h=h_in)
+    first = 0.5 / math.sqrt(G * h0)
+    sim = orc.OracleSim(mask, h, qx, qy, z, dx=1.0, order=order, first_dt=first, manning=[0.0])
+    while sim.time - sim.dt + first < t_end:  # t advances by the next dt (reference driver)
+        sim.run(1)
+    t = sim.time - sim.dt + first
+    xc = np.arange(nx) + 0.5
+    he, ue = ritter(xc, t, h0, x0)
+    got = sim.raster("h_c")[1:-1, 1:-1].mean(axis=0)
+    q = sim.raster("qx_c")[1:-1, 1:-1].mean(axis=0)
+    l1 = np.abs(got - he).sum() / np.abs(he).sum()
+    c0 = math.sqrt(G * h0)
+    mid = (xc - x0 > -0.5 * c0 * t) & (xc - x0 < 1.0 * c0 * t)
+    u_err = np.abs(q[mid] / got[mid] - ue[mid]).max() / (2 * c0)
+    thr = 1e-3 * h0
+    x_thr = x0 + (2 * c0 - 3 * math.sqrt(G * thr)) * t  # where the exact depth falls to thr
+    front = xc[np.nonzero(got > thr)[0].max()] - x_thr
+    return l1, u_err, front
+
+
+@pytest.mark.parametrize("order,l1_tol,u_tol,front_lo,front_hi", [(1, 0.010, 0.035, -16.0, 0.0),
+                                                                  (2, 0.004, 0.010, -2.0, 4.0)])
+def test_ritter_dry_bed_dam_break(oracle_lib, order, l1_tol, u_tol, front_lo, front_hi):
+    l1, u_err, front = ritter_run(oracle_lib, order)
+    assert l1 < l1_tol
+    assert u_err < u_tol
+    assert front_lo < front < front_hi
+
+
+# --------------------------------------------------------------------------
+# One-dry-side edges in closed form
+# --------------------------------------------------------------------------
+# For a still wet state (depth h, celerity a = sqrt(g h)) facing a dry state
+# across an edge with a flat bed, the wave speeds of the solver are
+#   wet side left:  s_L = min(-a, u* - a*) with u* = a, a* = a / 2  ->  -a,
+#                   s_R = u_L + 2 a_L = 2 a (dry right side), s_M = 2 a,
+# so the edge lies in the left star region and the mass / normal momentum
+# fluxes are  s_L s_R (0 - h) / (s_R - s_L) = 2/3 h a  and
+#   s_R (g h^2 / 2) / (s_R - s_L) = g h^2 / 3.
+#   wet side right: s_L = u_R - 2 a_R = -2 a (dry left side), s_R = a,
+#                   s_M = -2 a: the right star region, mass flux -2/3 h a.
+def drain_rate(h):
+    return 2.0 / 3.0 * h * math.sqrt(G * h)
+
+
+def single_row(n, west, east, wall=40):
+    """Mask of one valid row with the given west / east boundary codes; the
+    row's north and south edges are walls of type `wall`."""
+    mask = gen.ring_mask(1, n, wall=wall)
+    mask[1, 1] = west
+    mask[1, n] = east
+    return mask
+
+
+def one_edge_box(code):
+    """3 x 2 cells with slip walls, where only the middle row's east cell
+    carries `code`: its east edge is its one boundary edge.  Returns the
+    mask and that cell's compact index."""
+    mask = gen.ring_mask(3, 2, wall=40)
+    mask[2, 2] = code
+    return mask, int(np.flatnonzero(mask[mask > 0] == code)[0])
+
+
+def test_fall_edge_drains_a_still_cell(oracle_lib):
+    """Order 1 (the reference defines the fall type only there): still
+    water of one depth everywhere; the cell whose east edge is a fall edge
+    loses 2/3 h sqrt(g h) dt / dx of depth, and its x momentum gains the
+    west edge's pressure g h^2 / 2 minus the fall edge's g h^2 / 3."""
+    h0, dt, dx = 0.8, 0.01, 2.0
+    mask, k = one_edge_box(90)
+    h = gen.pad(np.full((3, 2), h0))
+    zero = gen.pad(np.zeros((3, 2)))
+    dh, dqx, dqy, _ = one_flux(oracle_lib, 1, mask, h, zero, zero, zero, dt=dt, dx=dx)
+    assert abs(dh[k] - (-drain_rate(h0) * dt / dx)) < 1e-15
+    assert abs(dqx[k] - (G * h0 * h0 / 6.0) * dt / dx) < 1e-14
+    assert abs(dqy[k]) < 1e-15
+    others = np.arange(dh.size) != k
+    assert np.abs(dh[others]).max() < 1e-15
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_wl_given_below_the_bed_drains_like_a_dry_ghost(oracle_lib, order):
+    """A WL-given level below the bed gives a dry ghost (h = max(wl, z) - z
+    = 0) with the cell's momentum (0 here): the still cell drains at
+    2/3 h sqrt(g h)."""
+    h0, zc, dt = 0.5, 1.25, 0.02
+    mask, k = one_edge_box(80)
+    h = gen.pad(np.full((3, 2), h0))
+    z = gen.pad(np.full((3, 2), zc))
+    zero = gen.pad(np.zeros((3, 2)))
+    gw = [[0.0, zc - 0.3], [1e9, zc - 0.3]]
+    dh, *_ = one_flux(oracle_lib, order, mask, h, zero, zero, z, dt=dt, gw=gw)
+    assert abs(dh[k] - (-drain_rate(h0) * dt)) < 1e-15
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_wl_given_lake_at_rest_on_a_sloped_bed(oracle_lib, order):
+    """WL given at the lake's own level on both ends of a sloped, still lake:
+    no flux anywhere.  The ghost depth is wl - z of the boundary cell (a
+    ghost of depth wl would drive a flux, the bed being 0.4 .. 1.1 m)."""
+    n, eta = 8, 2.0
+    mask = single_row(n, 80, 81)
+    zr = np.linspace(0.4, 1.1, n)[None, :]
+    z = gen.pad(zr)
+    h = gen.pad(eta - zr)
+    zero = gen.pad(np.zeros((1, n)))
+    gw = [[0.0, eta, eta], [1e9, eta, eta]]
+    dh, dqx, dqy, _ = one_flux(oracle_lib, order, mask, h, zero, zero, z, gw=gw)
+    assert np.abs(dh).max() < 1e-15
+    assert np.abs(dqx).max() < 1e-14 and np.abs(dqy).max() < 1e-14
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_wl_given_equals_h_given_of_the_same_level(oracle_lib, order):
+    """WL given at level w on a cell of bed z builds the same ghost as H
+    given with depth w - z, so every flux agrees."""
+    n = 6
+    rng = np.random.Generator(np.random.PCG64(21))
+    zr = rng.uniform(0.0, 0.5, (1, n))
+    hr = rng.uniform(0.2, 0.6, (1, n))
+    qr = rng.uniform(-0.3, 0.3, (1, n))
+    w = 1.6
+    z, h, qx = gen.pad(zr), gen.pad(hr), gen.pad(qr)
+    zero = gen.pad(np.zeros((1, n)))
+    a = one_flux(oracle_lib, order, single_row(n, 80, 50), h, qx, zero, z, gw=[[0.0, w], [1e9, w]])
+    b = one_flux(oracle_lib, order, single_row(n, 60, 50), h, qx, zero, z,
+                 gd=[[0.0, w - zr[0, 0]], [1e9, w - zr[0, 0]]])
+    for x, y in zip(a[:3], b[:3]):
+        assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_q_given_supercritical_inflow(oracle_lib, order):
+    """Q given (type 7) on the west end of a flume in uniform supercritical
+    flow (h = 1, u = 6 > sqrt(g)): the ghost keeps the cell's depth and takes
+    the given discharge q_g, both states are supercritical, so the west edge
+    passes the ghost's physical flux (q_g, q_g^2 / h + g h^2 / 2, q_g v / h)
+    and the east edge the cell's: dh = (q_g - 6) dt / dx."""
+    n, hh, u, qg, dt, dx = 8, 1.0, 6.0, 7.5, 0.01, 1.0
+    mask = single_row(n, 70, 50)
+    h = gen.pad(np.full((1, n), hh))
+    qx = gen.pad(np.full((1, n), hh * u))
+    zero = gen.pad(np.zeros((1, n)))
+    gq = [[0.0, qg, 0.0], [1e9, qg, 0.0]]
+    dh, dqx, dqy, _ = one_flux(oracle_lib, order, mask, h, qx, zero, zero, dt=dt, dx=dx, gq=gq)
+    F = lambda q: np.array([q, q * q / hh + 0.5 * G * hh * hh])
+    expect = -(F(hh * u) - F(qg)) * dt / dx
+    assert abs(dh[0] - expect[0]) < 1e-14
+    assert abs(dqx[0] - expect[1]) < 1e-13
+    assert np.abs(dh[1:]).max() < 1e-14  # uniform flow inside
+    assert np.abs(dqy).max() < 1e-14
+
+
+# --------------------------------------------------------------------------
+# Boundary series: the row a time selects
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("order", [1, 2])
+@pytest.mark.parametrize("t,depth", [(0.0, 0.2), (5.0, 0.6), (10.0, 0.6), (10.5, 1.0), (25.0, 1.0)])
+def test_series_row_is_the_first_with_time_at_or_after_t(oracle_lib, order, t, depth):
+    """H given with a 3-row series [(0, 0.2), (10, 0.6), (20, 1.0)] on the
+    west end of a dry flume: the boundary cell of the middle row (its only
+    boundary edge is the west one) receives the one-dry-side inflow
+    2/3 d sqrt(g d) dt / dx of the depth d of the first row whose time is
+    >= t (the last row after the table; no interpolation)."""
+    n, dt = 6, 0.01
+    mask = gen.ring_mask(3, n, wall=40)
+    mask[1:4, 1] = 60
+    zero = gen.pad(np.zeros((3, n)))
+    gd = [[0.0, 0.2], [10.0, 0.6], [20.0, 1.0]]
+    every = np.arange(int((mask > 0).sum()), dtype=np.int32)
+    dh, *_ = one_flux(oracle_lib, order, mask, zero, zero, zero, zero, t=t, dt=dt, gd=gd, listed=every)
+    mid = compact(mask, np.isin(np.arange(mask.size).reshape(mask.shape), [2 * mask.shape[1] + 1]), dtype=bool)
+    assert abs(dh[mid][0] - drain_rate(depth) * dt) < 1e-15
+
+
+# --------------------------------------------------------------------------
+# Wet/dry hydrostatic reconstruction: a lake at rest with emerging islands
+# --------------------------------------------------------------------------
+def island_lake(seed, eta, ny=14, nx=16):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    z = rng.uniform(-1.0, 1.0, (ny, nx))
+    return box(ny, nx, z=z, h=np.maximum(eta - z, 0.0)), z
+
+
+@pytest.mark.parametrize("order", [1, 2])
+@pytest.mark.parametrize("eta", [0.3, 0.0, -0.4])
+def test_lake_with_islands_has_no_mass_flux(oracle_lib, order, eta):
+    """Still water at one level around dry islands (bed above the level):
+    the reconstruction leaves no depth difference at any edge, wet/wet or
+    wet/dry, so no mass moves."""
+    (mask, h, qx, qy, z), zr = island_lake(9, eta)
+    assert 0.2 < (zr > eta).mean() < 0.8
+    dh, _, _, _ = one_flux(oracle_lib, order, mask, h, qx, qy, z)
+    assert np.abs(dh).max() < 1e-15
+
+
+@pytest.mark.parametrize("eta", [0.3, 0.0, -0.4])
+def test_lake_with_islands_order1_is_at_rest(oracle_lib, eta):
+    """Order 1 is well balanced across wet/dry edges: at a wet cell beside an
+    island the reconstruction dries the edge (no solver flux) and the bed
+    term supplies exactly the hydrostatic thrust g h^2 / 2."""
+    (mask, h, qx, qy, z), _ = island_lake(9, eta)
+    _, dqx, dqy, _ = one_flux(oracle_lib, 1, mask, h, qx, qy, z)
+    assert np.abs(dqx).max() < 1e-14 and np.abs(dqy).max() < 1e-14
+
+
+def test_order2_thin_cell_beside_an_island(oracle_lib):
+    """Order 2 on one row between slip walls: beds [0, 0, 0.5, 2, 2] under a
+    still level 0.6, so depths [0.6, 0.6, 0.1, 0, 0].  The reference's MUSCL
+    reconstructs depth and bed separately, so the thin cell 2 (depth 0.1
+    below its left neighbour's 0.6) is not in balance.  By hand, from the
+    minmod slopes, the clamp into the two cells' levels, the lift of the
+    higher cell's face and zb = max of the face beds:
+
+      cell 2, east edge (cell 3 dry, level 2): face depth 0.1 + 0.5 (-0.1)
+        = 0.05, bed 0.5 + 0.5 (0.5) = 0.75 (level 0.8, inside [0.6, 2]);
+        cell 3's face: depth 0, bed 2; the higher side keeps 2, zb = 2:
+        both faces dry, no solver flux; cell 2's face bed becomes its level
+        0.8: bed term 0.5 g (0.1 + 0) (0.8 - 0.5) = 0.015 g along +x.
+      cell 2, west edge (cell 1): face depth 0.1 + 0.5 (0.1) = 0.15, bed
+        0.5 + 0.5 (-0.5) = 0.25 (level 0.4, clamped up to 0.6); cell 1's
+        face: depth 0.6, bed 0 (its slopes vanish); zb = max(0.45, 0) =
+        0.45: both faces depth 0.15, still: pressure g 0.15^2 / 2 =
+        0.01125 g, and bed term 0.5 g (0.1 + 0.15) (0.45 - 0.5) =
+        -0.00625 g, both along the outward normal -x.
+      dqx(cell 2) = -(0.015 g) + 0.01125 g - 0.00625 g = -0.010 g  (x dt/dx)
+
+    Cells 0 and 1 balance (cell 1: pressure 0.18 g through its west edge
+    against pressure 0.01125 g plus bed term 0.16875 g through its east
+    edge), the dry cells see no flux, and no mass moves."""
+    n = 5
+    zr = np.array([[0.0, 0.0, 0.5, 2.0, 2.0]])
+    hr = np.maximum(0.6 - zr, 0.0)
+    mask = single_row(n, 40, 40)
+    h, z = gen.pad(hr), gen.pad(zr)
+    zero = gen.pad(np.zeros((1, n)))
+    dt, dx = 0.01, 1.0
+    dh, dqx, dqy, _ = one_flux(oracle_lib, 2, mask, h, zero, zero, z, dt=dt, dx=dx)
+    expect = np.array([0.0, 0.0, -0.010 * G, 0.0, 0.0]) * dt / dx
+    assert np.abs(dh).max() < 1e-15
+    assert np.abs(dqx - expect).max() < 1e-15
+    assert np.abs(dqy).max() < 1e-15
+    # order 1 keeps the same row at rest
+    _, dqx1, _, _ = one_flux(oracle_lib, 1, mask, h, zero, zero, z, dt=dt, dx=dx)
+    assert np.abs(dqx1).max() < 1e-15
+
+
+# --------------------------------------------------------------------------
+# MUSCL minmod faces (order 2)
+# --------------------------------------------------------------------------
+# Depth 0.1 m everywhere on a flat bed and discharges of 1 .. 33 m^2/s: every
+# state moves east at 10 .. 330 m/s against a celerity of about 1 m/s, so
+# each edge passes the physical mass flux of its west face, and with the
+# depth constant that flux is the face's discharge q_c + minmod(q_c - q_l,
+# q_r - q_c) / 2.  Then dh of cell i is -(face_i - face_{i-1}) dt / dx, with
+# the faces worked out by hand below for each data set.
+MUSCL_CASES = [
+    # linear: both one-sided slopes 0.25, every face q + 0.125
+    ("linear", [2.0 + 0.25 * i for i in range(8)], 4, -0.25),
+    # convex increasing: cell 4 (q 5) slopes 2 and 4 -> minmod 2, face 6;
+    # cell 3 (q 3) slopes 1 and 2 -> 1, face 3.5  (central slopes: -2.75)
+    ("monotone", [1.0, 1.5, 2.0, 3.0, 5.0, 9.0, 17.0, 33.0], 4, -2.5),
+    # local maximum at cell 4 (q 6): slopes 2 and -0.5 -> 0, face 6; cell 3
+    # (q 4) slopes 1 and 2 -> 1, face 4.5
+    ("extremum", [2.0, 2.0, 3.0, 4.0, 6.0, 5.5, 5.0, 5.0], 4, -1.5),
+    # just past the maximum, cell 5 (q 5.5): slopes -0.5 and -0.5 -> face
+    # 5.25, against the maximum's face 6
+    ("after_extremum", [2.0, 2.0, 3.0, 4.0, 6.0, 5.5, 5.0, 5.0], 5, 0.75),
+]
+
+
+@pytest.mark.parametrize("name,q,cell,rate", MUSCL_CASES, ids=[c[0] for c in MUSCL_CASES])
+def test_muscl_minmod_faces(oracle_lib, name, q, cell, rate):
+    n, dt, dx = len(q), 0.01, 1.0
+    mask = single_row(n, 50, 50)
+    h = gen.pad(np.full((1, n), 0.1))
+    qx = gen.pad(np.array([q]))
+    zero = gen.pad(np.zeros((1, n)))
+    dh, *_ = one_flux(oracle_lib, 2, mask, h, qx, zero, zero, dt=dt, dx=dx)
+    assert abs(dh[cell] - rate * dt / dx) < 1e-14
+
+
+# --------------------------------------------------------------------------
+# euler_update (euler_update_Kernel.cu:18-36)
+# --------------------------------------------------------------------------
+def test_euler_update_is_the_frictionless_update_on_listed_cells(oracle_lib):
+    """euler_update adds every increment to the listed cells only and sets
+    wl = h + z.  On cells that stay wet it must agree with the implicit
+    friction update run with n = 0 (no friction: q += dq there too), a
+    separately restated kernel; where a cell dries it keeps q + dq (the
+    friction update zeroes q instead)."""
+    rng = np.random.Generator(np.random.PCG64(31))
+    N = 64
+    h = rng.uniform(0.0, 1.0, N)
+    z = rng.uniform(-1.0, 1.0, N)
+    qx, qy = rng.normal(0.0, 0.3, N), rng.normal(0.0, 0.3, N)
+    dh, dqx, dqy = rng.normal(0.0, 0.05, N), rng.normal(0.0, 0.05, N), rng.normal(0.0, 0.05, N)
+    dh[:6] = -h[:6]                         # these cells dry out
+    dz = rng.normal(0.0, 0.01, N)
+    wet = np.sort(rng.choice(N, size=48, replace=False)).astype(np.int32)
+    wet = np.union1d(wet, np.arange(6)).astype(np.int32)
+    skip = np.setdiff1d(np.arange(N), wet)
+    E = [a.copy() for a in (h, h + z, z, qx, qy)]
+    oracle_lib.euler_update(wet, dh, dqx, dqy, dz, *E)
+    eh, ewl, ez, eqx, eqy = E
+    for new, old in ((eh, h), (ez, z), (eqx, qx), (eqy, qy)):
+        assert np.array_equal(new[skip], old[skip])
+    assert np.array_equal(ewl[skip], (h + z)[skip])
+    assert np.array_equal(ez[wet], z[wet] + dz[wet])
+    assert np.array_equal(ewl[wet], eh[wet] + ez[wet])
+    F = [a.copy() for a in (h, h + z, qx, qy, z)]
+    oracle_lib.friction_update(wet, dh.copy(), dqx.copy(), dqy.copy(), dz.copy(), np.zeros(N, np.uint8), *F,
+                               np.array([0.0]), 0.5)
+    fh, fwl, fqx, fqy, fz = F
+    stays = wet[eh[wet] >= 1e-6]
+    dries = wet[eh[wet] < 1e-6]
+    assert dries.size >= 6 and stays.size > 30
+    for e, f in ((eh, fh), (eqx, fqx), (eqy, fqy), (ez, fz), (ewl, fwl)):
+        assert np.array_equal(e[stays], f[stays])
+    assert np.array_equal(eqx[dries], qx[dries] + dqx[dries])
+    assert np.all(fqx[dries] == 0.0)
