@@ -31,6 +31,7 @@
 #include <math.h>
 #include <stdint.h>
 #include <cstring>
+#include <mutex>
 #include <utility>
 #include <string>
 #include <vector>
@@ -1215,6 +1216,38 @@ int grid_flux_launch(const hp_grid_desc* g, hipStream_t st) {
   return launch_flux(g, st);
 }
 
+// ---- allocation-free single-pass step ---------------------------------------
+// The single-pass step's only scratch memory is its per-tile partial records
+// (CFL minimum, FP64 sums), normally in the caller's workspace.  For callers
+// that cannot allocate, a file-scope device array holds them instead: every
+// GPU the library runs on keeps its own copy until the process ends.  Every
+// tile writes its record in every step before the time control reads it, so
+// nothing in it must start at zero; calls that overlap on one GPU share the
+// array and race.
+constexpr int64_t kScratchBytes = int64_t(8) << 20;
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+constexpr int kMaxScratchDevices = 64;
+static void* g_scratch[kMaxScratchDevices];
+static std::mutex g_scratch_mu;
+
+// Device address of the current device's scratch array, looked up once per
+// device (hipGetSymbolAddress with that device current).
+static int scratch_ptr(void** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(HP_ELAUNCH, std::string("hipGetDevice: ") + hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxScratchDevices) return fail(HP_EINVAL, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return fail(HP_ELAUNCH, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch[dev] = p;
+  }
+  *out = g_scratch[dev];
+  return HP_OK;
+}
+
 // After an odd number of single-pass steps the state sits in the second set
 // of planes: bring it back to the caller's (h, qx, qy).
 static int copy_state_back(const hp_grid_desc* g, hipStream_t st) {
@@ -1262,6 +1295,24 @@ int hp_grid_step_fused(const hp_grid_desc* g, int32_t nsteps, void* stream) {
   }
   if (nsteps & 1) return copy_state_back(g, st);
   return HP_OK;
+}
+
+int64_t hp_grid_noalloc_max_bytes(void) { return kScratchBytes; }
+
+int hp_grid_step_fused_noalloc(const hp_grid_desc* g, int32_t nsteps, void* stream) {
+  int rc = check_desc(g);
+  if (rc) return rc;
+  const int64_t need = partials_bytes(flux_tiles(g));
+  if (need > kScratchBytes)
+    return fail(HP_ESPACE, "domain needs " + std::to_string(need) + " bytes of per-tile records; the allocation-free "
+                           "step holds at most " + std::to_string(kScratchBytes) + " (use hp_grid_step_fused)");
+  void* p = nullptr;
+  rc = scratch_ptr(&p);
+  if (rc) return rc;
+  hp_grid_desc d = *g;
+  d.work = p;
+  d.work_bytes = kScratchBytes;
+  return hp_grid_step_fused(&d, nsteps, stream);
 }
 
 int hp_grid_step_fused_timed(const hp_grid_desc* g, int32_t nsteps, float* kernel_ms, float* step_ms,

@@ -216,6 +216,18 @@ int hp_grid_update_mode(const hp_grid_desc* g, int mode, void* stream);
  * hp_grid_flux + hp_grid_update (HP_UPDATE_STEP); dh/dqx/dqy are not written. */
 int hp_grid_step_fused(const hp_grid_desc* g, int32_t nsteps, void* stream);
 
+/* Allocation-free form of hp_grid_step_fused (no reference counterpart):
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The same
+ * kernels, with the per-tile partial records in a library-owned device
+ * array of hp_grid_noalloc_max_bytes() bytes (8 MiB, held per GPU until the
+ * process ends) instead of g->work, which is ignored.  A domain needing more
+ * (over about 200 000 flux tiles, e.g. beyond 8192 x 8192 cells at the
+ * default tile height) is rejected with HP_ESPACE.  Calls that overlap on one
+ * GPU share the array and race: serialise them. */
+int hp_grid_step_fused_noalloc(const hp_grid_desc* g, int32_t nsteps, void* stream);
+int64_t hp_grid_noalloc_max_bytes(void);
+
 /* Measurement form of hp_grid_step_fused (no reference counterpart; the
  * reference times nothing on this path): nsteps single-pass steps exactly
  * as hp_grid_step_fused runs them, with HIP events recorded on
