@@ -383,3 +383,45 @@ def test_euler_update_is_the_frictionless_update_on_listed_cells(oracle_lib):
         assert np.array_equal(e[stays], f[stays])
     assert np.array_equal(eqx[dries], qx[dries] + dqx[dries])
     assert np.all(fqx[dries] == 0.0)
+
+
+def test_order1_face_levels_average_when_their_order_flips(oracle_lib):
+    """Order 1 (fluxCal_jh_modified_surface_Kernel.cu:113-119): when the
+    edge's face water levels are ordered against the cells' levels, both
+    faces take their mean.  One row, beds [1.5, 1.0, 0.4, 0.0], depths
+    [0, 0.2, 0.75, 0.5], still.  Edge between cells 1 and 2 (levels 1.2 and
+    1.15): the bed faces are 1.0 + minmod(-0.5, -0.6) / 2 = 0.75 and
+    0.4 + minmod(0.4, 0.6) / 2 = 0.6, so the face levels 0.95 and 1.35 clamp
+    to 1.15 and 1.2 - the wrong way round - and become 1.175 each; beds
+    1.175 - 0.2 and 1.175 - 0.75, zb = 0.975, both depths 0.2: two equal
+    still states, no mass flux (without the mean the depths would be 0.2
+    and 0.25).  Cell 0 is a dry island above cell 1's level (no mass flux
+    either), so cell 1's depth does not change."""
+    zr = np.array([[1.5, 1.0, 0.4, 0.0]])
+    hr = np.array([[0.0, 0.2, 0.75, 0.5]])
+    mask = single_row(4, 40, 40)
+    zero = gen.pad(np.zeros((1, 4)))
+    dh, *_ = one_flux(oracle_lib, 1, mask, gen.pad(hr), zero, zero, gen.pad(zr))
+    assert dh[1] == 0.0
+    assert abs(dh[2]) > 1e-4  # the other edge of cell 2 does move water
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_contact_wave_upwinds_the_tangential_momentum(oracle_lib, order):
+    """Uniform depth 1 and eastward flow u = 0.5 (subsonic) with the
+    transverse velocity v jumping from +1 to -1 between cells 2 and 3.  The
+    middle wave speed of the solver, s_M, equals u for two states of equal
+    depth and normal velocity, so the edge's tangential flux is the mass
+    flux h u times the upwind (western) v: cell 2 sees 0.5 in and 0.5 out
+    (dqy = 0), cell 3 sees 0.5 in and -0.5 out (dqy = +1.0 dt / dx).  The
+    limiter keeps the jump sharp in order 2, so both orders agree."""
+    n, dt, dx = 6, 0.01, 1.0
+    mask = single_row(n, 50, 50, wall=50)  # open north / south edges: no wall meets the v flow
+    h = gen.pad(np.ones((1, n)))
+    qx = gen.pad(np.full((1, n), 0.5))
+    qy = gen.pad(np.array([[1.0, 1.0, 1.0, -1.0, -1.0, -1.0]]))
+    zero = gen.pad(np.zeros((1, n)))
+    dh, dqx, dqy, _ = one_flux(oracle_lib, order, mask, h, qx, qy, zero, dt=dt, dx=dx)
+    assert abs(dqy[2]) < 1e-15
+    assert abs(dqy[3] - 1.0 * dt / dx) < 1e-15
+    assert np.abs(dh[1:-1]).max() < 1e-15
