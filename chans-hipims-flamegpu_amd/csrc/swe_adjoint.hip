@@ -85,10 +85,9 @@ template <class T> struct AdjPlanes {
 
 template <class T>
 __device__ inline bool flux_listed(const hp_grid_desc& a, const T* h, int x, int y, T t) {
-  const T hs = T(kHs);
   const int P = (int)a.pitch;  // planes of fewer than 2^31 elements
   const int i = y * P + x;
-  if (t == T(0) || h[i] > hs) return true;
+  if (t == T(0) || gt_hs(h[i])) return true;
   const int nx[4] = {x + 1, x, x - 1, x};
   const int ny[4] = {y, y - 1, y, y + 1};
 #pragma unroll
@@ -97,7 +96,7 @@ __device__ inline bool flux_listed(const hp_grid_desc& a, const T* h, int x, int
     const int j = ny[k] * P + nx[k];
     if (a.type_slot[j] == 0) continue;
     if (nx[k] == a.cell0_x && ny[k] == a.cell0_y) continue;
-    if (h[j] > hs) return true;
+    if (gt_hs(h[j])) return true;
   }
   return false;
 }
@@ -119,9 +118,10 @@ __device__ inline bool adj_update_cell(int cls, T h0, T qx, T qy, T dh, T dqx, T
   const T rdt = rs * dt;
   const T dht = dh + rdt;
   if (!(fabs(dht) > T(0) || h0 >= T(0))) return false;  // not updated: lambda passes through
-  const T h1 = h0 + dht;
+  T h1 = h0;
+  const bool dry = new_depth_dry(h1, dht);
   T a_h1 = lh, a_qsx = T(0), a_qsy = T(0);
-  if (!(h1 < T(kHs))) {
+  if (!dry) {
     T a_hf = T(0), a_n = T(0);
     friction_vjp<T>(h1, s_man[cls], dt, qx + dqx, qy + dqy, lqx, lqy, a_qsx, a_qsy, a_hf, a_n);
     a_h1 += a_hf;

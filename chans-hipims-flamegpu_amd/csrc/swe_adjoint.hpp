@@ -48,8 +48,8 @@ __host__ __device__ inline bool solve_edge_vjp(const FrameCell<T>& c, const Fram
                                                FrameAdj<T>& al, FrameAdj<T>& ar, FrameAdj<T>& arr, T& a_etaC,
                                                T& a_etaR, bool frozen_slopes = false) {
   using P = Prec<T>;
-  const T hs = T(kHs), g = T(kG);
-  if (c.h < hs && r.h < hs) return false;
+  const T g = T(kG);
+  if (lt_hs(c.h) && lt_hs(r.h)) return false;
 
   // ---- forward: edge states ----
   FrameCell<T> eL, eR;
@@ -59,17 +59,17 @@ __host__ __device__ inline bool solve_edge_vjp(const FrameCell<T>& c, const Fram
     eR = r;
     if (ORDER == 1 && !r_is_ghost) {
       T rb = Prec<T>::limit(c.z - l.z, r.z - c.z);
-      if (r.h > hs) eL.z += T(0.5) * rb;
+      if (gt_hs(r.h)) eL.z += T(0.5) * rb;
       rb = Prec<T>::limit(r.z - rr.z, c.z - r.z);
-      if (c.h > hs) eR.z += T(0.5) * rb;
+      if (gt_hs(c.h)) eR.z += T(0.5) * rb;
     }
   } else {
     eL.h = c.h + T(0.5) * Prec<T>::limit(c.h - l.h, r.h - c.h);
     eL.z = c.z + T(0.5) * Prec<T>::limit(c.z - l.z, r.z - c.z);
     eR.h = r.h + T(0.5) * Prec<T>::limit(r.h - rr.h, c.h - r.h);
     eR.z = r.z + T(0.5) * Prec<T>::limit(r.z - rr.z, c.z - r.z);
-    zeroL = c.h < hs;
-    zeroR = r.h < hs;
+    zeroL = lt_hs(c.h);
+    zeroR = lt_hs(r.h);
     if (zeroL) { eL.qn = T(0); eL.qt = T(0); }
     else {
       eL.qn = c.qn + T(0.5) * Prec<T>::limit(c.qn - l.qn, r.qn - c.qn);
@@ -82,7 +82,7 @@ __host__ __device__ inline bool solve_edge_vjp(const FrameCell<T>& c, const Fram
     }
   }
   // ---- forward: velocities and hydrostatic reconstruction ----
-  const bool dryL0 = eL.h < hs, dryR0 = eR.h < hs;
+  const bool dryL0 = lt_hs(eL.h), dryR0 = lt_hs(eR.h);
   const T uL0 = dryL0 ? T(0) : P::div(eL.qn, eL.h), vL0 = dryL0 ? T(0) : P::div(eL.qt, eL.h);
   const T uR0 = dryR0 ? T(0) : P::div(eR.qn, eR.h), vR0 = dryR0 ? T(0) : P::div(eR.qt, eR.h);
   const T wL0 = eL.h + eL.z, wR0 = eR.h + eR.z;
@@ -110,7 +110,7 @@ __host__ __device__ inline bool solve_edge_vjp(const FrameCell<T>& c, const Fram
   const bool posL = wL3 - zbm > T(0), posR = wR3 - zbm > T(0);
   const T hL = posL ? wL3 - zbm : T(0), hR = posR ? wR3 - zbm : T(0);
   const T zL = wL3 - hL, zR = wR3 - hR;
-  const bool flux_on = !(hL < hs && hR < hs);
+  const bool flux_on = !(lt_hs(hL) && lt_hs(hR));
 
   const T s = dt / dx;
   const T Gb = T(0.5) * g * s;
@@ -133,7 +133,7 @@ __host__ __device__ inline bool solve_edge_vjp(const FrameCell<T>& c, const Fram
     const T base = (aL + aR) / T(2) + (uL0 - uR0) / T(4);
     const T u_star = (uL0 + uR0) / T(2) + aL - aR;
     const T a_star = P::star_celerity(base, g);
-    const bool sLdry = hL <= hs, sRdry = hR <= hs;
+    const bool sLdry = le_hs(hL), sRdry = le_hs(hR);
     const T x1 = uL0 - aL, x2 = u_star - a_star;
     const T y1 = uR0 + aR, y2 = u_star + a_star;
     const bool sLx1 = x1 <= x2, sRy1 = y1 >= y2;

@@ -112,15 +112,14 @@ __global__ void k_flux_mask(int64_t N, uint8_t* __restrict__ mask, const T* __re
                             const int32_t* __restrict__ index, const T* tp) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
-  const T hs = T(kHs);
   if (tp[0] == T(0)) {
     mask[i] = 1;
     return;
   }
-  bool m = h[i] > hs;
+  bool m = gt_hs(h[i]);
   for (int k = 1; k < 5; ++k) {
     const int32_t n = index[(int64_t)k * N + i];
-    if (n > 0) m = m || (h[n] > hs);  // id 0 never counts (fluxMaskGenerator_Kernel.cu:36)
+    if (n > 0) m = m || gt_hs(h[n]);  // id 0 never counts (fluxMaskGenerator_Kernel.cu:36)
   }
   mask[i] = m ? 1 : 0;
 }
@@ -143,9 +142,10 @@ __global__ void k_friction_compact(int64_t K, const int32_t* __restrict__ wet, T
   if (j >= K) return;
   const int32_t i = wet[j];
   const T dt = dtp[0];
-  const T hi = h[i] + dh[i];
+  T hi = h[i];
+  const bool dry = new_depth_dry(hi, dh[i]);
   T qxi, qyi;
-  if (hi < T(kHs)) {
+  if (dry) {
     qxi = T(0);
     qyi = T(0);
   } else {

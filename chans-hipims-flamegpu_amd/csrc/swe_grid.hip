@@ -97,6 +97,34 @@ template <class T> __device__ inline void to_global(T fn, T ft, int d, T& gx, T&
 }
 
 // --- reductions ------------------------------------------------------------
+// Time control of one domain from its step's folded CFL minimum and FP64
+// sums (one thread): dt = min of the candidates (kept when no cell was
+// listed, SWE_CUDA.py:818-822), capped at 10x the previous dt, clamped onto
+// the next export time, then t advanced by it (SWE_CUDA.py:824-831).
+__device__ inline void advance_ctrl(hp_ctrl* ctrl, double mn, double vol, double fs, double rs, double dx, bool sums) {
+  const double area = dx * dx;
+  const double t = ctrl->t;
+  const double dt_old = ctrl->dt;
+  double dt_new = isinf(mn) ? dt_old : mn;
+  const double upper = dt_old * 10.0;
+  dt_new = (upper < dt_new) ? upper : dt_new;
+  const double next_export = (ctrl->export_n + 1.0) * ctrl->export_T;
+  if (dt_new + t >= next_export) {
+    dt_new = next_export - t;
+    ctrl->export_n += 1.0;
+    ctrl->export_pending += 1.0;
+  }
+  ctrl->dt = dt_new;
+  ctrl->t = t + dt_new;
+  if (sums) {
+    ctrl->volume = vol * area;
+    ctrl->flux_volume += fs * area;
+    ctrl->rain_volume += rs * area;
+  }
+  ctrl->steps += 1.0;
+  ctrl->stopped_synced = 0.0;
+}
+
 // Per-block results of a pass that ends a step, in the workspace: the CFL
 // minimum of every block in one array (all the finalize pass reads when the
 // mass sums are off), then, when they are on, one FP64 record per block.
@@ -315,8 +343,7 @@ __device__ inline void update_state(const CellIn<T>& in, T& h, T& qx, T& qy, T d
     vol_h = h;
     return;
   }
-  h += dht;
-  if (h < T(kHs)) {
+  if (new_depth_dry(h, dht)) {
     qx = T(0);
     qy = T(0);
   } else {
@@ -496,7 +523,6 @@ __device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by, const Pa
   }
   __syncthreads();
 
-  const T hs = T(kHs);
   const bool t_zero = (t == T(0));
   T* __restrict__ dh = (T*)a.dh;
   T* __restrict__ dqx = (T*)a.dqx;
@@ -518,12 +544,12 @@ __device__ inline void flux_tile(const hp_grid_desc& a, int bx, int by, const Pa
       // is not compact cell 0 (fluxMaskGenerator_Kernel.cu:36); the
       // neighbours are looked at only for a dry cell (the four LDS reads of
       // that case issue together)
-      listed = t_zero | (s.h[r][c] > hs);
+      listed = t_zero | gt_hs(s.h[r][c]);
       if (!listed) {
-        const bool w_e = (s.ty[r][c + 1] != 0) & (s.h[r][c + 1] > hs) & !(gx + 1 == a.cell0_x && gy == a.cell0_y);
-        const bool w_n = (s.ty[r - 1][c] != 0) & (s.h[r - 1][c] > hs) & !(gx == a.cell0_x && gy - 1 == a.cell0_y);
-        const bool w_w = (s.ty[r][c - 1] != 0) & (s.h[r][c - 1] > hs) & !(gx - 1 == a.cell0_x && gy == a.cell0_y);
-        const bool w_s = (s.ty[r + 1][c] != 0) & (s.h[r + 1][c] > hs) & !(gx == a.cell0_x && gy + 1 == a.cell0_y);
+        const bool w_e = (s.ty[r][c + 1] != 0) & gt_hs(s.h[r][c + 1]) & !(gx + 1 == a.cell0_x && gy == a.cell0_y);
+        const bool w_n = (s.ty[r - 1][c] != 0) & gt_hs(s.h[r - 1][c]) & !(gx == a.cell0_x && gy - 1 == a.cell0_y);
+        const bool w_w = (s.ty[r][c - 1] != 0) & gt_hs(s.h[r][c - 1]) & !(gx - 1 == a.cell0_x && gy == a.cell0_y);
+        const bool w_s = (s.ty[r + 1][c] != 0) & gt_hs(s.h[r + 1][c]) & !(gx == a.cell0_x && gy + 1 == a.cell0_y);
         listed = w_e | w_n | w_w | w_s;
       }
       if (listed) {
@@ -632,6 +658,12 @@ __global__ __launch_bounds__(kThreads) void k_step_fused(hp_grid_desc a, int ntx
                                 (flags & 2) == 0, (flags & 4) != 0);
 }
 
+}  // namespace hp
+
+#include "swe_walk.hpp"
+
+namespace hp {
+
 // Infiltration pass over plane elements: a valid cell listed by the flux
 // rule of the current state (t == 0, wet, or a wet valid neighbour E, N, W,
 // S that is not compact cell 0: the wet list the reference's
@@ -647,10 +679,9 @@ __global__ __launch_bounds__(kThreads) void k_infiltration(hp_grid_desc a) {
   const T* __restrict__ h = (const T*)a.h;
   const uint8_t* __restrict__ ty = a.type_slot;
   const int row = (int)(i / pitch), col = (int)(i - row * pitch);
-  const T hs = T(kHs);
   T dF = T(0);
   if (col < a.nx && ty[i] != 0) {
-    bool listed = T(a.ctrl->t) == T(0) || h[i] > hs;
+    bool listed = T(a.ctrl->t) == T(0) || gt_hs(h[i]);
     const int nr[4] = {row, row - 1, row, row + 1};
     const int nc[4] = {col + 1, col, col - 1, col};
 #pragma unroll
@@ -658,7 +689,7 @@ __global__ __launch_bounds__(kThreads) void k_infiltration(hp_grid_desc a) {
       if (nr[k] < 0 || nr[k] >= a.ny || nc[k] < 0 || nc[k] >= a.nx) continue;
       const int64_t j = (int64_t)nr[k] * pitch + nc[k];
       const bool is0 = nc[k] == a.cell0_x && nr[k] == a.cell0_y;
-      listed = listed || (ty[j] != 0 && !is0 && h[j] > hs);
+      listed = listed || (ty[j] != 0 && !is0 && gt_hs(h[j]));
     }
     if (listed) {
       const int cls = a.landuse[i];
@@ -691,7 +722,6 @@ __device__ inline void update_chunk(const hp_grid_desc& a, int64_t chunk, const 
     return;
   }
   const T dt = T(a.ctrl->dt);
-  const T hs = T(kHs);
   const T cfl_dx = T(a.cfl) * T(a.dx);
   for (int k = tid; k < a.n_classes; k += kThreads) s_man[k] = ((const T*)a.manning)[k];
   const RainRows rr = rain_rows(a, a.ctrl->t);
@@ -751,7 +781,7 @@ __device__ inline void update_chunk(const hp_grid_desc& a, int64_t chunk, const 
       if (cls == 255) continue;
       T h = vh.x[j];
       if (MODE == HP_UPDATE_CFL) {
-        if (h > hs) {
+        if (gt_hs(h)) {
           vhm.x[j] = fmax(h, vhm.x[j]);
           mn = fmin(mn, cfl_candidate<T>(h, vqx.x[j], vqy.x[j], cfl_dx));
         }
@@ -761,17 +791,17 @@ __device__ inline void update_chunk(const hp_grid_desc& a, int64_t chunk, const 
       const T dhf = vdh.x[j];
       const T rdt = vr.x[j] * dt;
       const T dht = (dhf + rdt) - vdf.x[j];  // flux, + rain, - infiltration (the reference's order)
-      const bool listed = (MODE == HP_UPDATE_EULER) ? (fabs(dht) > T(0) || h > hs) : (fabs(dht) > T(0) || h >= T(0));
+      const bool listed = (MODE == HP_UPDATE_EULER) ? (fabs(dht) > T(0) || gt_hs(h)) : (fabs(dht) > T(0) || h >= T(0));
       if (!listed) {
         if (sums) vol += (double)h;
         continue;
       }
-      h += dht;
+      const bool dry = new_depth_dry(h, dht);
       T qx, qy;
       if (MODE == HP_UPDATE_EULER) {
         qx = vqx.x[j] + vdqx.x[j];
         qy = vqy.x[j] + vdqy.x[j];
-      } else if (h < hs) {
+      } else if (dry) {
         qx = T(0);
         qy = T(0);
       } else {
@@ -875,27 +905,7 @@ __device__ inline void finalize_domain(hp_ctrl* ctrl, const double* __restrict__
       fs += s_red[2][k];
       rs += s_red[3][k];
     }
-    const double area = dx * dx;
-    const double t = ctrl->t;
-    const double dt_old = ctrl->dt;
-    double dt_new = isinf(mn) ? dt_old : mn;  // no listed cell: keep dt (SWE_CUDA.py:818-822)
-    const double upper = dt_old * 10.0;
-    dt_new = (upper < dt_new) ? upper : dt_new;
-    const double next_export = (ctrl->export_n + 1.0) * ctrl->export_T;
-    if (dt_new + t >= next_export) {
-      dt_new = next_export - t;
-      ctrl->export_n += 1.0;
-      ctrl->export_pending += 1.0;
-    }
-    ctrl->dt = dt_new;
-    ctrl->t = t + dt_new;
-    if (sums) {
-      ctrl->volume = vol * area;
-      ctrl->flux_volume += fs * area;
-      ctrl->rain_volume += rs * area;
-    }
-    ctrl->steps += 1.0;
-    ctrl->stopped_synced = 0.0;
+    advance_ctrl(ctrl, mn, vol, fs, rs, dx, sums);
   }
 }
 
@@ -947,14 +957,19 @@ static int64_t update_blocks(const hp_grid_desc* g) {
   return (total + kChunk - 1) / kChunk;
 }
 
+// Rows per flux work unit: FP32 column-walk units of 60 columns x 4, 8 or
+// 16 rows (default 8); FP64 LDS tiles of 64 x 4 cells.
 static int flux_rows(const hp_grid_desc* g) {
   if (g->dtype == HP_F32) return (g->tile_rows == 4 || g->tile_rows == 16) ? g->tile_rows : 8;
   return 4;
 }
 
+static int flux_cols(const hp_grid_desc* g) { return g->dtype == HP_F32 ? kWalkOwn : kTX; }
+
+// Flux work units of a domain: one partial record each in the single-pass step.
 static int64_t flux_tiles(const hp_grid_desc* g) {
-  const int ty = flux_rows(g);
-  return (int64_t)((g->nx + kTX - 1) / kTX) * (int64_t)((g->ny + ty - 1) / ty);
+  const int ty = flux_rows(g), tx = flux_cols(g);
+  return (int64_t)((g->nx + tx - 1) / tx) * (int64_t)((g->ny + ty - 1) / ty);
 }
 
 static int check_desc(const hp_grid_desc* g) {
@@ -996,6 +1011,8 @@ static int check_desc(const hp_grid_desc* g) {
   if ((uintptr_t)g->infil_dF % ((uintptr_t)kVec * (g->dtype == HP_F32 ? 4 : 8)))
     return fail(HP_EINVAL, "infil_dF must be aligned to 4 elements");
   if (flux_tiles(g) > 0x7fffffffLL || update_blocks(g) > 0x7fffffffLL) return fail(HP_EINVAL, "domain too large");
+  if (g->dtype == HP_F32 && g->pitch * (int64_t)g->ny > 0xffffffffLL)
+    return fail(HP_EINVAL, "float planes are limited to 2^32 elements");
   return HP_OK;
 }
 
@@ -1006,19 +1023,28 @@ static int launch_flux_single(const hp_grid_desc* g, hipStream_t st) {
   return check_launch("k_flux");
 }
 
-template <class T, int ORDER>
-static int launch_flux_t(const hp_grid_desc* g, hipStream_t st) {
-  const int ty = flux_rows(g);
-  if (sizeof(T) == 4) {
-    if (ty == 4) return launch_flux_single<T, ORDER, 4>(g, st);
-    return ty == 16 ? launch_flux_single<T, ORDER, 16>(g, st) : launch_flux_single<T, ORDER, 8>(g, st);
+// FP32: column-walk units (k_walk), 64-lane workgroups.
+template <int ORDER, int R, bool FUSED>
+static int launch_walk_r(const hp_grid_desc* g, hipStream_t st, int flags) {
+  const int ns = (g->nx + kWalkOwn - 1) / kWalkOwn, nb = (g->ny + R - 1) / R;
+  hipLaunchKernelGGL((k_walk<ORDER, R, FUSED>), dim3(ns * nb), dim3(kWalkLanes), 0, st, *g, ns, flags);
+  return check_launch(FUSED ? "k_walk (single-pass step)" : "k_walk (flux pass)");
+}
+
+template <bool FUSED>
+static int launch_walk(const hp_grid_desc* g, hipStream_t st, int flags) {
+  const int r = flux_rows(g);
+  if (g->order == 2) {
+    if (r == 4) return launch_walk_r<2, 4, FUSED>(g, st, flags);
+    return r == 16 ? launch_walk_r<2, 16, FUSED>(g, st, flags) : launch_walk_r<2, 8, FUSED>(g, st, flags);
   }
-  return launch_flux_single<T, ORDER, 4>(g, st);
+  if (r == 4) return launch_walk_r<1, 4, FUSED>(g, st, flags);
+  return r == 16 ? launch_walk_r<1, 16, FUSED>(g, st, flags) : launch_walk_r<1, 8, FUSED>(g, st, flags);
 }
 
 static int launch_flux(const hp_grid_desc* g, hipStream_t st) {
-  if (g->dtype == HP_F32) return g->order == 2 ? launch_flux_t<float, 2>(g, st) : launch_flux_t<float, 1>(g, st);
-  return g->order == 2 ? launch_flux_t<double, 2>(g, st) : launch_flux_t<double, 1>(g, st);
+  if (g->dtype == HP_F32) return launch_walk<false>(g, st, 0);
+  return g->order == 2 ? launch_flux_single<double, 2, 4>(g, st) : launch_flux_single<double, 1, 4>(g, st);
 }
 
 template <class T, int ORDER, int TY>
@@ -1029,15 +1055,7 @@ static int launch_fused_t(const hp_grid_desc* g, hipStream_t st, int flags) {
 }
 
 static int launch_step_fused(const hp_grid_desc* g, hipStream_t st, int flags = 0) {
-  const int ty = flux_rows(g);
-  if (g->dtype == HP_F32) {
-    if (g->order == 2) {
-      if (ty == 4) return launch_fused_t<float, 2, 4>(g, st, flags);
-      return ty == 16 ? launch_fused_t<float, 2, 16>(g, st, flags) : launch_fused_t<float, 2, 8>(g, st, flags);
-    }
-    if (ty == 4) return launch_fused_t<float, 1, 4>(g, st, flags);
-    return ty == 16 ? launch_fused_t<float, 1, 16>(g, st, flags) : launch_fused_t<float, 1, 8>(g, st, flags);
-  }
+  if (g->dtype == HP_F32) return launch_walk<true>(g, st, flags);
   if (g->order == 2) return launch_fused_t<double, 2, 4>(g, st, flags);
   return launch_fused_t<double, 1, 4>(g, st, flags);
 }
@@ -1134,15 +1152,27 @@ static void launch_fused_multi(const hp_ensemble* e, int parity, hipStream_t st)
                      e->partials, (int)e->n_flux_tiles);
 }
 
+template <int ORDER, int R, bool FUSED>
+static void launch_walk_multi(const hp_ensemble* e, int parity, hipStream_t st) {
+  hipLaunchKernelGGL((k_walk_multi<ORDER, R, FUSED>), dim3((unsigned)e->n_flux_tiles), dim3(kWalkLanes), 0, st,
+                     (const hp_grid_desc*)e->doms + (parity ? e->n_domains : 0), (const int2*)e->flux_map,
+                     e->partials, (int)e->n_flux_tiles);
+}
+
+template <bool FUSED>
+static void launch_walk_multi_any(const hp_ensemble* e, int parity, hipStream_t st) {
+  const int r = e->tile_rows;
+  if (e->order == 2)
+    (r == 4 ? launch_walk_multi<2, 4, FUSED> : r == 16 ? launch_walk_multi<2, 16, FUSED> : launch_walk_multi<2, 8, FUSED>)(e, parity, st);
+  else
+    (r == 4 ? launch_walk_multi<1, 4, FUSED> : r == 16 ? launch_walk_multi<1, 16, FUSED> : launch_walk_multi<1, 8, FUSED>)(e, parity, st);
+}
+
 // One single-pass step of every domain: even steps read the domains' own
 // planes and write their second set, odd steps the reverse.
 static int ensemble_fused_step_once(const hp_ensemble* e, int parity, hipStream_t st) {
-  const int ty = e->tile_rows;
   if (e->dtype == HP_F32) {
-    if (e->order == 2)
-      (ty == 4 ? launch_fused_multi<float, 2, 4> : ty == 16 ? launch_fused_multi<float, 2, 16> : launch_fused_multi<float, 2, 8>)(e, parity, st);
-    else
-      (ty == 4 ? launch_fused_multi<float, 1, 4> : ty == 16 ? launch_fused_multi<float, 1, 16> : launch_fused_multi<float, 1, 8>)(e, parity, st);
+    launch_walk_multi_any<true>(e, parity, st);
   } else {
     if (e->order == 2) launch_fused_multi<double, 2, 4>(e, parity, st);
     else launch_fused_multi<double, 1, 4>(e, parity, st);
@@ -1157,12 +1187,8 @@ static int ensemble_fused_step_once(const hp_ensemble* e, int parity, hipStream_
 
 static int ensemble_step_once(const hp_ensemble* e, hipStream_t st) {
   const bool f32 = e->dtype == HP_F32;
-  const int ty = e->tile_rows;
   if (f32) {
-    if (e->order == 2)
-      (ty == 4 ? launch_flux_multi<float, 2, 4> : ty == 16 ? launch_flux_multi<float, 2, 16> : launch_flux_multi<float, 2, 8>)(e, st);
-    else
-      (ty == 4 ? launch_flux_multi<float, 1, 4> : ty == 16 ? launch_flux_multi<float, 1, 16> : launch_flux_multi<float, 1, 8>)(e, st);
+    launch_walk_multi_any<false>(e, 0, st);
   } else {
     if (e->order == 2) launch_flux_multi<double, 2, 4>(e, st);
     else launch_flux_multi<double, 1, 4>(e, st);
@@ -1248,6 +1274,31 @@ static int scratch_ptr(void** out) {
   return HP_OK;
 }
 
+// Workspace of a domain: the partial records of the larger of the update
+// pass and the flux / single-pass step.
+static int64_t workspace_need(const hp_grid_desc* g) {
+  const int64_t nt = flux_tiles(g), nu = update_blocks(g);
+  return partials_bytes(nu > nt ? nu : nt);
+}
+
+// The checks every single-pass entry point makes.
+static int check_fused(const hp_grid_desc* g) {
+  int rc = check_desc(g);
+  if (rc) return rc;
+  if (!g->h_out || !g->qx_out || !g->qy_out) return fail(HP_EINVAL, "the fused step needs h_out, qx_out, qy_out");
+  if (!g->work || g->work_bytes < workspace_need(g))
+    return fail(HP_ESPACE, "workspace smaller than hp_grid_workspace_bytes()");
+  return HP_OK;
+}
+
+// One single-pass step and its time control.  first: the first step of a
+// call (a stopped domain copies its state through even when marked in step).
+static int single_pass_step(const hp_grid_desc* d, hipStream_t st, bool first) {
+  int rc = launch_step_fused(d, st, first ? 4 : 0);
+  if (rc) return rc;
+  return launch_finalize_fused(d, st);
+}
+
 // After an odd number of single-pass steps the state sits in the second set
 // of planes: bring it back to the caller's (h, qx, qy).
 static int copy_state_back(const hp_grid_desc* g, hipStream_t st) {
@@ -1267,27 +1318,20 @@ extern "C" {
 
 int64_t hp_grid_workspace_bytes(const hp_grid_desc* g) {
   if (!g || g->pitch <= 0 || g->ny <= 0 || g->nx <= 0) return 0;
-  const int64_t n = update_blocks(g) > flux_tiles(g) ? update_blocks(g) : flux_tiles(g);
-  return partials_bytes(n);
+  return workspace_need(g);
 }
 
 int hp_grid_step_fused(const hp_grid_desc* g, int32_t nsteps, void* stream) {
-  int rc = check_desc(g);
+  int rc = check_fused(g);
   if (rc) return rc;
   if (nsteps < 0) return fail(HP_EINVAL, "nsteps must be >= 0");
-  if (!g->h_out || !g->qx_out || !g->qy_out) return fail(HP_EINVAL, "the fused step needs h_out, qx_out, qy_out");
-  const int64_t nt = flux_tiles(g);
-  if (!g->work || g->work_bytes < partials_bytes(nt))
-    return fail(HP_ESPACE, "workspace smaller than hp_grid_workspace_bytes()");
   hipStream_t st = (hipStream_t)stream;
   hp_grid_desc d = *g;
   for (int32_t k = 0; k < nsteps; ++k) {
     // the first step of a call copies a stopped domain's state through even
     // if an earlier call left both sets in step: the host may have written
     // the state in between
-    rc = launch_step_fused(&d, st, k == 0 ? 4 : 0);
-    if (rc) return rc;
-    rc = launch_finalize_fused(&d, st);
+    rc = single_pass_step(&d, st, k == 0);
     if (rc) return rc;
     std::swap(d.h, d.h_out);
     std::swap(d.qx, d.qx_out);
@@ -1302,7 +1346,7 @@ int64_t hp_grid_noalloc_max_bytes(void) { return kScratchBytes; }
 int hp_grid_step_fused_noalloc(const hp_grid_desc* g, int32_t nsteps, void* stream) {
   int rc = check_desc(g);
   if (rc) return rc;
-  const int64_t need = partials_bytes(flux_tiles(g));
+  const int64_t need = workspace_need(g);
   if (need > kScratchBytes)
     return fail(HP_ESPACE, "domain needs " + std::to_string(need) + " bytes of per-tile records; the allocation-free "
                            "step holds at most " + std::to_string(kScratchBytes) + " (use hp_grid_step_fused)");
@@ -1317,13 +1361,10 @@ int hp_grid_step_fused_noalloc(const hp_grid_desc* g, int32_t nsteps, void* stre
 
 int hp_grid_step_fused_timed(const hp_grid_desc* g, int32_t nsteps, float* kernel_ms, float* step_ms,
                              void* stream) {
-  int rc = check_desc(g);
+  int rc = check_fused(g);
   if (rc) return rc;
   if (nsteps <= 0) return fail(HP_EINVAL, "nsteps must be positive");
   if (!kernel_ms || !step_ms) return fail(HP_EINVAL, "kernel_ms and step_ms are required");
-  if (!g->h_out || !g->qx_out || !g->qy_out) return fail(HP_EINVAL, "the fused step needs h_out, qx_out, qy_out");
-  if (!g->work || g->work_bytes < partials_bytes(flux_tiles(g)))
-    return fail(HP_ESPACE, "workspace smaller than hp_grid_workspace_bytes()");
   hipStream_t st = (hipStream_t)stream;
   // ev[2k]: before step k's single-pass kernel, ev[2k+1]: between it and its
   // time control, ev[2 nsteps]: after the last time control

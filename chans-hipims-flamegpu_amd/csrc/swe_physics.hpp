@@ -40,6 +40,31 @@ namespace hp {
 constexpr double kG = 9.81;
 constexpr double kHs = 1.0e-6;
 
+// Wet/dry threshold tests of a depth against h_small = 1e-6 m, decided as
+// FP64 comparisons in both precisions.  For a float x no FP64 arithmetic is
+// needed: fl(1e-6) = 9.99999997e-7 lies just below 1e-6 and no float lies
+// between the two, so x < 1e-6 (and x <= 1e-6) exactly when x <= fl(1e-6),
+// and x > 1e-6 (and x >= 1e-6) exactly when x > fl(1e-6).
+__host__ __device__ inline bool lt_hs(double x) { return x < kHs; }
+__host__ __device__ inline bool le_hs(double x) { return x <= kHs; }
+__host__ __device__ inline bool gt_hs(double x) { return x > kHs; }
+__host__ __device__ inline bool lt_hs(float x) { return x <= float(kHs); }
+__host__ __device__ inline bool le_hs(float x) { return x <= float(kHs); }
+__host__ __device__ inline bool gt_hs(float x) { return x > float(kHs); }
+
+// h += dh, and whether the new depth is below h_small, decided on the exact
+// sum in FP64 (the sum of two floats is exact in double, and rounding it to
+// float gives the float sum).
+__host__ __device__ inline bool new_depth_dry(double& h, double dh) {
+  h += dh;
+  return h < kHs;
+}
+__host__ __device__ inline bool new_depth_dry(float& h, float dh) {
+  const double hn = (double)h + (double)dh;
+  h = (float)hn;
+  return hn < kHs;
+}
+
 template <class T> __host__ __device__ inline T minmod(T x, T y) {
   const T m = fabs(x) < fabs(y) ? x : y;
   return x * y <= T(0) ? T(0) : m;
@@ -50,6 +75,8 @@ template <> struct Prec<double> {
   static constexpr bool local_datum = false;
   __host__ __device__ static inline double limit(double x, double y) { return minmod<double>(x, y); }
   __host__ __device__ static inline double clamp(double x, double lo, double hi) { return fmax(fmin(x, hi), lo); }
+  __host__ __device__ static inline double max2(double x, double y) { return fmax(x, y); }
+  __host__ __device__ static inline double min2(double x, double y) { return fmin(x, y); }
   __host__ __device__ static inline double div(double a, double b) { return a / b; }
   __host__ __device__ static inline double sqrt_(double x) { return __builtin_sqrt(x); }
   __host__ __device__ static inline double rcbrt_friction(double h) { return pow(h, -1.0 / 3.0); }
@@ -76,6 +103,10 @@ template <> struct Prec<float> {
   __device__ static inline float limit(float x, float y) { return __builtin_amdgcn_fmed3f(x, y, 0.0f); }
   // clamp to [lo, hi] (lo <= hi) as one median-of-three
   __device__ static inline float clamp(float x, float lo, float hi) { return __builtin_amdgcn_fmed3f(x, lo, hi); }
+  // max / min as a median with an infinity: no quieting of the (loaded,
+  // possibly signalling) inputs that fmaxf / fminf need in IEEE mode
+  __device__ static inline float max2(float x, float y) { return __builtin_amdgcn_fmed3f(x, y, INFINITY); }
+  __device__ static inline float min2(float x, float y) { return __builtin_amdgcn_fmed3f(x, y, -INFINITY); }
   __device__ static inline float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
   __device__ static inline float sqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
   // h^(-1/3) = exp2(-log2(h) / 3) with the hardware log2 / exp2 (h >= 1e-6)
@@ -198,16 +229,16 @@ __host__ __device__ inline void hllc_edge(FrameCell<T>& eL, FrameCell<T>& eR, T 
   // under divergent branches.  The selected values are the same expressions
   // the branches would produce.
   using P = Prec<T>;
-  const T hs = T(kHs), g = T(kG);
-  const T uL0 = eL.h < hs ? T(0) : P::div(eL.qn, eL.h);
-  const T vL0 = eL.h < hs ? T(0) : P::div(eL.qt, eL.h);
-  const T uR0 = eR.h < hs ? T(0) : P::div(eR.qn, eR.h);
-  const T vR0 = eR.h < hs ? T(0) : P::div(eR.qt, eR.h);
+  const T g = T(kG);
+  const T uL0 = lt_hs(eL.h) ? T(0) : P::div(eL.qn, eL.h);
+  const T vL0 = lt_hs(eL.h) ? T(0) : P::div(eL.qt, eL.h);
+  const T uR0 = lt_hs(eR.h) ? T(0) : P::div(eR.qn, eR.h);
+  const T vR0 = lt_hs(eR.h) ? T(0) : P::div(eR.qt, eR.h);
 
   // hydrostatic reconstruction
   T wL = eL.h + eL.z;
   T wR = eR.h + eR.z;
-  const T emax = fmax(etaC, etaR), emin = fmin(etaC, etaR);
+  const T emax = P::max2(etaC, etaR), emin = P::min2(etaC, etaR);
   wL = P::clamp(wL, emin, emax);
   wR = P::clamp(wR, emin, emax);
   if (ORDER == 1) {
@@ -235,8 +266,8 @@ __host__ __device__ inline void hllc_edge(FrameCell<T>& eL, FrameCell<T>& eR, T 
   const T base = (aL + aR) / T(2) + (uL0 - uR0) / T(4);
   const T u_star = (uL0 + uR0) / T(2) + aL - aR;
   const T a_star = P::star_celerity(base, g);
-  const T sL = hL <= hs ? uR0 - T(2) * aR : fmin(uL0 - aL, u_star - a_star);
-  const T sR = hR <= hs ? uL0 + T(2) * aL : fmax(uR0 + aR, u_star + a_star);
+  const T sL = le_hs(hL) ? uR0 - T(2) * aR : fmin(uL0 - aL, u_star - a_star);
+  const T sR = le_hs(hR) ? uL0 + T(2) * aL : fmax(uR0 + aR, u_star + a_star);
   const T sM = P::div(sL * hR * (uR0 - sR) - sR * hL * (uL0 - sL), hR * (uR0 - sR) - hL * (uL0 - sL));
   const T qnfL = uL0 * qnL + T(0.5) * g * hL * hL;
   const T qnfR = uR0 * qnR + T(0.5) * g * hR * hR;
@@ -246,7 +277,7 @@ __host__ __device__ inline void hllc_edge(FrameCell<T>& eL, FrameCell<T>& eR, T 
   const T ft_s = fh_s * (star_l ? vL0 : vR0);
   const bool left = T(0) <= sL;
   const bool star = star_l || (sM < T(0) && T(0) <= sR);
-  const bool dry = hL < hs && hR < hs;
+  const bool dry = lt_hs(hL) && lt_hs(hR);
   fh = dry ? T(0) : (left ? qnL : (star ? fh_s : qnR));
   fn = dry ? T(0) : (left ? qnfL : (star ? fn_s : qnfR));
   ft = dry ? T(0) : (left ? uL0 * qtL : (star ? ft_s : uR0 * qtR));
@@ -267,10 +298,10 @@ template <class T, int ORDER>
 __host__ __device__ inline EdgeOut<T> solve_edge(const FrameCell<T>& c, const FrameCell<T>& l, const FrameCell<T>& r,
                                         const FrameCell<T>& rr, bool r_is_ghost, T etaC, T etaR, T dx, T dt,
                                         T dt_dx) {
-  const T hs = T(kHs), g = T(kG);
+  const T g = T(kG);
   EdgeOut<T> o;
   o.fh = T(0); o.fn = T(0); o.ft = T(0); o.bed_c = T(0); o.bed_r = T(0);
-  o.active = !(c.h < hs && r.h < hs);
+  o.active = !(lt_hs(c.h) && lt_hs(r.h));
   if (!o.active) return o;
 
   FrameCell<T> eL, eR;
@@ -281,8 +312,8 @@ __host__ __device__ inline EdgeOut<T> solve_edge(const FrameCell<T>& c, const Fr
       const T dz = r.z - c.z;
       const T zl = eL.z + T(0.5) * Prec<T>::limit(c.z - l.z, dz);
       const T zr = eR.z + T(0.5) * Prec<T>::limit(r.z - rr.z, -dz);
-      eL.z = r.h > hs ? zl : eL.z;
-      eR.z = c.h > hs ? zr : eR.z;
+      eL.z = gt_hs(r.h) ? zl : eL.z;
+      eR.z = gt_hs(c.h) ? zr : eR.z;
     }
   } else {
     if (r_is_ghost) {
@@ -300,10 +331,10 @@ __host__ __device__ inline EdgeOut<T> solve_edge(const FrameCell<T>& c, const Fr
       const T qtL = c.qt + T(0.5) * Prec<T>::limit(c.qt - l.qt, dqt);
       const T qnR = r.qn + T(0.5) * Prec<T>::limit(r.qn - rr.qn, -dqn);
       const T qtR = r.qt + T(0.5) * Prec<T>::limit(r.qt - rr.qt, -dqt);
-      eL.qn = c.h < hs ? T(0) : qnL;
-      eL.qt = c.h < hs ? T(0) : qtL;
-      eR.qn = r.h < hs ? T(0) : qnR;
-      eR.qt = r.h < hs ? T(0) : qtR;
+      eL.qn = lt_hs(c.h) ? T(0) : qnL;
+      eL.qt = lt_hs(c.h) ? T(0) : qtL;
+      eR.qn = lt_hs(r.h) ? T(0) : qnR;
+      eR.qt = lt_hs(r.h) ? T(0) : qtR;
     }
   }
   T fh, fn, ft;
@@ -395,7 +426,7 @@ __host__ __device__ inline T green_ampt(T h, T F0, T ks, T phi, T dtheta, T dt) 
 // correctly rounded either).
 template <class T>
 __device__ inline T cfl_candidate(T h, T qx, T qy, T cfl_dx) {
-  if (!(h > T(kHs))) return T(60.0);
+  if (!gt_hs(h)) return T(60.0);
   const float gh = (float)(T(kG) * h);
   const float a = sizeof(T) == 4 ? __builtin_amdgcn_sqrtf(gh) : __builtin_sqrtf(gh);
   return Prec<T>::div(cfl_dx, T(a) + Prec<T>::div(fmax(fabs(qx), fabs(qy)), h));

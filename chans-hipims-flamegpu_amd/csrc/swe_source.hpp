@@ -44,11 +44,13 @@ struct RainRows {
   bool zero;         // every station rate is 0 (after the table, or one row)
 };
 
-__device__ inline RainRows rain_rows(const hp_grid_desc& a, double t_now) {
+// The descriptor through any pointer type (generic, or the kernel-argument
+// address space).
+template <class DP> __device__ inline RainRows rain_rows_at(DP a, double t_now) {
   RainRows r{nullptr, nullptr, 0.0, 0, true};
-  if (a.rain_mode != HP_RAIN_STATION || a.rain_table == nullptr || a.rain_rows <= 0) return r;
-  const int rows = a.rain_rows, cols = a.rain_cols;
-  const double* tab = a.rain_table;
+  if (a->rain_mode != HP_RAIN_STATION || a->rain_table == nullptr || a->rain_rows <= 0) return r;
+  const int rows = a->rain_rows, cols = a->rain_cols;
+  const double* tab = a->rain_table;
   r.n_st = cols - 1;
   const bool after = !(t_now < tab[(rows - 1) * cols]);
   int i = 0;
@@ -61,6 +63,8 @@ __device__ inline RainRows rain_rows(const hp_grid_desc& a, double t_now) {
   }
   return r;
 }
+
+__device__ inline RainRows rain_rows(const hp_grid_desc& a, double t_now) { return rain_rows_at(&a, t_now); }
 
 // Rain rate of a cell whose station index is k.
 template <class T> __device__ inline T station_rate(const RainRows& rr, int k, double scale) {

@@ -210,7 +210,7 @@ enum hp_update_mode {
 int hp_grid_update_mode(const hp_grid_desc* g, int mode, void* stream);
 
 /* nsteps single-pass steps: flux, rain, friction, update, h_max and the CFL
- * minimum of each tile in one kernel reading (h, qx, qy) and writing
+ * minimum of each unit of cells in one kernel reading (h, qx, qy) and writing
  * (h_out, qx_out, qy_out), then the time-control block; the two buffers swap
  * every step and the result is left in (h, qx, qy).  Same results as
  * hp_grid_flux + hp_grid_update (HP_UPDATE_STEP); dh/dqx/dqy are not written. */
@@ -222,8 +222,8 @@ int hp_grid_step_fused(const hp_grid_desc* g, int32_t nsteps, void* stream);
  * kernels, with the per-tile partial records in a library-owned device
  * array of hp_grid_noalloc_max_bytes() bytes (8 MiB, held per GPU until the
  * process ends) instead of g->work, which is ignored.  A domain needing more
- * (over about 200 000 flux tiles, e.g. beyond 8192 x 8192 cells at the
- * default tile height) is rejected with HP_ESPACE.  Calls that overlap on one
+ * (over about 200 000 flux units; 8192 x 8192 float cells fit at the
+ * default unit height) is rejected with HP_ESPACE.  Calls that overlap on one
  * GPU share the array and race: serialise them. */
 int hp_grid_step_fused_noalloc(const hp_grid_desc* g, int32_t nsteps, void* stream);
 int64_t hp_grid_noalloc_max_bytes(void);

@@ -2,7 +2,7 @@
 (tests/golden/*.npz): the same inputs run free (not in lockstep) for the
 fixture's step count through the two-pass and the single-pass step, with
 the same time schedule emerging on the device; depths, discharges, h_max,
-and every step's t and dt agree with the oracle's committed outputs to 1e-9
+and every step's t and dt agree with the oracle's committed outputs to 1e-10
 relative."""
 import glob
 import os
@@ -17,7 +17,7 @@ from hipims_amd.grid import GridSolver
 pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
 FIXTURES = sorted(glob.glob(os.path.join(HERE, "golden", "*.npz")))
-TOL = 1e-9
+TOL = 1e-10
 
 
 def load(path):

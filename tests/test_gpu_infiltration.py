@@ -136,7 +136,7 @@ def test_godunov_sequence_and_mass_balance(oracle_lib, cuda, tmp_path):
     h = num.get_h().cpu().numpy()
     F = num._cumulativeWaterDepth.cpu().numpy()
     e = max(rel_err(h, o.h_c), rel_err(F, o.F))
-    assert e <= 1e-9, e
+    assert e <= 1e-10, e
     ctrl = num.solver.ctrl_dict()
     area = c.dx * c.dx
     balance = ctrl["volume"] - (compact(c.mask, c.h).sum() * area + ctrl["rain_volume"] + ctrl["flux_volume"]
