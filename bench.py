@@ -42,6 +42,15 @@ def alg_bytes(esize: int, rain_field: bool) -> dict:
     return {"k_flux": flux, "k_update": update, "k_step_fused": fused}
 
 
+def kernel_names(esize: int) -> dict:
+    """The kernel behind each pass: float domains run both the flux pass and
+    the single-pass step as column-walk kernels (k_walk, its flux-pass and
+    step instances), double domains the LDS-tile kernels."""
+    if esize == 4:
+        return {"k_flux": "k_walk_flux", "k_update": "k_update", "k_step_fused": "k_walk_step"}
+    return {"k_flux": "k_flux", "k_update": "k_update", "k_step_fused": "k_step_fused"}
+
+
 def profiled_traffic(kernel: str, workload: str):
     """HBM bytes per launch of `kernel` on `workload` as recorded from the
     rocprofv3 FETCH_SIZE / WRITE_SIZE runs (copy-calibrated) in profiles/, or
@@ -118,36 +127,48 @@ def workload_tag(a) -> str:
     return f"roofline_{a.size}_{a.dtype}_order{a.order}"
 
 
+# every tenth timed step carries HIP events around its kernels
+KERNEL_SAMPLE_EVERY = 10
+
+
 def time_two_pass(g, steps):
-    """Two-pass step (flux kernel; update + finalize), HIP events between the
-    kernels on the launch stream."""
+    """Two-pass step (flux kernel; update + finalize): HIP events on the
+    launch stream around all the steps, and between the kernels of every
+    tenth step (the per-kernel means)."""
     import torch
     stream = torch.cuda.current_stream(g.device)
-    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
+    sampled = list(range(0, steps, KERNEL_SAMPLE_EVERY))
+    ev = {k: [torch.cuda.Event(enable_timing=True) for _ in range(3)] for k in sampled}
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     wall = time.perf_counter()
     start.record(stream)
     for k in range(steps):
-        ev[k][0].record(stream)
+        e = ev.get(k)
+        if e:
+            e[0].record(stream)
         g.flux()
-        ev[k][1].record(stream)
+        if e:
+            e[1].record(stream)
         g.update()
-        ev[k][2].record(stream)
+        if e:
+            e[2].record(stream)
     stop.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - wall
+    n = len(sampled)
     return {"path": "two_pass", "steps": steps, "total_ms": start.elapsed_time(stop), "wall_s": wall,
-            "kernels_ms": {"k_flux": sum(e[0].elapsed_time(e[1]) for e in ev) / steps,
-                           "k_update": sum(e[1].elapsed_time(e[2]) for e in ev) / steps}}
+            "kernels_ms": {"k_flux": sum(e[0].elapsed_time(e[1]) for e in ev.values()) / n,
+                           "k_update": sum(e[1].elapsed_time(e[2]) for e in ev.values()) / n}}
 
 
 def time_fused(g, steps):
     """Single-pass step (fused kernel + time control): `steps` steps in one
     call of hp_grid_step_fused_timed, which records HIP events on the launch
-    stream before each single-pass kernel, between it and its time control
-    and after the last step, so the kernel's own duration and the steps'
-    total come from the same launches."""
+    stream around all the steps and around the single-pass kernel of every
+    tenth step (an event between two kernels delays the next launch by a few
+    microseconds, so only those steps carry it), so the kernel's own duration
+    and the steps' total come from the same launches."""
     import ctypes
 
     import torch
@@ -158,11 +179,11 @@ def time_fused(g, steps):
     torch.cuda.synchronize()
     k_ms, tot_ms = ctypes.c_float(), ctypes.c_float()
     wall = time.perf_counter()
-    _lib.check(_lib.lib().hp_grid_step_fused_timed(ctypes.byref(g.desc), int(steps), ctypes.byref(k_ms),
-                                                    ctypes.byref(tot_ms), _lib.stream_ptr(g.device)),
-               "hp_grid_step_fused_timed")
+    _lib.check(_lib.lib().hp_grid_step_fused_timed(ctypes.byref(g.desc), int(steps), KERNEL_SAMPLE_EVERY,
+                                                    ctypes.byref(k_ms), ctypes.byref(tot_ms),
+                                                    _lib.stream_ptr(g.device)), "hp_grid_step_fused_timed")
     wall = time.perf_counter() - wall
-    kern = k_ms.value / steps
+    kern = k_ms.value
     return {"path": "fused", "steps": steps, "total_ms": tot_ms.value, "wall_s": wall,
             "kernels_ms": {"k_step_fused": kern, "k_finalize+gaps": tot_ms.value / steps - kern}}
 
@@ -474,7 +495,7 @@ def adjoint_mode(a):
     del cal_all
     torch.cuda.empty_cache()
     line = {"mode": "adjoint", "size": a.adj_size, "order": a.adj_order, "slopes": a.adj_slopes, "tile_rows": rows,
-            "plain_path": "single-pass step (k_step_fused + k_finalize)",
+            "plain_path": "single-pass step (column-walk step kernel + k_finalize)",
             "steps": a.adj_steps, "checkpoint": a.adj_checkpoint,
             "recompute": "serial" if a.adj_serial else "overlapped on a second stream",
             "gauges": int(len(gauges)), "dtype": "f32 state, f64 gradient sums", "loss": float(loss),
@@ -583,7 +604,10 @@ def main():
     assert per_kernel[dom] <= ms_step, "the dominant kernel's time exceeds the step it belongs to"
     achieved = B[dom] * cells / (per_kernel[dom] / 1e3) / 1e9
     two_pass = (B["k_flux"] + B["k_update"]) * value / 1e9
-    valu = profiled_valu(dom, workload_tag(a))
+    names = kernel_names(esize)
+    kname = names[dom]
+    per_kernel = {names.get(k, k): v for k, v in per_kernel.items()}
+    valu = profiled_valu(kname, workload_tag(a))
     vpeak, vsrc = valu_peak()
     valu_frac = None if valu is None else valu * cells / (per_kernel[dom] / 1e3) / vpeak
     t_wall = best["wall_s"]
@@ -617,11 +641,11 @@ def main():
         # names the resource the kernel is closer to saturating: the HBM
         # fraction above or the VALU issue fraction from the counted
         # instructions per cell
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": profiled_traffic(dom, workload_tag(a)),
-                     "alg_bytes_per_cell": B[dom], "kernel_ms": per_kernel[dom],
-                     "timed_with": "HIP events around each launch of the timed steps",
+                     "traffic": profiled_traffic(kname, workload_tag(a)),
+                     "alg_bytes_per_cell": B[dom], "kernel_ms": per_kernel[kname],
+                     "timed_with": "HIP events around the kernel of every tenth timed step",
                      "valu_lane_instr_per_cell": valu,
                      "valu_issue_frac": valu_frac,
                      "limiter": "valu" if valu_frac is not None and valu_frac > achieved / HBM_PEAK_GBS else "hbm",

@@ -1359,16 +1359,17 @@ int hp_grid_step_fused_noalloc(const hp_grid_desc* g, int32_t nsteps, void* stre
   return hp_grid_step_fused(&d, nsteps, stream);
 }
 
-int hp_grid_step_fused_timed(const hp_grid_desc* g, int32_t nsteps, float* kernel_ms, float* step_ms,
-                             void* stream) {
+int hp_grid_step_fused_timed(const hp_grid_desc* g, int32_t nsteps, int32_t sample_every, float* kernel_ms,
+                             float* step_ms, void* stream) {
   int rc = check_fused(g);
   if (rc) return rc;
-  if (nsteps <= 0) return fail(HP_EINVAL, "nsteps must be positive");
+  if (nsteps <= 0 || sample_every <= 0) return fail(HP_EINVAL, "nsteps and sample_every must be positive");
   if (!kernel_ms || !step_ms) return fail(HP_EINVAL, "kernel_ms and step_ms are required");
   hipStream_t st = (hipStream_t)stream;
-  // ev[2k]: before step k's single-pass kernel, ev[2k+1]: between it and its
-  // time control, ev[2 nsteps]: after the last time control
-  std::vector<hipEvent_t> ev(2 * (size_t)nsteps + 1, nullptr);
+  // ev[0] / ev[1]: around all nsteps steps; ev[2 + 2j] / ev[3 + 2j]: around
+  // the single-pass kernel of the j-th sampled step
+  const int32_t samples = (nsteps + sample_every - 1) / sample_every;
+  std::vector<hipEvent_t> ev(2 + 2 * (size_t)samples, nullptr);
   auto release = [&]() {
     for (hipEvent_t e : ev)
       if (e) (void)hipEventDestroy(e);
@@ -1380,11 +1381,14 @@ int hp_grid_step_fused_timed(const hp_grid_desc* g, int32_t nsteps, float* kerne
     }
   }
   hp_grid_desc d = *g;
+  (void)hipEventRecord(ev[0], st);
   for (int32_t k = 0; k < nsteps && rc == HP_OK; ++k) {
-    (void)hipEventRecord(ev[2 * k], st);
+    const bool sample = k % sample_every == 0;
+    const size_t j = 2 + 2 * (size_t)(k / sample_every);
+    if (sample) (void)hipEventRecord(ev[j], st);
     rc = launch_step_fused(&d, st, k == 0 ? 4 : 0);
     if (rc == HP_OK) {
-      (void)hipEventRecord(ev[2 * k + 1], st);
+      if (sample) (void)hipEventRecord(ev[j + 1], st);
       rc = launch_finalize_fused(&d, st);
     }
     std::swap(d.h, d.h_out);
@@ -1392,19 +1396,18 @@ int hp_grid_step_fused_timed(const hp_grid_desc* g, int32_t nsteps, float* kerne
     std::swap(d.qy, d.qy_out);
   }
   if (rc == HP_OK) {
-    (void)hipEventRecord(ev[2 * (size_t)nsteps], st);
+    (void)hipEventRecord(ev[1], st);
     if (nsteps & 1) rc = copy_state_back(g, st);  // after the timed steps
-    if (rc == HP_OK && hipEventSynchronize(ev[2 * (size_t)nsteps]) != hipSuccess)
-      rc = fail(HP_ELAUNCH, "hipEventSynchronize failed");
+    if (rc == HP_OK && hipEventSynchronize(ev[1]) != hipSuccess) rc = fail(HP_ELAUNCH, "hipEventSynchronize failed");
   }
   if (rc == HP_OK) {
     float sum = 0.0f, ms = 0.0f;
-    for (int32_t k = 0; k < nsteps; ++k) {
-      (void)hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]);
+    for (int32_t j = 0; j < samples; ++j) {
+      (void)hipEventElapsedTime(&ms, ev[2 + 2 * (size_t)j], ev[3 + 2 * (size_t)j]);
       sum += ms;
     }
-    (void)hipEventElapsedTime(&ms, ev[0], ev[2 * (size_t)nsteps]);
-    *kernel_ms = sum;
+    (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
+    *kernel_ms = sum / (float)samples;
     *step_ms = ms;
   }
   release();

@@ -102,7 +102,7 @@ SIGNATURES = {
     "hp_grid_step_fused": (_I, [ctypes.POINTER(HpGridDesc), _I32, _P]),
     "hp_grid_step_fused_noalloc": (_I, [ctypes.POINTER(HpGridDesc), _I32, _P]),
     "hp_grid_noalloc_max_bytes": (_I64, []),
-    "hp_grid_step_fused_timed": (_I, [ctypes.POINTER(HpGridDesc), _I32, ctypes.POINTER(ctypes.c_float),
+    "hp_grid_step_fused_timed": (_I, [ctypes.POINTER(HpGridDesc), _I32, _I32, ctypes.POINTER(ctypes.c_float),
                                       ctypes.POINTER(ctypes.c_float), _P]),
     "hp_ensemble_workspace_bytes": (_I64, [_I32, ctypes.POINTER(HpGridDesc)]),
     "hp_ensemble_init": (_I, [_I32, ctypes.POINTER(HpGridDesc), _I32, _P, _I64, ctypes.POINTER(HpEnsemble), _P]),

@@ -230,14 +230,16 @@ int64_t hp_grid_noalloc_max_bytes(void);
 
 /* Measurement form of hp_grid_step_fused (no reference counterpart; the
  * reference times nothing on this path): nsteps single-pass steps exactly
- * as hp_grid_step_fused runs them, with HIP events recorded on
- * `stream` before each single-pass kernel, between it and its time control,
- * and after the last time control.  Synchronises on the last event, then
- * writes the summed single-pass kernel durations to *kernel_ms and the
- * duration of all nsteps steps to *step_ms, so both come from the same
- * launches (the state copy after an odd count falls outside both). */
-int hp_grid_step_fused_timed(const hp_grid_desc* g, int32_t nsteps, float* kernel_ms, float* step_ms,
-                             void* stream);
+ * as hp_grid_step_fused runs them, with HIP events recorded on `stream`
+ * before the first step and after the last, and around the single-pass
+ * kernel of every sample_every-th step (an event between two kernels delays
+ * the second by a few microseconds, so only the sampled steps carry that
+ * cost).  Synchronises on the last event, then writes the mean duration of
+ * the sampled single-pass kernels to *kernel_ms and the duration of all
+ * nsteps steps to *step_ms, so both come from the same launches (the state
+ * copy after an odd count falls outside both). */
+int hp_grid_step_fused_timed(const hp_grid_desc* g, int32_t nsteps, int32_t sample_every, float* kernel_ms,
+                             float* step_ms, void* stream);
 
 /* Infiltration pass: cells listed by the flux rule of the current state
  * (the wet list of the flux pass) take the Green-Ampt increment; infil_dF
