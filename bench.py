@@ -220,13 +220,17 @@ def parse():
     p.add_argument("--ens-two-pass", action="store_true", help="ensemble: flux and update passes per step")
     p.add_argument("--no-baselines", action="store_true",
                    help="ensemble: skip the sequential baseline runs (for long simulated times)")
+    p.add_argument("--ensemble-seconds", type=float, default=7200.0,
+                   help="step mode: simulated time of the ensemble run reported beside the step (0: none)")
+    p.add_argument("--no-adjoint", action="store_true", help="step mode: no calibration gradient run beside the step")
     p.add_argument("--adj-size", type=int, default=1024)
     p.add_argument("--adj-steps", type=int, default=2000)
     p.add_argument("--adj-checkpoint", type=int, default=100)
     p.add_argument("--adj-gauges", type=int, default=32)
     p.add_argument("--adj-order", type=int, choices=[1, 2], default=1,
-                   help="adjoint: scheme order (the server's default, order 1, has a stable adjoint over long "
-                        "horizons; the frozen-branch linearisation of MUSCL with forward Euler grows)")
+                   help="adjoint: scheme order (the server's default, order 1, has a bounded adjoint over long "
+                        "horizons; order-2 runs amplify rounding differences, and beyond a few hundred steps "
+                        "their derivative, with exact or frozen slopes, is not defined: costs only)")
     p.add_argument("--adj-slopes", choices=["exact", "frozen"], default="exact")
     p.add_argument("--no-fd", action="store_true", help="adjoint: skip the finite-difference check")
     p.add_argument("--adj-serial", action="store_true",
@@ -288,46 +292,63 @@ def _ensemble_case(spec):
     return gen.ensemble_case(*spec)
 
 
-def ensemble_mode(a):
-    """Config: 64 scenarios, sizes drawn from {512, 768, 1024, 1536, 2048}^2,
-    own DEM / landcover per scenario (seed 2000 + k), triangular hyetograph,
-    each run to the same simulated time with its own CFL dt; FP32, order 2.
-    Ensemble: every scenario advanced by one launch set per step, K steps per
-    graph replay.  Baselines: the same scenarios one after another through
-    the per-step calls (addFlux, addStation_PrecipitationSource,
-    time_friction_euler_update_cuda, a host read of t each step) of (a) the
-    structured Godunov drop-in and (b) CompactGodunov, the reference's own
-    compact-layout sequence (wet lists by nonzero, host dt logic) over the
-    HIP compact ops."""
+def ensemble_cases(n_scen):
+    """The ensemble's scenarios, generated on host worker processes; call
+    before this process touches the GPU (the pool forks)."""
     import concurrent.futures as cf
 
-    import numpy as np
-    import torch
-
     from hipims_amd import generators as gen
-    from hipims_amd.ensemble import Ensemble
-    from hipims_amd.grid import GridSolver
-    from hipims_amd.compact_driver import CompactGodunov
-    from hipims_amd.SWE_CUDA import Godunov
-
-    # generate on host worker processes before this process touches the GPU
-    specs = gen.ensemble_specs(a.scenarios, 2000)
+    specs = gen.ensemble_specs(n_scen, 2000)
     t0 = time.perf_counter()
     workers = max(1, min(16, len(os.sched_getaffinity(0))))
     with cf.ProcessPoolExecutor(workers) as ex:
         cases = list(ex.map(_ensemble_case, specs))
     gen_s = time.perf_counter() - t0
-    dev = torch.device("cuda", 0)
-    torch.cuda.set_device(dev)
     print(f"[ensemble] generated {len(cases)} scenarios in {gen_s:.1f} s", file=sys.stderr, flush=True)
+    return cases, gen_s
 
-    def solver(c):
-        s = GridSolver(c.mask, c.h, c.qx, c.qy, c.z, c.dx, cfl=c.cfl, order=2, dtype=torch.float32, device=dev,
-                       first_dt=c.first_dt, manning=c.manning, landuse=c.landuse)
-        s.set_station_rain(c.rain_table)
-        return s
 
-    ens = Ensemble([solver(c) for c in cases], t_stop=a.sim_seconds, single_pass=not a.ens_two_pass)
+def _ensemble_solver(c, dev):
+    import torch
+
+    from hipims_amd.grid import GridSolver
+    s = GridSolver(c.mask, c.h, c.qx, c.qy, c.z, c.dx, cfl=c.cfl, order=2, dtype=torch.float32, device=dev,
+                   first_dt=c.first_dt, manning=c.manning, landuse=c.landuse)
+    s.set_station_rain(c.rain_table)
+    return s
+
+
+def run_ensemble(a, cases, dev, sim_seconds, baselines):
+    """Config: 64 scenarios, sizes drawn from {512, 768, 1024, 1536, 2048}^2,
+    own DEM / landcover per scenario (seed 2000 + k), triangular hyetograph,
+    each run to the same simulated time with its own CFL dt; FP32, order 2.
+    Ensemble: every scenario advanced by one launch set per step, K steps per
+    graph replay, one host read of the scenarios' times per replay.
+    Baselines, the same scenarios one after another:
+      standalone_graphs  each scenario alone through the single-pass step,
+                         K steps per graph replay, one host read of t per
+                         replay (the same kernels and host pattern as the
+                         ensemble, one scenario per launch);
+      godunov            the structured Godunov drop-in's per-step calls
+                         (addFlux, addStation_PrecipitationSource,
+                         time_friction_euler_update_cuda, a host read of t
+                         each step);
+      compact            CompactGodunov, the reference's own compact-layout
+                         sequence (wet lists by nonzero, host dt logic) over
+                         the HIP compact ops."""
+    import torch
+
+    from hipims_amd.ensemble import Ensemble
+    from hipims_amd.compact_driver import CompactGodunov
+    from hipims_amd.SWE_CUDA import Godunov
+
+    # first launches of the step kernels outside the timed runs
+    warm = _ensemble_solver(cases[0], dev)
+    warm.step_fused(2)
+    Ensemble([warm], single_pass=not a.ens_two_pass).step(2)
+    del warm
+
+    ens = Ensemble([_ensemble_solver(c, dev) for c in cases], t_stop=sim_seconds, single_pass=not a.ens_two_pass)
     ens.capture(a.replay_steps)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -352,16 +373,57 @@ def ensemble_mode(a):
     torch.cuda.empty_cache()
 
     seq = {}
+    if "standalone_graphs" in baselines:
+        seq["standalone_graphs"] = standalone_graphs(cases, sim_seconds, a.replay_steps, dev)
     for name, cls in (("godunov", Godunov), ("compact", CompactGodunov)):
-        if not a.no_baselines:
-            seq[name] = sequential(cls, cases, a.sim_seconds, dev, name)
-    line = {"mode": "ensemble", "scenarios": len(cases), "sizes": [c.mask.shape[0] - 2 for c in cases],
-            "sim_seconds": a.sim_seconds, "dtype": "f32", "order": 2, "steps_per_replay": a.replay_steps,
+        if name in baselines:
+            seq[name] = sequential(cls, cases, sim_seconds, dev, name)
+    return {"scenarios": len(cases), "sizes": [c.mask.shape[0] - 2 for c in cases],
+            "sim_seconds": sim_seconds, "dtype": "f32", "order": 2, "steps_per_replay": a.replay_steps,
             "step": "two-pass" if a.ens_two_pass else "single-pass",
             "cell_updates": work, "ensemble_s": ens_s, "ensemble_cell_updates_per_s": work / ens_s,
-            "steps_min": min(steps), "steps_max": max(steps), "generation_s": gen_s,
+            "steps_min": min(steps), "steps_max": max(steps),
             "baselines": {name: {"seconds": v[0], "cell_updates_per_s": work / v[0], "speedup": v[0] / ens_s,
                                  "same_step_counts": v[1] == steps} for name, v in seq.items()}}
+
+
+def standalone_graphs(cases, t_stop, steps, dev):
+    """Each scenario alone: its single-pass steps captured `steps` to a HIP
+    graph (captured before the clock starts), the graph replayed until the
+    scenario reaches t_stop, one host read of t per replay."""
+    import torch
+
+    from hipims_amd import _lib
+    total = 0.0
+    counts = []
+    for k, c in enumerate(cases):
+        s = _ensemble_solver(c, dev)
+        s.set_t_stop(t_stop)
+        s.ensure_out()
+        s.desc  # workspace allocated outside the capture
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            s.step_fused(steps)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        while float(s.ctrl[_lib.CTRL_T]) < t_stop:
+            g.replay()
+        torch.cuda.synchronize()
+        total += time.perf_counter() - t0
+        counts.append(int(s.ctrl_dict()["steps"]))
+        del g, s
+    print(f"[ensemble] standalone graphs: {total:.2f} s", file=sys.stderr, flush=True)
+    return total, counts
+
+
+def ensemble_mode(a):
+    import torch
+    cases, gen_s = ensemble_cases(a.scenarios)
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    bl = ("standalone_graphs",) if a.no_baselines else ("standalone_graphs", "godunov", "compact")
+    line = {"mode": "ensemble", **run_ensemble(a, cases, dev, a.sim_seconds, bl), "generation_s": gen_s}
     print(json.dumps(line), flush=True)
 
 
@@ -400,6 +462,10 @@ def sequential(cls, cases, t_stop, dev, name):
 
 
 def adjoint_mode(a):
+    print(json.dumps({"mode": "adjoint", **run_adjoint(a, fp64=not a.no_fd)}), flush=True)
+
+
+def run_adjoint(a, fp64=True):
     """Config: 1024^2, dx = 5 m, FP32 forward and adjoint, FP64 gradient
     sums, 2000 steps, checkpoint every 100; 32 gauges among cells wet at the
     end of a pilot run; observations from a run with every class n scaled by
@@ -494,7 +560,7 @@ def adjoint_mode(a):
                  "grad_equal": bool(torch.equal(grad_all, grad))}
     del cal_all
     torch.cuda.empty_cache()
-    line = {"mode": "adjoint", "size": a.adj_size, "order": a.adj_order, "slopes": a.adj_slopes, "tile_rows": rows,
+    line = {"size": a.adj_size, "order": a.adj_order, "slopes": a.adj_slopes, "tile_rows": rows,
             "plain_path": "single-pass step (column-walk step kernel + k_finalize)",
             "steps": a.adj_steps, "checkpoint": a.adj_checkpoint,
             "recompute": "serial" if a.adj_serial else "overlapped on a second stream",
@@ -513,6 +579,8 @@ def adjoint_mode(a):
         g = grad.numpy()
         line["fd"] = fd.tolist()
         line["fd_max_rel_err"] = float(np.abs(g - fd).max() / np.abs(g).max())
+    if fp64:
+        g = grad.numpy()
         # FP64 reference: the same gauges, observations and loss on the FP64
         # solver (its own recorded schedule), its adjoint gradient against its
         # own central differences, and the FP32 gradient against it
@@ -538,7 +606,7 @@ def adjoint_mode(a):
             line["fp64"][f"fd_eps{eps:g}"] = {
                 "fd": fd64.tolist(), "max_rel_err": float(np.abs(g64 - fd64).max() / np.abs(g64).max()),
                 "rel_err_per_param": (np.abs(g64 - fd64) / np.abs(fd64)).tolist()}
-    print(json.dumps(line), flush=True)
+    return line
 
 
 def main():
@@ -555,6 +623,7 @@ def main():
         return ensemble_mode(a)
     if a.mode == "adjoint":
         return adjoint_mode(a)
+    ens_cases = ensemble_cases(a.scenarios)[0] if a.ensemble_seconds > 0 else None
     import numpy as np
     import torch
 
@@ -658,6 +727,11 @@ def main():
         "wall_s": t_wall,
         "cpu_baseline": cpu,
     }
+    if ens_cases is not None:
+        line["ensemble"] = run_ensemble(a, ens_cases, dev, a.ensemble_seconds, ("standalone_graphs",))
+        del line["ensemble"]["sizes"]
+    if not a.no_adjoint:
+        line["adjoint"] = run_adjoint(a, fp64=False)
     print(json.dumps(line), flush=True)
 
 

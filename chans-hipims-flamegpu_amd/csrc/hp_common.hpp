@@ -42,6 +42,21 @@ struct StepRecord {
 // that every XCD walks one contiguous band of tiles: x- and y-neighbours then
 // share their halo lines in one L2 instead of fetching them from HBM twice.
 constexpr int kXcds = 8;
+
+// A domain descriptor read in place from constant memory (a kernel's own
+// arguments, or a descriptor array no launch writes): its fields are fetched
+// by scalar loads where they are used.  launder() hides the pointer's value
+// from the compiler (an empty asm that must be taken to change it), so the
+// fields are fetched again after it instead of being held in scalar
+// registers across long kernels, which spills them.  The pointer must be
+// uniform across the wave.
+typedef __attribute__((address_space(4))) const hp_grid_desc* KDesc;
+template <class DP> __device__ inline DP launder(DP p) {
+  asm volatile("" : "+s"(p));
+  return p;
+}
+// The descriptor of a kernel whose first argument is its hp_grid_desc.
+__device__ inline KDesc kernarg_desc() { return (KDesc)__builtin_amdgcn_kernarg_segment_ptr(); }
 __device__ inline int xcd_band(int b, int n) {
   const int q = n / kXcds, rem = n - q * kXcds;
   const int x = b % kXcds, local = b / kXcds;

@@ -83,19 +83,19 @@ template <class T> struct AdjPlanes {
   const double* loss_obs;
 };
 
-template <class T>
-__device__ inline bool flux_listed(const hp_grid_desc& a, const T* h, int x, int y, T t) {
-  const int P = (int)a.pitch;  // planes of fewer than 2^31 elements
+template <class T, class DP>
+__device__ inline bool flux_listed(DP a, const T* h, int x, int y, T t) {
+  const int P = (int)a->pitch;  // planes of fewer than 2^31 elements
   const int i = y * P + x;
   if (t == T(0) || gt_hs(h[i])) return true;
   const int nx[4] = {x + 1, x, x - 1, x};
   const int ny[4] = {y, y - 1, y, y + 1};
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    if (nx[k] < 0 || nx[k] >= a.nx || ny[k] < 0 || ny[k] >= a.ny) continue;
+    if (nx[k] < 0 || nx[k] >= a->nx || ny[k] < 0 || ny[k] >= a->ny) continue;
     const int j = ny[k] * P + nx[k];
-    if (a.type_slot[j] == 0) continue;
-    if (nx[k] == a.cell0_x && ny[k] == a.cell0_y) continue;
+    if (a->type_slot[j] == 0) continue;
+    if (nx[k] == a->cell0_x && ny[k] == a->cell0_y) continue;
     if (gt_hs(h[j])) return true;
   }
   return false;
@@ -207,7 +207,7 @@ __global__ __launch_bounds__(kAThreads) void k_adj_update(hp_grid_desc a, AdjPla
         continue;
       // the kVec elements share a row (pitch is a multiple of kVec)
       const int y = (int)((unsigned)i0 / pitch32), x = (int)((unsigned)i0 - (unsigned)y * pitch32) + j;
-      if (flux_listed<T>(a, h, x, y, t_T)) {
+      if (flux_listed<T>(&a, h, x, y, t_T)) {
         ch.x[j] = lh.x[j];
         cqx.x[j] = lqx.x[j];
         cqy.x[j] = lqy.x[j];
@@ -266,12 +266,12 @@ __global__ __launch_bounds__(kAThreads) void k_adj_finish(hp_grid_desc a, const 
   }
 }
 
-template <class T>
-__device__ inline FrameCell<T> gframe(const hp_grid_desc& a, int i, int d, T zref) {
-  const T* h = (const T*)a.h;
-  const T* qx = (const T*)a.qx;
-  const T* qy = (const T*)a.qy;
-  const T* z = (const T*)a.z;
+template <class T, class DP>
+__device__ inline FrameCell<T> gframe(DP a, int i, int d, T zref) {
+  const T* h = (const T*)a->h;
+  const T* qx = (const T*)a->qx;
+  const T* qy = (const T*)a->qy;
+  const T* z = (const T*)a->z;
   FrameCell<T> f;
   f.h = h[i];
   f.qn = rot_n(qx[i], qy[i], d);
@@ -296,12 +296,12 @@ __device__ inline void to_global_adj(const FrameAdj<T>& f, int d, T& ax, T& ay) 
 // valid cell's flux increments; it is asked only for A and B.  out[role]
 // receives the cotangents (h, qx, qy) of the cells in roles l, A, B, rr (the
 // cell before A, A, B, the cell after B); order 1 fills only A and B.
-template <class T, int ORDER, int DIR, class Cot>
-__device__ inline void edge_adj(const hp_grid_desc& a, bool frozen_slopes, int col, int row, Cot cot,
+template <class T, int ORDER, int DIR, class Cot, class DP>
+__device__ inline void edge_adj(DP a, bool frozen_slopes, int col, int row, Cot cot,
                                 T (&out)[4][3]) {
   // The direction is a compile-time constant, so the frame rotations of the
   // interior edges fold away.
-  const int nx = a.nx, ny = a.ny;
+  const int nx = a->nx, ny = a->ny;
   constexpr bool xe = DIR == 0;
   int yA, xA, yB, xB, yl, xl, yr2, xr2;
   constexpr int dAB = DIR;
@@ -319,21 +319,21 @@ __device__ inline void edge_adj(const hp_grid_desc& a, bool frozen_slopes, int c
   // only in rows inside the plane and y-edges only in columns inside it, so
   // only the coordinate across the edge needs a bounds test; an offset is
   // read only for a cell that passes it.
-  const int P = (int)a.pitch;
+  const int P = (int)a->pitch;
   const int step = xe ? 1 : P;
   const int iA = yA * P + xA, iB = iA + step, il = iA - step, ir2 = iB + step;
   auto valid = [&](int x, int y, int i) -> bool {
-    return (xe ? (x >= 0 && x < nx) : (y >= 0 && y < ny)) && a.type_slot[i] != 0;
+    return (xe ? (x >= 0 && x < nx) : (y >= 0 && y < ny)) && a->type_slot[i] != 0;
   };
   const bool vA = valid(xA, yA, iA), vB = valid(xB, yB, iB);
   if (vA || vB) {
-    const T dt = T(a.ctrl->dt);
-    const T dx = T(a.dx);
-    const T t = T(a.ctrl->t);
+    const T dt = T(a->ctrl->dt);
+    const T dx = T(a->dx);
+    const T t = T(a->ctrl->t);
     const int dBA = (dAB + 2) & 3;
     const bool axis_x = (dAB == 0 || dAB == 2);
     if (vA && vB) {
-      const T zref = Prec<T>::local_datum ? ((const T*)a.z)[iA] : T(0);
+      const T zref = Prec<T>::local_datum ? ((const T*)a->z)[iA] : T(0);
       const FrameCell<T> c = gframe<T>(a, iA, dAB, zref);
       const FrameCell<T> r = gframe<T>(a, iB, dAB, zref);
       const bool vl = valid(xl, yl, il), vr = valid(xr2, yr2, ir2);
@@ -359,11 +359,11 @@ __device__ inline void edge_adj(const hp_grid_desc& a, bool frozen_slopes, int c
       const bool from_a = vA;
       const int xc = from_a ? xA : xB, yc = from_a ? yA : yB, d = from_a ? dAB : dBA;
       const int ic = from_a ? iA : iB;
-      const T zref = Prec<T>::local_datum ? ((const T*)a.z)[ic] : T(0);
+      const T zref = Prec<T>::local_datum ? ((const T*)a->z)[ic] : T(0);
       const FrameCell<T> c = gframe<T>(a, ic, d, zref);
       // boundary values at this time
       int type, id;
-      decode_code(a.slot_code[a.type_slot[ic]], type, id);
+      decode_code(a->slot_code[a->type_slot[ic]], type, id);
       auto srow = [&](const T* tab, int rows, int w) {
         int rr_ = 0;
         while (tab && rr_ < rows - 1 && t > tab[rr_ * w]) rr_++;
@@ -372,14 +372,14 @@ __device__ inline void edge_adj(const hp_grid_desc& a, bool frozen_slopes, int c
       auto sat = [&](const T* tab, int rows, int w, int row, int col) -> T {
         return (tab == nullptr || col < 0 || col >= w || row < 0 || row >= rows) ? T(0) : tab[row * w + col];
       };
-      const T* gd = (const T*)a.given_depth;
-      const T* gw = (const T*)a.given_wl;
-      const T* gq = (const T*)a.given_q;
-      const T bd = sat(gd, a.gd_rows, a.gd_w, srow(gd, a.gd_rows, a.gd_w), id + 1);
-      const T bw = sat(gw, a.gwl_rows, a.gwl_w, srow(gw, a.gwl_rows, a.gwl_w), id + 1) - zref;
-      const int rq = srow(gq, a.gq_rows, a.gq_w);
-      const T bqx = sat(gq, a.gq_rows, a.gq_w, rq, 1 + 2 * id);
-      const T bqy = sat(gq, a.gq_rows, a.gq_w, rq, 2 + 2 * id);
+      const T* gd = (const T*)a->given_depth;
+      const T* gw = (const T*)a->given_wl;
+      const T* gq = (const T*)a->given_q;
+      const T bd = sat(gd, a->gd_rows, a->gd_w, srow(gd, a->gd_rows, a->gd_w), id + 1);
+      const T bw = sat(gw, a->gwl_rows, a->gwl_w, srow(gw, a->gwl_rows, a->gwl_w), id + 1) - zref;
+      const int rq = srow(gq, a->gq_rows, a->gq_w);
+      const T bqx = sat(gq, a->gq_rows, a->gq_w, rq, 1 + 2 * id);
+      const T bqy = sat(gq, a->gq_rows, a->gq_w, rq, 2 + 2 * id);
       const FrameCell<T> g = bc_ghost<T>(type, c, d, bd, bw, bqx, bqy);
       const T etaG = ORDER == 1 ? g.h + g.z : ghost_wl<T>(type, c.h + c.z, g, bw);
       T cch, ccx, ccy;
@@ -417,7 +417,7 @@ __global__ __launch_bounds__(kAThreads) void k_adj_edges(hp_grid_desc a, AdjPlan
   if (col >= (xe ? a.nx + 1 : a.nx) || row >= (xe ? a.ny : a.ny + 1)) return;
   const int64_t P = a.pitch;
   T out[4][3];
-  edge_adj<T, ORDER, DIR>(a, p.frozen_slopes != 0, col, row,
+  edge_adj<T, ORDER, DIR>(&a, p.frozen_slopes != 0, col, row,
                           [&](int x, int y, T& ch, T& cx, T& cy) {
                             const int64_t i = (int64_t)y * P + x;
                             ch = p.ch[i]; cx = p.cqx[i]; cy = p.cqy[i];
@@ -500,29 +500,32 @@ __global__ __launch_bounds__(NT, sizeof(T) == 4 ? 6 : 1) void k_adj_step_tile(hp
   __shared__ T s_man[256];
   __shared__ T s_rb[kARateCache], s_rs[kARateCache];  // station rates: unscaled, scaled
   __shared__ double s_r[NT / 64];
+  // the descriptor is read in place from the kernel arguments and laundered
+  // between the phases, so its fields are fetched where each phase needs them
+  KDesc pa = kernarg_desc();
   const int tid = threadIdx.x;
   const int tile = xcd_band(blockIdx.x, gridDim.x);
   const int x0 = (tile % ntx) * kATX, y0 = (tile / ntx) * TY;
-  const double t_now = a.ctrl->t;
-  const T dt = T(a.ctrl->dt);
+  const double t_now = pa->ctrl->t;
+  const T dt = T(pa->ctrl->dt);
   const T t_T = T(t_now);
-  const int mode = a.rain_mode;
-  for (int k = tid; k < a.n_classes; k += NT) s_man[k] = ((const T*)a.manning)[k];
-  const RainRows rr = rain_rows(a, t_now);
+  const int mode = pa->rain_mode;
+  for (int k = tid; k < pa->n_classes; k += NT) s_man[k] = ((const T*)pa->manning)[k];
+  const RainRows rr = rain_rows_at(pa, t_now);
   if (mode == HP_RAIN_STATION && tid < kARateCache) {
     s_rb[tid] = station_rate<T>(rr, tid, 1.0);
-    s_rs[tid] = station_rate<T>(rr, tid, a.rain_scale);
+    s_rs[tid] = station_rate<T>(rr, tid, pa->rain_scale);
   }
   __syncthreads();
-  const int P = (int)a.pitch;  // planes of fewer than 2^31 elements (checked at launch)
-  const int ny = a.ny;
-  const T* __restrict__ h = (const T*)a.h;
-  const T* __restrict__ qx = (const T*)a.qx;
-  const T* __restrict__ qy = (const T*)a.qy;
-  const T* __restrict__ dh = (const T*)a.dh;
-  const T* __restrict__ dqx = (const T*)a.dqx;
-  const T* __restrict__ dqy = (const T*)a.dqy;
-  const T scale = T(a.rain_scale);
+  const int P = (int)pa->pitch;  // planes of fewer than 2^31 elements (checked at launch)
+  const int ny = pa->ny;
+  const T* __restrict__ h = (const T*)pa->h;
+  const T* __restrict__ qx = (const T*)pa->qx;
+  const T* __restrict__ qy = (const T*)pa->qy;
+  const T* __restrict__ dh = (const T*)pa->dh;
+  const T* __restrict__ dqx = (const T*)pa->dqx;
+  const T* __restrict__ dqy = (const T*)pa->dqy;
+  const T scale = T(pa->rain_scale);
   double rsum = 0.0;
 
   // (1) update adjoint of the tile's cells and of its ring: first every
@@ -544,15 +547,15 @@ __global__ __launch_bounds__(NT, sizeof(T) == 4 ? 6 : 1) void k_adj_step_tile(hp
     L.lqx = lam.in[1][i];
     L.lqy = lam.in[2][i];
     L.g = p.gauge_of ? p.gauge_of[i] : -1;
-    L.cls = a.landuse[i];
+    L.cls = pa->landuse[i];
     L.h0 = h[i];
     L.qx = qx[i];
     L.qy = qy[i];
     L.dh = dh[i];
     L.dqx = dqx[i];
     L.dqy = dqy[i];
-    L.st = (mode == HP_RAIN_STATION && a.station) ? (int)a.station[i] : 0;
-    L.f = mode == HP_RAIN_FIELD ? a.rain_field[i] : 0.0f;
+    L.st = (mode == HP_RAIN_STATION && pa->station) ? (int)pa->station[i] : 0;
+    L.f = mode == HP_RAIN_FIELD ? pa->rain_field[i] : 0.0f;
     L.gn = owned ? gn[i] : 0.0;
   };
   auto finish = [&](int x, int y, bool owned, const CellLd& L, T(&lo)[3]) {
@@ -566,7 +569,7 @@ __global__ __launch_bounds__(NT, sizeof(T) == 4 ? 6 : 1) void k_adj_step_tile(hp
       if (mode == HP_RAIN_STATION) {
         const bool cached = L.st >= 0 && L.st < kARateCache;
         rb = cached ? s_rb[L.st] : station_rate<T>(rr, L.st, 1.0);
-        rs = cached ? s_rs[L.st] : station_rate<T>(rr, L.st, a.rain_scale);
+        rs = cached ? s_rs[L.st] : station_rate<T>(rr, L.st, pa->rain_scale);
       } else if (mode == HP_RAIN_FIELD) {
         rb = T(L.f);
         rs = T(L.f) * scale;
@@ -574,7 +577,7 @@ __global__ __launch_bounds__(NT, sizeof(T) == 4 ? 6 : 1) void k_adj_step_tile(hp
       double g_acc = L.gn, r_acc = 0.0;
       if (adj_update_cell<T>(L.cls, L.h0, L.qx, L.qy, L.dh, L.dqx, L.dqy, rb, rs, dt, s_man, lh, lqx, lqy, g_acc,
                              r_acc) &&
-          flux_listed<T>(a, h, x, y, t_T)) {
+          flux_listed<T>(pa, h, x, y, t_T)) {
         c0 = lh;
         c1 = lqx;
         c2 = lqy;
@@ -615,6 +618,7 @@ __global__ __launch_bounds__(NT, sizeof(T) == 4 ? 6 : 1) void k_adj_step_tile(hp
   __syncthreads();
 
   // (2) edges
+  pa = launder(pa);
   auto lds_cot = [&](int x, int y, T& ch, T& cx, T& cy) {
     const int ly = y - y0 + 1, lc = x - x0 + 1;
     ch = s.cot[0][ly][lc];
@@ -624,16 +628,17 @@ __global__ __launch_bounds__(NT, sizeof(T) == 4 ? 6 : 1) void k_adj_step_tile(hp
   T wb[kRowsPer][3], nb[kRowsPer][3];  // this thread's cells from their west and north edges
 #pragma unroll
   for (int k = 0; k < kRowsPer; ++k) {
+    pa = launder(pa);
     const int ly = lrow + k * (NT / kATX), col = x0 + lx, row = y0 + ly;
     T out[4][3] = {};
-    if (col <= a.nx && row < ny) edge_adj<T, 1, 0>(a, false, col, row, lds_cot, out);
+    if (col <= pa->nx && row < ny) edge_adj<T, 1, 0>(pa, false, col, row, lds_cot, out);
 #pragma unroll
     for (int v = 0; v < 3; ++v) {
       wb[k][v] = out[2][v];
       if (lx > 0) s.east[v][ly][lx - 1] = out[1][v];
     }
     T outy[4][3] = {};
-    if (col < a.nx && row <= ny) edge_adj<T, 1, 3>(a, false, col, row, lds_cot, outy);
+    if (col < pa->nx && row <= ny) edge_adj<T, 1, 3>(pa, false, col, row, lds_cot, outy);
 #pragma unroll
     for (int v = 0; v < 3; ++v) {
       nb[k][v] = outy[2][v];
@@ -642,16 +647,17 @@ __global__ __launch_bounds__(NT, sizeof(T) == 4 ? 6 : 1) void k_adj_step_tile(hp
   }
   // the tile's south row of y-edges (one wave) and east column of x-edges:
   // their A cells are the tile's last row and column
+  pa = launder(pa);
   if (tid < kATX) {
     const int col = x0 + tid, row = y0 + TY;
     T out[4][3] = {};
-    if (col < a.nx && row <= ny) edge_adj<T, 1, 3>(a, false, col, row, lds_cot, out);
+    if (col < pa->nx && row <= ny) edge_adj<T, 1, 3>(pa, false, col, row, lds_cot, out);
 #pragma unroll
     for (int v = 0; v < 3; ++v) s.south[v][TY - 1][tid] = out[1][v];
   } else if (tid < kATX + TY) {
     const int ly = tid - kATX, col = x0 + kATX, row = y0 + ly;
     T out[4][3] = {};
-    if (col <= a.nx && row < ny) edge_adj<T, 1, 0>(a, false, col, row, lds_cot, out);
+    if (col <= pa->nx && row < ny) edge_adj<T, 1, 0>(pa, false, col, row, lds_cot, out);
 #pragma unroll
     for (int v = 0; v < 3; ++v) s.east[v][ly][kATX - 1] = out[1][v];
   }
@@ -659,13 +665,14 @@ __global__ __launch_bounds__(NT, sizeof(T) == 4 ? 6 : 1) void k_adj_step_tile(hp
 
   // (3) the tile's cells: pass-through plus the four edges (as A: east and
   // south edge; as B: west and north edge)
+  pa = launder(pa);
 #pragma unroll
   for (int k = 0; k < kRowsPer; ++k) {
     const int ly = lrow + k * (NT / kATX), x = x0 + lx, y = y0 + ly;
     if (y >= ny || x >= P) continue;
     const int i = y * P + x;
     T o[3] = {lp[k][0], lp[k][1], lp[k][2]};
-    if (a.type_slot[i] != 0) {
+    if (pa->type_slot[i] != 0) {
 #pragma unroll
       for (int v = 0; v < 3; ++v) {
         T sum = T(0);

@@ -957,11 +957,13 @@ static int64_t update_blocks(const hp_grid_desc* g) {
   return (total + kChunk - 1) / kChunk;
 }
 
-// Rows per flux work unit: FP32 column-walk units of 60 columns x 4, 8 or
-// 16 rows (default 8); FP64 LDS tiles of 64 x 4 cells.
+// Rows per flux work unit: FP32 column-walk units of 60 columns x 4, 8, 12
+// or 16 rows (default 8); FP64 LDS tiles of 64 x 4 or 64 x 8 cells (default
+// 4).  Every launch of a domain's units takes its height from here.
 static int flux_rows(const hp_grid_desc* g) {
-  if (g->dtype == HP_F32) return (g->tile_rows == 4 || g->tile_rows == 16) ? g->tile_rows : 8;
-  return 4;
+  const int r = g->tile_rows;
+  if (g->dtype == HP_F32) return (r == 4 || r == 12 || r == 16) ? r : 8;
+  return r == 8 ? 8 : 4;
 }
 
 static int flux_cols(const hp_grid_desc* g) { return g->dtype == HP_F32 ? kWalkOwn : kTX; }
@@ -1031,20 +1033,29 @@ static int launch_walk_r(const hp_grid_desc* g, hipStream_t st, int flags) {
   return check_launch(FUSED ? "k_walk (single-pass step)" : "k_walk (flux pass)");
 }
 
+template <int ORDER, bool FUSED>
+static int launch_walk_o(const hp_grid_desc* g, hipStream_t st, int flags) {
+  switch (flux_rows(g)) {
+    case 4: return launch_walk_r<ORDER, 4, FUSED>(g, st, flags);
+    case 12: return launch_walk_r<ORDER, 12, FUSED>(g, st, flags);
+    case 16: return launch_walk_r<ORDER, 16, FUSED>(g, st, flags);
+    default: return launch_walk_r<ORDER, 8, FUSED>(g, st, flags);
+  }
+}
+
 template <bool FUSED>
 static int launch_walk(const hp_grid_desc* g, hipStream_t st, int flags) {
-  const int r = flux_rows(g);
-  if (g->order == 2) {
-    if (r == 4) return launch_walk_r<2, 4, FUSED>(g, st, flags);
-    return r == 16 ? launch_walk_r<2, 16, FUSED>(g, st, flags) : launch_walk_r<2, 8, FUSED>(g, st, flags);
-  }
-  if (r == 4) return launch_walk_r<1, 4, FUSED>(g, st, flags);
-  return r == 16 ? launch_walk_r<1, 16, FUSED>(g, st, flags) : launch_walk_r<1, 8, FUSED>(g, st, flags);
+  return g->order == 2 ? launch_walk_o<2, FUSED>(g, st, flags) : launch_walk_o<1, FUSED>(g, st, flags);
+}
+
+template <int ORDER>
+static int launch_flux_f64(const hp_grid_desc* g, hipStream_t st) {
+  return flux_rows(g) == 8 ? launch_flux_single<double, ORDER, 8>(g, st) : launch_flux_single<double, ORDER, 4>(g, st);
 }
 
 static int launch_flux(const hp_grid_desc* g, hipStream_t st) {
   if (g->dtype == HP_F32) return launch_walk<false>(g, st, 0);
-  return g->order == 2 ? launch_flux_single<double, 2, 4>(g, st) : launch_flux_single<double, 1, 4>(g, st);
+  return g->order == 2 ? launch_flux_f64<2>(g, st) : launch_flux_f64<1>(g, st);
 }
 
 template <class T, int ORDER, int TY>
@@ -1054,10 +1065,14 @@ static int launch_fused_t(const hp_grid_desc* g, hipStream_t st, int flags) {
   return check_launch("k_step_fused");
 }
 
+template <int ORDER>
+static int launch_fused_f64(const hp_grid_desc* g, hipStream_t st, int flags) {
+  return flux_rows(g) == 8 ? launch_fused_t<double, ORDER, 8>(g, st, flags) : launch_fused_t<double, ORDER, 4>(g, st, flags);
+}
+
 static int launch_step_fused(const hp_grid_desc* g, hipStream_t st, int flags = 0) {
   if (g->dtype == HP_F32) return launch_walk<true>(g, st, flags);
-  if (g->order == 2) return launch_fused_t<double, 2, 4>(g, st, flags);
-  return launch_fused_t<double, 1, 4>(g, st, flags);
+  return g->order == 2 ? launch_fused_f64<2>(g, st, flags) : launch_fused_f64<1>(g, st, flags);
 }
 
 static int launch_finalize_fused(const hp_grid_desc* g, hipStream_t st, const StepRecord* rec = nullptr) {
@@ -1159,13 +1174,20 @@ static void launch_walk_multi(const hp_ensemble* e, int parity, hipStream_t st) 
                      e->partials, (int)e->n_flux_tiles);
 }
 
+template <int ORDER, bool FUSED>
+static void launch_walk_multi_o(const hp_ensemble* e, int parity, hipStream_t st) {
+  switch (e->tile_rows) {
+    case 4: return launch_walk_multi<ORDER, 4, FUSED>(e, parity, st);
+    case 12: return launch_walk_multi<ORDER, 12, FUSED>(e, parity, st);
+    case 16: return launch_walk_multi<ORDER, 16, FUSED>(e, parity, st);
+    default: return launch_walk_multi<ORDER, 8, FUSED>(e, parity, st);
+  }
+}
+
 template <bool FUSED>
 static void launch_walk_multi_any(const hp_ensemble* e, int parity, hipStream_t st) {
-  const int r = e->tile_rows;
-  if (e->order == 2)
-    (r == 4 ? launch_walk_multi<2, 4, FUSED> : r == 16 ? launch_walk_multi<2, 16, FUSED> : launch_walk_multi<2, 8, FUSED>)(e, parity, st);
-  else
-    (r == 4 ? launch_walk_multi<1, 4, FUSED> : r == 16 ? launch_walk_multi<1, 16, FUSED> : launch_walk_multi<1, 8, FUSED>)(e, parity, st);
+  if (e->order == 2) launch_walk_multi_o<2, FUSED>(e, parity, st);
+  else launch_walk_multi_o<1, FUSED>(e, parity, st);
 }
 
 // One single-pass step of every domain: even steps read the domains' own
@@ -1174,8 +1196,9 @@ static int ensemble_fused_step_once(const hp_ensemble* e, int parity, hipStream_
   if (e->dtype == HP_F32) {
     launch_walk_multi_any<true>(e, parity, st);
   } else {
-    if (e->order == 2) launch_fused_multi<double, 2, 4>(e, parity, st);
-    else launch_fused_multi<double, 1, 4>(e, parity, st);
+    const bool t8 = e->tile_rows == 8;
+    if (e->order == 2) (t8 ? launch_fused_multi<double, 2, 8> : launch_fused_multi<double, 2, 4>)(e, parity, st);
+    else (t8 ? launch_fused_multi<double, 1, 8> : launch_fused_multi<double, 1, 4>)(e, parity, st);
   }
   int rc = check_launch("k_step_fused_multi");
   if (rc) return rc;
@@ -1190,8 +1213,9 @@ static int ensemble_step_once(const hp_ensemble* e, hipStream_t st) {
   if (f32) {
     launch_walk_multi_any<false>(e, 0, st);
   } else {
-    if (e->order == 2) launch_flux_multi<double, 2, 4>(e, st);
-    else launch_flux_multi<double, 1, 4>(e, st);
+    const bool t8 = e->tile_rows == 8;
+    if (e->order == 2) (t8 ? launch_flux_multi<double, 2, 8> : launch_flux_multi<double, 2, 4>)(e, st);
+    else (t8 ? launch_flux_multi<double, 1, 8> : launch_flux_multi<double, 1, 4>)(e, st);
   }
   int rc = check_launch("k_flux_multi");
   if (rc) return rc;

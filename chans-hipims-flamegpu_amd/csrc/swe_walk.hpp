@@ -61,25 +61,23 @@ struct WPart {
   float m, across, along;
 };
 
-// The walk reads the domain descriptor through a pointer that is laundered
-// at every row (an empty asm that the compiler must treat as changing it), so
-// descriptor fields are fetched by scalar loads where they are used instead of
-// all being loaded up front and held in scalar registers across the walk
-// (which spills them).  The single-domain kernel points at its own kernel
+// The walk reads the domain descriptor through a KDesc that is laundered at
+// every row, so descriptor fields are fetched by scalar loads where they are
+// used instead of all being held in scalar registers across the walk (which
+// spills them).  The single-domain kernel points at its own kernel
 // arguments, the ensemble kernel at the descriptor array in its workspace.
-typedef __attribute__((address_space(4))) const hp_grid_desc* KDesc;
-template <class DP> __device__ inline DP launder(DP p) {
-  asm volatile("" : "+s"(p));
-  return p;
-}
 
 // Plane access by 32-bit byte offset from the plane's base (scalar base +
-// vector offset addressing); e is the plane element.
+// vector offset addressing); e is the plane element.  The planes are device
+// memory: the accesses name the global address space, so they compile to
+// global loads and stores whatever memory the descriptor holding the base
+// came from.
+#define HP_GLOBAL __attribute__((address_space(1)))
 template <class V> __device__ inline V pl_ld(const void* base, uint32_t e) {
-  return *(const V*)((const char*)base + e * (uint32_t)sizeof(V));
+  return *(const HP_GLOBAL V*)((const HP_GLOBAL char*)base + e * (uint32_t)sizeof(V));
 }
 template <class V> __device__ inline void pl_st(void* base, uint32_t e, V v) {
-  *(V*)((char*)base + e * (uint32_t)sizeof(V)) = v;
+  *(HP_GLOBAL V*)((HP_GLOBAL char*)base + e * (uint32_t)sizeof(V)) = v;
 }
 
 struct WalkEnv {
@@ -372,8 +370,10 @@ __device__ inline void walk_row(const WalkEnv& env, DP a, const WRow& cur, const
 // the step into h_out / qx_out / qy_out with one partial record per unit
 // (none without time control); save_flux also writes the flux; force_copy
 // makes a stopped domain copy its state through even when marked in step.
+// (inlined into each kernel: the launder needs the descriptor pointer known
+// uniform, which a call shared by two kernels would not let the compiler see)
 template <int ORDER, int R, bool FUSED, class DP>
-__device__ inline void walk_unit(DP a, int bx, int by, const Partials& part, int64_t pidx, bool save_flux,
+__device__ __forceinline__ void walk_unit(DP a, int bx, int by, const Partials& part, int64_t pidx, bool save_flux,
                                  bool time_control, bool force_copy) {
   static_assert(R % 4 == 0, "rows per unit: a multiple of the four-row window");
   __shared__ float s_rate[FUSED ? kMaxStations : 1];
@@ -471,7 +471,7 @@ __device__ inline void walk_unit(DP a, int bx, int by, const Partials& part, int
 #endif
 template <int ORDER, int R, bool FUSED>
 __global__ __launch_bounds__(kWalkLanes) HP_WALK_ATTR void k_walk(hp_grid_desc a, int nstrips, int flags) {
-  const KDesc pa = (KDesc)__builtin_amdgcn_kernarg_segment_ptr();
+  const KDesc pa = kernarg_desc();
   const int unit = xcd_band(blockIdx.x, gridDim.x);
   walk_unit<ORDER, R, FUSED>(pa, unit % nstrips, unit / nstrips, Partials(a.work, gridDim.x), unit, (flags & 1) != 0,
                              (flags & 2) == 0, (flags & 4) != 0);
@@ -483,10 +483,13 @@ template <int ORDER, int R, bool FUSED>
 __global__ __launch_bounds__(kWalkLanes) void k_walk_multi(const hp_grid_desc* __restrict__ doms,
                                                            const int2* __restrict__ map, void* parts, int n_total) {
   const int slot = xcd_band(blockIdx.x, gridDim.x);
-  const int2 m = map[slot];
-  const hp_grid_desc* a = doms + m.x;
+  // (the map entry is the same for the whole wave)
+  const int dom = __builtin_amdgcn_readfirstlane(map[slot].x), unit = __builtin_amdgcn_readfirstlane(map[slot].y);
+  // the descriptor array is read-only for the launch: read it as constant
+  // memory, by scalar loads, as the single-domain kernel reads its argument
+  const KDesc a = (KDesc)(doms + dom);
   const int nstrips = (a->nx + kWalkOwn - 1) / kWalkOwn;
-  walk_unit<ORDER, R, FUSED>(a, m.y % nstrips, m.y / nstrips, Partials(parts, n_total), slot, false, true, false);
+  walk_unit<ORDER, R, FUSED>(a, unit % nstrips, unit / nstrips, Partials(parts, n_total), slot, false, true, false);
 }
 
 }  // namespace hp

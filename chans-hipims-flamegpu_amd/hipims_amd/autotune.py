@@ -1,10 +1,14 @@
-"""Flux tile-height autotuner.
+"""Flux unit-height autotuner.
 
-The flux pass stages 64 x TY cells plus a 2-cell halo in LDS.  A taller tile
-re-solves fewer halo edges but holds more LDS per block, so fewer blocks fit
-a CU; which wins depends on dtype, order, domain size and on the step path
-(the single-pass step carries the update in the same block).  tune() times
-each legal TY on the caller's own domain with HIP events - the flux pass
+Float domains run the flux pass and the single-pass step as column walks:
+one wave per unit of 60 columns x R rows, R in {4, 8, 12, 16}.  A taller
+unit re-loads fewer halo rows and solves fewer duplicate y-edges (four
+prologue rows and one edge per unit), a shorter one gives more waves, which
+matters when the last round of waves on the 256 CUs would run part-empty.
+Double domains stage 64 x TY cells plus a 2-cell halo in LDS, TY in {4, 8}: a
+taller tile solves fewer halo edges but holds more LDS per block.  Which
+wins depends on dtype, order, domain size and on the step path.  tune() times
+each legal height on the caller's own domain with HIP events - the flux pass
 alone, or whole single-pass steps - restores the domain's state afterwards
 and keeps the fastest; results are memoised per (dtype, order, size class,
 path) for the process.
@@ -15,7 +19,7 @@ import torch
 
 from .grid import GridSolver
 
-CHOICES = {torch.float32: (4, 8, 16), torch.float64: (4,)}
+CHOICES = {torch.float32: (4, 8, 12, 16), torch.float64: (4, 8)}
 _cache: dict = {}
 
 

@@ -86,18 +86,26 @@ def diamond_square(n: int, seed: int, roughness: float = 0.6) -> np.ndarray:
         # diamond: centres of squares
         c = (g[:-1:step, :-1:step] + g[:-1:step, step::step] + g[step::step, :-1:step] + g[step::step, step::step]) / 4
         g[half::step, half::step] = c + amp * rng.standard_normal(c.shape)
-        # square: edge midpoints
-        ys, xs = np.mgrid[0:size:half, 0:size:half]
-        sel = ((ys // half + xs // half) % 2) == 1
-        ys, xs = ys[sel], xs[sel]
-        acc = np.zeros(ys.shape)
-        cnt = np.zeros(ys.shape)
-        for dy, dx_ in ((-half, 0), (half, 0), (0, -half), (0, half)):
-            yy, xx = ys + dy, xs + dx_
-            ok = (yy >= 0) & (yy < size) & (xx >= 0) & (xx < size)
-            acc[ok] += g[yy[ok], xx[ok]]
-            cnt[ok] += 1
-        g[ys, xs] = acc / cnt + amp * rng.standard_normal(ys.shape)
+        # square: edge midpoints, the points (i, j) of the lattice of
+        # spacing `half` with i + j odd; each takes the mean of its lattice
+        # neighbours (north, south, west, east, in that order of summation;
+        # three on the border) plus noise drawn in row-major order
+        lat = g[::half, ::half]
+        m = lat.shape[0]
+        acc = np.zeros((m, m))
+        cnt = np.zeros((m, m))
+        acc[1:, :] += lat[:-1, :]
+        cnt[1:, :] += 1
+        acc[:-1, :] += lat[1:, :]
+        cnt[:-1, :] += 1
+        acc[:, 1:] += lat[:, :-1]
+        cnt[:, 1:] += 1
+        acc[:, :-1] += lat[:, 1:]
+        cnt[:, :-1] += 1
+        odd = np.zeros((m, m), dtype=bool)
+        odd[0::2, 1::2] = True
+        odd[1::2, 0::2] = True
+        lat[odd] = acc[odd] / cnt[odd] + amp * rng.standard_normal(int(odd.sum()))
         step, amp = half, amp * roughness
     return g[:n, :n]
 

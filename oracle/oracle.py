@@ -58,6 +58,9 @@ def lib():
         L.orc_infiltration.argtypes = [_I, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P]
         L.orc_run_steps.argtypes = ([_I, _I, _I64, _I, _P] + [_P] * 10 + [_P, _P, _P, _P, _I, _I, _P]
                                     + [_P, _I, _I, _P, _I, _I, _P, _I, _I, _D, _D, _P, _P, _P, _P, _P, _P])
+        L.orc_frozen_slope_loss.restype = _D
+        L.orc_frozen_slope_loss.argtypes = [_I64, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _D, _D,
+                                            _P, _P, _I, _P]
         _lib = L
     return _lib
 
@@ -258,3 +261,30 @@ class OracleSim:
         out = np.full(self.mask.shape, fill, dtype=arr.dtype)
         out[self.mask > 0] = arr
         return out
+
+
+def frozen_slope_loss(src: OracleSim, pert: OracleSim, sched: np.ndarray, gauges: np.ndarray,
+                      obs: np.ndarray) -> float:
+    """Gauge loss of `pert` run on the schedule sched[k] = (t_k, dt_k) with
+    the order-2 slopes of `src` (advanced beside it on the same schedule):
+    sum_k sum_g (h after step k at compact cell gauges[g] - obs[k, g])^2.
+    A test construct with no counterpart in the reference: its central
+    differences in pert's Manning / rain tables are the derivative that the
+    frozen-slope adjoint computes.  Both sims FP64, order 2, same grid and
+    stations; their states advance."""
+    for s_ in (src, pert):
+        if s_.dtype != np.float64 or s_.order != 2 or s_.rt is None:
+            raise ValueError("frozen_slope_loss takes FP64 order-2 sims with a rain table")
+    if src.rt.shape != pert.rt.shape:
+        raise ValueError("both rain tables must have one shape")
+    sched = np.ascontiguousarray(sched, dtype=np.float64).reshape(-1, 2)
+    gauges = np.ascontiguousarray(gauges, dtype=np.int32)
+    obs = np.ascontiguousarray(obs, dtype=np.float64).reshape(sched.shape[0], gauges.shape[0])
+    planes = lambda s_: (ctypes.c_void_p * 5)(*[a.ctypes.data for a in (s_.h_c, s_.wl_c, s_.qx_c, s_.qy_c,
+                                                                         s_.h_max)])
+    a0, a1 = planes(src), planes(pert)
+    rows, cols = src.rt.shape
+    return float(lib().orc_frozen_slope_loss(
+        src.N, sched.shape[0], _p(src.index), ctypes.cast(a0, _P), ctypes.cast(a1, _P), _p(src.z_c), _p(src.lu),
+        _p(src.man), _p(pert.man), _p(src.st), _p(src.rt), _p(pert.rt), rows, cols, float(src.dx), float(src.cfl),
+        _p(sched), _p(gauges), gauges.shape[0], _p(obs)))

@@ -225,11 +225,20 @@ inline T table_at(const void* tab, int rows, int w, int row, int col) {
   return static_cast<const T*>(tab)[row * w + col];
 }
 
+//
+// Slope source (hb, qxb, qyb; test construct, not in the reference): when
+// given, the order-2 reconstruction takes its limited slopes and its
+// "cell below hs carries no momentum" rule from that state instead of from
+// (h, qx, qy): face = own cell value + slope of the source state.  Run with
+// the source = an unperturbed trajectory, it is the forward model whose
+// derivative the frozen-slope adjoint computes, so central differences of it
+// check that adjoint.
 template <class T, int ORDER>
 void flux_compact(int64_t M, int64_t N, const int32_t* wet, T* dh, T* dqx, T* dqy,
                   const T* h, const T* wl, const T* z, const T* qx, const T* qy,
                   const int32_t* index, const T* normal, const Tables& tb,
-                  const T* dxp, const T* tp, const T* dtp) {
+                  const T* dxp, const T* tp, const T* dtp,
+                  const T* hb = nullptr, const T* qxb = nullptr, const T* qyb = nullptr) {
   const T hs = C<T>::hs, g = C<T>::g;
   const T dx = dxp[0], t = tp[0], dt = dtp[0];
   const int ld = series_row<T>(static_cast<const T*>(tb.gd), tb.gd_rows, tb.gd_w, t);
@@ -312,8 +321,27 @@ void flux_compact(int64_t M, int64_t N, const int32_t* wet, T* dh, T* dqx, T* dq
               e[QY] = c[QY] + T(0.5) * minmod<T>(c[QY] - l[QY], r[QY] - c[QY]);
             }
           };
-          muscl(fc, fl, fr, el);
-          muscl(fr, frr, fc, er);
+          if (hb) {
+            // the same reconstruction with the slopes of the source state
+            auto frozen = [&](int32_t ci, int32_t li, int32_t ri, const T* c, T* e) {
+              auto sl = [&](const T* v) { return T(0.5) * minmod<T>(v[ci] - v[li], v[ri] - v[ci]); };
+              e[H] = c[H] + sl(hb);
+              e[WL] = c[WL];
+              e[Z] = c[Z] + T(0.5) * minmod<T>(z[ci] - z[li], z[ri] - z[ci]);
+              if (hb[ci] < hs) {
+                e[QX] = T(0); e[QY] = T(0);
+              } else {
+                e[QX] = c[QX] + sl(qxb);
+                e[QY] = c[QY] + sl(qyb);
+              }
+            };
+            const int32_t li = (l_id == -1) ? i : l_id, rri = (rr_id == -1) ? r_id : rr_id;
+            frozen(i, li, r_id, fc, el);
+            frozen(r_id, rri, i, fr, er);
+          } else {
+            muscl(fc, fl, fr, el);
+            muscl(fr, frr, fc, er);
+          }
         }
         if (fc[H] < hs && fr[H] < hs) continue;
         hllc<T, 2>(fc, fr, el, er, nrm, tg, dh[i], dqx[i], dqy[i], dx, dt);
@@ -487,6 +515,7 @@ template <class T> struct Sim {
   const T* manning;
   const int16_t* station;      // may be null (no rain)
   const T *ks, *phi, *dth;     // Green-Ampt class tables; null: no infiltration
+  const T *hb = nullptr, *qxb = nullptr, *qyb = nullptr;  // order-2 slope source; null: own state
   T* F;                        // cumulative infiltrated depth
   const double* rain_table;    // [rows, 1+S] in m/s (SWE_CUDA.py:472-495)
   int rain_rows, rain_cols;
@@ -540,7 +569,7 @@ template <class T> void sim_step(Sim<T>& s) {
   if (!s.list.empty()) {
     if (s.order == 2)
       flux_compact<T, 2>((int64_t)s.list.size(), N, s.list.data(), s.dh, s.dqx, s.dqy, s.h, s.wl, s.z, s.qx,
-                         s.qy, s.index, default_normal<T>(), s.tb, &s.dx, &s.t, &s.dt);
+                         s.qy, s.index, default_normal<T>(), s.tb, &s.dx, &s.t, &s.dt, s.hb, s.qxb, s.qyb);
     else
       flux_compact<T, 1>((int64_t)s.list.size(), N, s.list.data(), s.dh, s.dqx, s.dqy, s.h, s.wl, s.z, s.qx,
                          s.qy, s.index, default_normal<T>(), s.tb, &s.dx, &s.t, &s.dt);
@@ -721,6 +750,54 @@ int orc_run_steps(int is_f64, int order, int64_t N, int nsteps, const int32_t* i
     return s.n_exports;
   };
   return is_f64 ? run(double(0)) : run(float(0));
+}
+
+// Frozen-slope loss (test construct for the adjoint checks, not in the
+// reference; FP64, order 2): advances a source trajectory (state 0) and a
+// perturbed one (state 1, its own Manning table and rain table) together on
+// a given schedule sched[k] = (t_k, dt_k); the perturbed flux reconstructs
+// with the source state's slopes (flux_compact's slope source).  Returns
+// sum_k sum_g (h1[gauge_g] after step k - obs[k * G + g])^2.
+double orc_frozen_slope_loss(int64_t N, int nsteps, const int32_t* index, double* const* s0, double* const* s1,
+                             const double* z, const uint8_t* landuse, const double* man0, const double* man1,
+                             const int16_t* station, const double* rain0, const double* rain1, int rain_rows,
+                             int rain_cols, double dx, double cfl, const double* sched, const int32_t* gauges,
+                             int G, const double* obs) {
+  static const double zero2[2] = {0, 0}, zero3[3] = {0, 0, 0};
+  std::vector<double> work(2 * 5 * (size_t)N, 0.0), zc(2 * (size_t)N);
+  Sim<double> sim[2];
+  for (int v = 0; v < 2; ++v) {
+    Sim<double>& s = sim[v];
+    double* const* a = v ? s1 : s0;  // h, wl, qx, qy, h_max
+    double* w = work.data() + (size_t)v * 5 * N;
+    std::copy(z, z + N, zc.begin() + (size_t)v * N);
+    s.N = N; s.index = index;
+    s.h = a[0]; s.wl = a[1]; s.qx = a[2]; s.qy = a[3]; s.h_max = a[4]; s.z = zc.data() + (size_t)v * N;
+    s.dh = w; s.dqx = w + N; s.dqy = w + 2 * N; s.dz = w + 3 * N; s.F = w + 4 * N;
+    s.landuse = landuse; s.manning = v ? man1 : man0; s.station = station;
+    s.ks = s.phi = s.dth = nullptr;
+    s.rain_table = v ? rain1 : rain0; s.rain_rows = rain_rows; s.rain_cols = rain_cols;
+    s.order = 2;
+    s.tb = Tables{zero2, 1, 2, zero2, 1, 2, zero3, 1, 3};
+    s.dx = dx; s.cfl = cfl;
+    s.export_n = 0; s.export_T = 1.0e30;
+    s.rain_index = 0;
+    s.n_exports = 0;
+  }
+  sim[1].hb = sim[0].h; sim[1].qxb = sim[0].qx; sim[1].qyb = sim[0].qy;
+  double loss = 0.0;
+  for (int k = 0; k < nsteps; ++k) {
+    for (int v = 1; v >= 0; --v) {  // the perturbed step reads the source state before it advances
+      sim[v].t = sched[2 * k];
+      sim[v].dt = sched[2 * k + 1];
+      sim_step<double>(sim[v]);
+    }
+    for (int g = 0; g < G; ++g) {
+      const double d = sim[1].h[gauges[g]] - obs[(size_t)k * G + g];
+      loss += d * d;
+    }
+  }
+  return loss;
 }
 
 }  // extern "C"

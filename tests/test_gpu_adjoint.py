@@ -200,3 +200,97 @@ def test_overlapped_recompute_matches_serial(cuda):
     say(f"  {g[True].numpy()}")
     assert torch.equal(g[False], g[True])
     assert all(torch.equal(a, b) for a, b in zip(lam[False], lam[True]))
+
+
+@pytest.mark.xfail(reason="open: the frozen-slope adjoint removes the slope terms of the exact order-2 adjoint "
+                          "only to within 1-2 % of their size (gradient errors 1e-3 to 1e-2 against central "
+                          "differences of the frozen-slope model from 3 steps on), cause not found",
+                   strict=False)
+def test_order2_frozen_slope_adjoint_matches_fd_of_frozen_slope_forward(oracle_lib, cuda):
+    """Order 2, 200 steps in segments of 50, FP64: the frozen-slope adjoint
+    (slope terms of the MUSCL faces held at the unperturbed run's) is the
+    exact derivative of the forward model whose faces are the cell value plus
+    the unperturbed run's limited slope.  The oracle runs that model (beside
+    the unperturbed trajectory, on the recorded schedule, against the same
+    observations), and its central differences at eps 1e-6 must match the
+    adjoint to 1e-4 of the largest component.
+
+    Order-2 runs amplify rounding differences (a 1e-15 perturbation reaches
+    1e-2 within a few hundred steps on this case), so the reverse sweep runs
+    on the oracle's trajectory: its states replace the checkpoints and its
+    gauge depths the recorded ones, and the HIP path recomputes only the 50
+    steps of each segment from them.  The horizon is 200 steps because
+    beyond about 300 even the frozen-slope model's central differences
+    depend on eps (its branch switches: HLLC regions, hydrostatic clamps,
+    wet/dry), so no derivative of an order-2 run is defined there to check
+    against."""
+    say("[adjoint order 2, frozen slopes, fp64, 200 steps, ck 50, vs FD of the frozen-slope forward]")
+    K, ck = 200, 50
+    c = gen.rainfall_runoff(96, 5.0, seed=4242, rain_mmh=150.0, relief=8.0)
+
+    def solver():
+        s = GridSolver(c.mask, c.h, c.qx, c.qy, c.z, c.dx, cfl=0.5, order=2, dtype=torch.float64, first_dt=0.5,
+                       manning=c.manning, landuse=c.landuse)
+        s.set_station_rain(c.rain_table)
+        return s
+    pilot = solver()
+    pilot.step(K)
+    h = pilot.raster(pilot.h)
+    rng = np.random.Generator(np.random.PCG64(1004))
+    wet = np.argwhere((h > 1e-3) & (c.mask > 0))
+    gauges = wet[rng.choice(len(wet), size=min(16, len(wet)), replace=False)]
+    cal = Calibration(solver(), gauges, K, ck, slopes="frozen")
+    th0 = cal.theta0()
+    pert = torch.tensor(np.random.Generator(np.random.PCG64(7)).choice([0.8, 1.2], size=th0.numel() - 1))
+    th_true = th0.clone()
+    th_true[:-1] *= pert
+    cal.obs = cal.simulate(th_true)
+    cal.run(th0)
+
+    sched = cal.sched[:K].cpu().numpy()
+    obs = cal.obs[1:].cpu().numpy()
+    compact_of = np.cumsum(c.mask > 0).reshape(c.mask.shape) - 1
+    gauges_c = compact_of[gauges[:, 0], gauges[:, 1]].astype(np.int32)
+    C = th0.numel() - 1
+
+    def sim(theta):
+        th = np.asarray(theta, dtype=np.float64)
+        rt = c.rain_table.copy()
+        rt[:, 1:] *= th[C]
+        return oracle_lib.OracleSim(c.mask, c.h, c.qx, c.qy, c.z, c.dx, cfl=0.5, order=2, first_dt=0.5,
+                                    manning=th[:C], landuse=c.landuse, rain_table=rt,
+                                    station=np.zeros_like(c.mask))
+
+    # the oracle's trajectory on the recorded schedule: checkpoint states and
+    # gauge depths, loaded into the calibration in place of its own
+    o = sim(th0.numpy())
+    hist = np.zeros((K, len(gauges_c)))
+    s = cal.s
+    for k in range(K):
+        if k % ck == 0:
+            for v, arr in enumerate((o.h_c, o.qx_c, o.qy_c)):
+                s.load_compact(cal.ckpts[k][v], torch.from_numpy(arr))
+        o.ctrl[0:2] = sched[k]
+        o.run(1)
+        hist[k] = o.h_c[gauges_c]
+    cal.tail = None
+    cal.hist[1:] = torch.from_numpy(hist).to(cal.hist)
+    grad = cal.gradient().numpy()
+    loss = float(((hist - obs) ** 2).sum())
+
+    def frozen_loss(theta):
+        return oracle_lib.frozen_slope_loss(sim(th0.numpy()), sim(theta), sched, gauges_c, obs)
+
+    l_cpu = frozen_loss(th0.numpy())
+    say(f"  loss {loss:.12e} (frozen-slope model at theta0 {l_cpu:.12e})")
+    assert abs(l_cpu - loss) <= 1e-12 * loss
+    fd = np.zeros(th0.numel())
+    for j in range(th0.numel()):
+        d = 1e-6 * abs(float(th0[j]))
+        tp, tm = th0.numpy().copy(), th0.numpy().copy()
+        tp[j] += d
+        tm[j] -= d
+        fd[j] = (frozen_loss(tp) - frozen_loss(tm)) / (2 * d)
+    err = float(np.abs(grad - fd).max() / np.abs(grad).max())
+    say(f"  adjoint {grad}\n  fd      {fd}\n  max rel err {err:.2e}")
+    assert np.isfinite(grad).all() and err <= 1e-4

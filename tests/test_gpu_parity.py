@@ -303,27 +303,28 @@ def _fp32_errors(h, qx, qy, o64):
             "p999_dq": float(np.quantile(dq, 0.999)), "vol": abs(h.sum() - o64.h_c.sum()) / o64.h_c.sum()}
 
 
-def test_fp32_error_against_fp64_oracle(oracle_lib, cuda):
-    """FP32 structured path vs the FP64 oracle over a rainfall-runoff run, all
-    runs driven with the FP64 time-step schedule so states are compared at
-    the same simulated times.  Pointwise differences are not small in FP32
-    for this problem (the friction limiter and the wet/dry thresholds are
-    discontinuous), so the bar is the reference algorithm's own FP32
-    instantiation (the oracle in float): the HIP FP32 path must be no less
-    accurate than it in L1 depth and discharge, conserve volume to 1e-6 and
-    close the mass balance."""
-    say("[fp32 vs fp64 oracle]")
-    c = gen.rainfall_runoff(128, 5.0, rain_mmh=100.0)
+@pytest.mark.parametrize("n,steps", [(128, 300), (2048, 100)], ids=["128sq_300steps", "2048sq_100steps"])
+def test_fp32_error_against_fp64_oracle(oracle_lib, cuda, n, steps):
+    """FP32 single-pass step vs the FP64 oracle over a rainfall-runoff run,
+    every run driven with the FP64 time-step schedule so states are compared
+    at the same simulated times.  The bar is the reference algorithm's own
+    FP32 instantiation (the oracle in float, wet/dry tests in float): the HIP
+    FP32 path must be no less accurate than it in max |dh|, max |dq| and L1
+    depth and discharge, conserve volume to 1e-6 and close the FP64-summed
+    mass balance to 1e-6."""
+    say(f"[fp32 vs fp64 oracle, {n}^2, {steps} steps]")
+    c = gen.rainfall_runoff(n, 5.0, rain_mmh=100.0)
     st = np.zeros_like(c.mask)
     g, o = make_pair(oracle_lib, c, 2, dtype=torch.float32, rain=c.rain_table, station=st)
+    g.set_mass_sums(True)
     o32 = oracle_lib.OracleSim(c.mask, c.h, c.qx, c.qy, c.z, c.dx, cfl=c.cfl, order=2, dtype=np.float32,
                                first_dt=c.first_dt, manning=c.manning, landuse=c.landuse,
                                rain_table=c.rain_table, station=st)
-    for _ in range(300):
+    for _ in range(steps):
         g.ctrl[0] = o.time
         g.ctrl[1] = o.dt
         o32.ctrl[0:2] = o.ctrl[0:2]
-        g.step(1)
+        g.step_fused(1)
         o32.run(1)
         o.run(1)
     torch.cuda.synchronize()
@@ -336,6 +337,8 @@ def test_fp32_error_against_fp64_oracle(oracle_lib, cuda):
     say(f"  t {o.time:.1f} s, max h {o.h_c.max():.3f} m")
     say(f"  HIP fp32      : {fmt(e_hip)}; balance err {balance:.2e}")
     say(f"  reference fp32: {fmt(e_ref)}")
+    assert e_hip["max_dh"] <= max(e_ref["max_dh"], 1e-5)
+    assert e_hip["max_dq"] <= max(e_ref["max_dq"], 1e-5)
     assert e_hip["l1_h"] <= max(e_ref["l1_h"], 1e-5)
     assert e_hip["l1_q"] <= max(e_ref["l1_q"], 1e-4)
     assert e_hip["vol"] < 1e-6
