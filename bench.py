@@ -220,8 +220,9 @@ def parse():
     p.add_argument("--ens-two-pass", action="store_true", help="ensemble: flux and update passes per step")
     p.add_argument("--no-baselines", action="store_true",
                    help="ensemble: skip the sequential baseline runs (for long simulated times)")
-    p.add_argument("--ensemble-seconds", type=float, default=7200.0,
-                   help="step mode: simulated time of the ensemble run reported beside the step (0: none)")
+    p.add_argument("--ensemble-seconds", type=float, default=900.0,
+                   help="step mode: simulated time of every scenario in the ensemble run reported beside the "
+                        "step (0: none; the configured 2 h takes minutes: --mode ensemble --sim-seconds 7200)")
     p.add_argument("--no-adjoint", action="store_true", help="step mode: no calibration gradient run beside the step")
     p.add_argument("--adj-size", type=int, default=1024)
     p.add_argument("--adj-steps", type=int, default=2000)
@@ -396,6 +397,7 @@ def standalone_graphs(cases, t_stop, steps, dev):
     from hipims_amd import _lib
     total = 0.0
     counts = []
+    last = time.perf_counter()
     for k, c in enumerate(cases):
         s = _ensemble_solver(c, dev)
         s.set_t_stop(t_stop)
@@ -413,6 +415,9 @@ def standalone_graphs(cases, t_stop, steps, dev):
         total += time.perf_counter() - t0
         counts.append(int(s.ctrl_dict()["steps"]))
         del g, s
+        if time.perf_counter() - last > 10.0:
+            last = time.perf_counter()
+            print(f"[ensemble] standalone graphs: {k + 1} scenarios, {total:.1f} s", file=sys.stderr, flush=True)
     print(f"[ensemble] standalone graphs: {total:.2f} s", file=sys.stderr, flush=True)
     return total, counts
 

@@ -294,3 +294,100 @@ def test_order2_frozen_slope_adjoint_matches_fd_of_frozen_slope_forward(oracle_l
     err = float(np.abs(grad - fd).max() / np.abs(grad).max())
     say(f"  adjoint {grad}\n  fd      {fd}\n  max rel err {err:.2e}")
     assert np.isfinite(grad).all() and err <= 1e-4
+
+
+def _oracle_loss(o, sched, gauges_c, obs):
+    """The calibration loss of an oracle run on a given schedule."""
+    loss = 0.0
+    for k in range(sched.shape[0]):
+        o.ctrl[0:2] = sched[k]
+        o.run(1)
+        d = o.h_c[gauges_c] - obs[k]
+        loss += float((d * d).sum())
+    return loss
+
+
+@pytest.mark.parametrize("order,n,steps,ck,warm", [(1, 64, 2000, 100, 0), (2, 96, 20, 5, 300)],
+                         ids=["order1_2000steps", "order2_20steps_wet"])
+def test_adjoint_matches_oracle_central_differences(oracle_lib, cuda, order, n, steps, ck, warm):
+    """The FP64 adjoint against central differences of the oracle (the CPU
+    restatement of the reference step), not of the HIP forward: the reverse
+    sweep runs on the oracle's trajectory (its states as the checkpoints, its
+    gauge depths as the record, the HIP path recomputing each segment from
+    them), and the differences come from oracle runs on the recorded
+    schedule.  Order 1 over the configured 2000 steps; order 2 over 20 steps
+    from a wet state (300 oracle steps in), the horizon over which the
+    order-2 model is differentiable on this case.  Over 2000 steps a cell
+    crossing the wet/dry threshold between the plus and minus runs puts a
+    jump into a single difference (order 1, one parameter: -9.97, -10.28,
+    -9.91, -9.886, -9.886 at eps 1e-4 .. 1e-8), so each component is the
+    median of the differences at eps 1e-6, 1e-7 and 1e-8.  Bar 1e-4 of the
+    largest component."""
+    say(f"[adjoint order {order} vs oracle central differences, {steps} steps]")
+    c = gen.rainfall_runoff(n, 5.0, seed=4242, rain_mmh=150.0, relief=8.0)
+    first_dt = 0.5
+    if warm:
+        o = oracle_lib.OracleSim(c.mask, c.h, c.qx, c.qy, c.z, c.dx, cfl=0.5, order=order, first_dt=first_dt,
+                                 manning=c.manning, landuse=c.landuse, rain_table=c.rain_table,
+                                 station=np.zeros_like(c.mask))
+        o.run(warm)
+        c.h, c.qx, c.qy = o.raster("h_c", 0.0), o.raster("qx_c", 0.0), o.raster("qy_c", 0.0)
+        first_dt = o.dt
+
+    def solver():
+        s = GridSolver(c.mask, c.h, c.qx, c.qy, c.z, c.dx, cfl=0.5, order=order, dtype=torch.float64,
+                       first_dt=first_dt, manning=c.manning, landuse=c.landuse)
+        s.set_station_rain(c.rain_table)
+        return s
+    pilot = solver()
+    pilot.step(steps)
+    h = pilot.raster(pilot.h)
+    rng = np.random.Generator(np.random.PCG64(1004))
+    wet = np.argwhere((h > 1e-3) & (c.mask > 0))
+    gauges = wet[rng.choice(len(wet), size=min(16, len(wet)), replace=False)]
+    cal = Calibration(solver(), gauges, steps, ck)
+    th0 = cal.theta0()
+    C = th0.numel() - 1
+    pert = torch.tensor(np.random.Generator(np.random.PCG64(7)).choice([0.8, 1.2], size=C))
+    th_true = th0.clone()
+    th_true[:-1] *= pert
+    cal.obs = cal.simulate(th_true)
+    cal.run(th0)
+    sched = cal.sched[:steps].cpu().numpy()
+    obs = cal.obs[1:].cpu().numpy()
+    compact_of = np.cumsum(c.mask > 0).reshape(c.mask.shape) - 1
+    gauges_c = compact_of[gauges[:, 0], gauges[:, 1]]
+
+    def sim(theta):
+        th = np.asarray(theta, dtype=np.float64)
+        rt = c.rain_table.copy()
+        rt[:, 1:] *= th[C]
+        return oracle_lib.OracleSim(c.mask, c.h, c.qx, c.qy, c.z, c.dx, cfl=0.5, order=order, first_dt=first_dt,
+                                    manning=th[:C], landuse=c.landuse, rain_table=rt,
+                                    station=np.zeros_like(c.mask))
+
+    o = sim(th0.numpy())
+    hist = np.zeros((steps, len(gauges_c)))
+    for k in range(steps):
+        if k % ck == 0:
+            for v, arr in enumerate((o.h_c, o.qx_c, o.qy_c)):
+                cal.s.load_compact(cal.ckpts[k][v], torch.from_numpy(arr))
+        o.ctrl[0:2] = sched[k]
+        o.run(1)
+        hist[k] = o.h_c[gauges_c]
+    cal.tail = None
+    cal.hist[1:] = torch.from_numpy(hist).to(cal.hist)
+    grad = cal.gradient().numpy()
+    fds = np.zeros((3, C + 1))
+    for e, eps in enumerate((1e-6, 1e-7, 1e-8)):
+        for j in range(C + 1):
+            d = eps * abs(float(th0[j]))
+            tp, tm = th0.numpy().copy(), th0.numpy().copy()
+            tp[j] += d
+            tm[j] -= d
+            fds[e, j] = (_oracle_loss(sim(tp), sched, gauges_c, obs)
+                         - _oracle_loss(sim(tm), sched, gauges_c, obs)) / (2 * d)
+    fd = np.median(fds, axis=0)
+    err = float(np.abs(grad - fd).max() / np.abs(grad).max())
+    say(f"  adjoint {grad}\n  oracle fd (eps 1e-6, 1e-7, 1e-8)\n{fds}\n  max rel err vs the median {err:.2e}")
+    assert np.isfinite(grad).all() and np.abs(grad).max() > 0 and err <= 1e-4
