@@ -349,7 +349,11 @@ def run_ensemble(a, cases, dev, sim_seconds, baselines):
     Ensemble([warm], single_pass=not a.ens_two_pass).step(2)
     del warm
 
-    ens = Ensemble([_ensemble_solver(c, dev) for c in cases], t_stop=sim_seconds, single_pass=not a.ens_two_pass)
+    from hipims_amd import autotune
+    members = [_ensemble_solver(c, dev) for c in cases]
+    ens_rows = autotune.tune_ensemble(members, single_pass=not a.ens_two_pass)
+    ens_tuning = dict(members[0].tune_timings)
+    ens = Ensemble(members, t_stop=sim_seconds, single_pass=not a.ens_two_pass)
     ens.capture(a.replay_steps)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -381,6 +385,7 @@ def run_ensemble(a, cases, dev, sim_seconds, baselines):
             seq[name] = sequential(cls, cases, sim_seconds, dev, name)
     return {"scenarios": len(cases), "sizes": [c.mask.shape[0] - 2 for c in cases],
             "sim_seconds": sim_seconds, "dtype": "f32", "order": 2, "steps_per_replay": a.replay_steps,
+            "tile_rows": ens_rows, "tile_rows_tuning_ms_per_step": {str(k): v for k, v in ens_tuning.items()},
             "step": "two-pass" if a.ens_two_pass else "single-pass",
             "cell_updates": work, "ensemble_s": ens_s, "ensemble_cell_updates_per_s": work / ens_s,
             "steps_min": min(steps), "steps_max": max(steps),
@@ -390,16 +395,18 @@ def run_ensemble(a, cases, dev, sim_seconds, baselines):
 
 def standalone_graphs(cases, t_stop, steps, dev):
     """Each scenario alone: its single-pass steps captured `steps` to a HIP
-    graph (captured before the clock starts), the graph replayed until the
-    scenario reaches t_stop, one host read of t per replay."""
+    graph (captured before the clock starts) at the unit height tuned for
+    its size, the graph replayed until the scenario reaches t_stop, one host
+    read of t per replay."""
     import torch
 
-    from hipims_amd import _lib
+    from hipims_amd import _lib, autotune
     total = 0.0
     counts = []
     last = time.perf_counter()
     for k, c in enumerate(cases):
         s = _ensemble_solver(c, dev)
+        autotune.tune(s, path="fused")  # memoised per size class
         s.set_t_stop(t_stop)
         s.ensure_out()
         s.desc  # workspace allocated outside the capture

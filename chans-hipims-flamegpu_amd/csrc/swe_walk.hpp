@@ -375,7 +375,7 @@ __device__ inline void walk_row(const WalkEnv& env, DP a, const WRow& cur, const
 template <int ORDER, int R, bool FUSED, class DP>
 __device__ __forceinline__ void walk_unit(DP a, int bx, int by, const Partials& part, int64_t pidx, bool save_flux,
                                  bool time_control, bool force_copy) {
-  static_assert(R % 4 == 0, "rows per unit: a multiple of the four-row window");
+  static_assert(R == 2 || R % 4 == 0, "rows per unit: 2 or a multiple of the four-row window");
   __shared__ float s_rate[FUSED ? kMaxStations : 1];
   __shared__ double s_sums[FUSED ? 3 * kWalkLanes : 1];
   const int lane = (int)threadIdx.x;
@@ -439,6 +439,7 @@ __device__ __forceinline__ void walk_unit(DP a, int bx, int by, const Partials& 
                            time_control);
     walk_row<ORDER, FUSED>(env, a, D, A, B, C, k, gx, y0 + r + 1, own, need_x, col_in, s_rate, save_flux,
                            time_control);
+    if (R == 2) break;
     walk_row<ORDER, FUSED>(env, a, A, B, C, D, k, gx, y0 + r + 2, own, need_x, col_in, s_rate, save_flux,
                            time_control);
     walk_row<ORDER, FUSED>(env, a, B, C, D, A, k, gx, y0 + r + 3, own, need_x, col_in, s_rate, save_flux,
@@ -466,11 +467,8 @@ __device__ __forceinline__ void walk_unit(DP a, int bx, int by, const Partials& 
 // flags: 1 also save the step's flux in dh, dqx, dqy; 2 no time-control
 // outputs (a frozen schedule); 4 a stopped domain copies its state through
 // even when marked in step.
-#ifndef HP_WALK_ATTR
-#define HP_WALK_ATTR
-#endif
 template <int ORDER, int R, bool FUSED>
-__global__ __launch_bounds__(kWalkLanes) HP_WALK_ATTR void k_walk(hp_grid_desc a, int nstrips, int flags) {
+__global__ __launch_bounds__(kWalkLanes) void k_walk(hp_grid_desc a, int nstrips, int flags) {
   const KDesc pa = kernarg_desc();
   const int unit = xcd_band(blockIdx.x, gridDim.x);
   walk_unit<ORDER, R, FUSED>(pa, unit % nstrips, unit / nstrips, Partials(a.work, gridDim.x), unit, (flags & 1) != 0,

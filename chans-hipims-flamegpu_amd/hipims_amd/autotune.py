@@ -1,7 +1,7 @@
 """Flux unit-height autotuner.
 
 Float domains run the flux pass and the single-pass step as column walks:
-one wave per unit of 60 columns x R rows, R in {4, 8, 12, 16}.  A taller
+one wave per unit of 60 columns x R rows, R in {2, 4, 8, 12, 16}.  A taller
 unit re-loads fewer halo rows and solves fewer duplicate y-edges (four
 prologue rows and one edge per unit), a shorter one gives more waves, which
 matters when the last round of waves on the 256 CUs would run part-empty.
@@ -19,7 +19,7 @@ import torch
 
 from .grid import GridSolver
 
-CHOICES = {torch.float32: (4, 8, 12, 16), torch.float64: (4, 8)}
+CHOICES = {torch.float32: (2, 4, 8, 12, 16), torch.float64: (4, 8)}
 _cache: dict = {}
 
 
@@ -67,4 +67,41 @@ def tune(s: GridSolver, reps: int = 20, apply: bool = True, path: str = "flux") 
         s.tune_timings = timings
     if apply:
         s.set_tile_rows(best)
+    return best
+
+
+def tune_ensemble(solvers: list[GridSolver], single_pass: bool = True, steps: int = 10) -> int:
+    """Best unit height for one ensemble launch over `solvers` (all must share
+    dtype and order): each legal height is timed over `steps` ensemble steps
+    (after two untimed ones) and every domain's state is restored
+    afterwards.  Sets the winner on every solver and returns it."""
+    from .ensemble import Ensemble
+
+    if not solvers:
+        raise ValueError("no domains")
+    dt = solvers[0].dtype
+    steps += steps & 1  # the single-pass ensemble steps in pairs
+    saved = [(s.U.clone(), s.h_max.clone(), s.ctrl.clone()) for s in solvers]
+    timings = {}
+    for ty in CHOICES[dt]:
+        for s in solvers:
+            s.set_tile_rows(ty)
+        e = Ensemble(solvers, single_pass=single_pass)
+        e.step(2)
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        e.step(steps)
+        b.record()
+        torch.cuda.synchronize(solvers[0].device)
+        timings[ty] = a.elapsed_time(b) / steps
+        del e
+        for s, (u, hm, c) in zip(solvers, saved):
+            s.U.copy_(u)
+            s.h_max.copy_(hm)
+            s.ctrl.copy_(c)
+            s.state_written()
+    best = min(timings, key=timings.get)
+    for s in solvers:
+        s.set_tile_rows(best)
+    solvers[0].tune_timings = timings
     return best
