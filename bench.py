@@ -690,7 +690,7 @@ def main():
     per_kernel = {names.get(k, k): v for k, v in per_kernel.items()}
     valu = profiled_valu(kname, workload_tag(a))
     vpeak, vsrc = valu_peak()
-    valu_frac = None if valu is None else valu * cells / (per_kernel[dom] / 1e3) / vpeak
+    valu_frac = None if valu is None else valu * cells / (per_kernel[kname] / 1e3) / vpeak
     t_wall = best["wall_s"]
 
     cpu = None

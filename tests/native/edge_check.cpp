@@ -45,6 +45,55 @@ int ec_edge_vjp(int order, const double* cells, double dx, double dt, const doub
   return act ? 1 : 0;
 }
 
+// Interior order-2 edge whose faces take their limited slopes (and the
+// dry-cell momentum rule) from `base` (4 cells like `cells`) instead of from
+// the cells themselves: the model the frozen-slope reverse differentiates.
+int ec_edge_frozen(const double* cells, const double* base, double dx, double dt, double* out) {
+  const FrameCell<double> c = cell(cells + 4), r = cell(cells + 8);
+  const FrameCell<double> bl = cell(base), bc = cell(base + 4), br = cell(base + 8), brr = cell(base + 12);
+  using P = Prec<double>;
+  const double g = kG;
+  EdgeOut<double> o;
+  o.fh = o.fn = o.ft = o.bed_c = o.bed_r = 0.0;
+  o.active = !(lt_hs(c.h) && lt_hs(r.h));
+  if (!o.active) {
+    out[0] = out[1] = out[2] = out[3] = out[4] = 0.0;
+    return 0;
+  }
+  FrameCell<double> eL, eR;
+  eL.h = c.h + 0.5 * P::limit(bc.h - bl.h, br.h - bc.h);
+  eL.z = c.z + 0.5 * P::limit(bc.z - bl.z, br.z - bc.z);
+  eR.h = r.h + 0.5 * P::limit(br.h - brr.h, bc.h - br.h);
+  eR.z = r.z + 0.5 * P::limit(br.z - brr.z, bc.z - br.z);
+  eL.qn = lt_hs(bc.h) ? 0.0 : c.qn + 0.5 * P::limit(bc.qn - bl.qn, br.qn - bc.qn);
+  eL.qt = lt_hs(bc.h) ? 0.0 : c.qt + 0.5 * P::limit(bc.qt - bl.qt, br.qt - bc.qt);
+  eR.qn = lt_hs(br.h) ? 0.0 : r.qn + 0.5 * P::limit(br.qn - brr.qn, bc.qn - br.qn);
+  eR.qt = lt_hs(br.h) ? 0.0 : r.qt + 0.5 * P::limit(br.qt - brr.qt, bc.qt - br.qt);
+  double fh, fn, ft;
+  hllc_edge<double, 2>(eL, eR, c.h + c.z, r.h + r.z, fh, fn, ft);
+  out[0] = P::scale(fh, dx, dt, dt / dx);
+  out[1] = P::scale(fn, dx, dt, dt / dx);
+  out[2] = P::scale(ft, dx, dt, dt / dx);
+  out[3] = P::scale(0.5 * g * (eL.h + c.h) * (eL.z - c.z), dx, dt, dt / dx);
+  out[4] = P::scale(0.5 * g * (eR.h + r.h) * (eR.z - r.z), dx, dt, dt / dx);
+  return 1;
+}
+
+// The frozen-slope VJP at `cells` (grad[12] as ec_edge_vjp; l and rr get none).
+int ec_edge_vjp_frozen(const double* cells, double dx, double dt, const double* v, double* grad) {
+  const FrameCell<double> l = cell(cells), c = cell(cells + 4), r = cell(cells + 8), rr = cell(cells + 12);
+  FrameAdj<double> a[4] = {zero_adj<double>(), zero_adj<double>(), zero_adj<double>(), zero_adj<double>()};
+  double aec = 0.0, aer = 0.0;
+  const bool act = solve_edge_vjp<double, 2>(c, l, r, rr, false, c.h + c.z, r.h + r.z, dx, dt, v[0], v[1], v[2],
+                                             v[3], v[4], a[1], a[0], a[2], a[3], aec, aer, true);
+  a[1].h += aec;
+  a[2].h += aer;
+  for (int k = 0; k < 4; ++k) {
+    grad[3 * k] = a[k].h; grad[3 * k + 1] = a[k].qn; grad[3 * k + 2] = a[k].qt;
+  }
+  return act ? 1 : 0;
+}
+
 // Boundary edge of cell c (h, qn, qt, z) in direction d with ghost `type`.
 int ec_ghost_edge(int order, int type, int d, const double* c4, const double* given, double dx, double dt,
                   double* out) {

@@ -27,6 +27,8 @@ def ec():
                                 _P]
     L.ec_ghost_edge_vjp.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P, ctypes.c_double,
                                     ctypes.c_double, _P, _P]
+    L.ec_edge_frozen.argtypes = [_P, _P, ctypes.c_double, ctypes.c_double, _P]
+    L.ec_edge_vjp_frozen.argtypes = [_P, ctypes.c_double, ctypes.c_double, _P, _P]
     L.ec_friction.argtypes = [ctypes.c_double] * 5 + [_P]
     L.ec_friction_vjp.argtypes = [ctypes.c_double] * 5 + [_P, _P]
     return L
@@ -71,6 +73,34 @@ def random_cells(rng, kind):
             h = rng.uniform(0.0, 0.02)
         cells[4 * k: 4 * k + 4] = [h, rng.uniform(-1.5, 1.5) * h, rng.uniform(-1.0, 1.0) * h, rng.uniform(-0.4, 0.4)]
     return cells
+
+
+@pytest.mark.parametrize("kind", ["wet", "mixed", "thin"])
+def test_frozen_slope_edge_vjp_matches_finite_differences(ec, kind):
+    """The order-2 reverse with frozen slopes against central differences of
+    the edge whose faces take their limited slopes (and dry-cell momentum
+    rule) from a fixed base state: only the two cells of the edge receive
+    cotangents, and those are the derivative of that edge."""
+    rng = np.random.Generator(np.random.PCG64(300 + len(kind)))
+    dx, dt = 5.0, 0.7
+    total_ok = total_skip = 0
+    for _ in range(200):
+        x = random_cells(rng, kind)
+        base = x.copy()
+        v = rng.standard_normal(5)
+
+        def f(xx):
+            out = np.zeros(5)
+            ec.ec_edge_frozen(P(np.ascontiguousarray(xx)), P(base), dx, dt, P(out))
+            return out
+        grad = np.zeros(12)
+        ec.ec_edge_vjp_frozen(P(x), dx, dt, P(v), P(grad))
+        assert np.all(grad[[0, 1, 2, 9, 10, 11]] == 0.0)  # l and rr
+        ok, sk = fd_check(f, x, [4, 5, 6, 8, 9, 10], v, grad[3:9], depth_idx=(4, 8))
+        total_ok += ok
+        total_skip += sk
+    # thin films put more stencils across a branch switch (skipped)
+    assert total_ok > 0.5 * (total_ok + total_skip), (total_ok, total_skip)
 
 
 @pytest.mark.parametrize("order", [1, 2])

@@ -204,7 +204,9 @@ def test_overlapped_recompute_matches_serial(cuda):
 
 @pytest.mark.xfail(reason="open: the frozen-slope adjoint removes the slope terms of the exact order-2 adjoint "
                           "only to within 1-2 % of their size (gradient errors 1e-3 to 1e-2 against central "
-                          "differences of the frozen-slope model from 3 steps on), cause not found",
+                          "differences of the frozen-slope model from 3 steps on); its edge reverse matches the "
+                          "frozen edge exactly (CPU test), so the residual enters where the steps are assembled; "
+                          "cause not found",
                    strict=False)
 def test_order2_frozen_slope_adjoint_matches_fd_of_frozen_slope_forward(oracle_lib, cuda):
     """Order 2, 200 steps in segments of 50, FP64: the frozen-slope adjoint

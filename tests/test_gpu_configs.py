@@ -1,6 +1,6 @@
 """The configured workloads, each run once on the GPU at its configured size.
 
-config1  flat 256 x 256 dam break, dx 1 m, FP64, order 2, closed (slip) walls,
+config1  flat 256 x 256 dam break, dx 1 m, FP64, order 2, rigid walls (code 30),
          1000 steps run free on the HIP path and on the oracle, compared at
          four checkpoints (h, qx, qy, h_max, t, dt within 1e-7 relative run
          free, 1e-10 for one step from the oracle's state).
@@ -77,7 +77,7 @@ def test_config1_dam_break_256_fp64_order2_1000_steps_free_running(oracle_lib, c
     1e-7; at each checkpoint one step from the oracle's state is also taken
     in lockstep, which must agree to 1e-10."""
     say(f"[config1: dam break 256^2 FP64 order 2, 1000 steps free running, {path}]")
-    c = gen.dam_break(256, wall=40)
+    c = gen.dam_break(256, wall=30)
     g = solver(c, torch.float64)
     o = oracle_of(oracle_lib, c)
     lock = solver(c, torch.float64)
