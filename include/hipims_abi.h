@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define HP_ABI_VERSION 9
+#define HP_ABI_VERSION 10
 
 enum hp_status {
   HP_OK = 0,
