@@ -218,8 +218,13 @@ def parse():
     p.add_argument("--sim-seconds", type=float, default=900.0, help="ensemble: simulated time of every scenario")
     p.add_argument("--replay-steps", type=int, default=50, help="ensemble: steps per graph replay (even)")
     p.add_argument("--ens-two-pass", action="store_true", help="ensemble: flux and update passes per step")
+    p.add_argument("--ens-retune-below", type=float, default=0.25,
+                   help="ensemble: time the unit height again once the active scenarios hold less than this "
+                        "fraction of the cells it was chosen for (0: never)")
     p.add_argument("--no-baselines", action="store_true",
-                   help="ensemble: skip the sequential baseline runs (for long simulated times)")
+                   help="ensemble: of the sequential baseline runs keep only the standalone graphs (for long "
+                        "simulated times)")
+    p.add_argument("--ensemble-only", action="store_true", help="ensemble: no baseline runs at all")
     p.add_argument("--ensemble-seconds", type=float, default=900.0,
                    help="step mode: simulated time of every scenario in the ensemble run reported beside the "
                         "step (0: none; the configured 2 h takes minutes: --mode ensemble --sim-seconds 7200)")
@@ -366,9 +371,10 @@ def run_ensemble(a, cases, dev, sim_seconds, baselines):
             print(f"[ensemble] {k} replays, {n_act} scenarios active, {now - t0:.1f} s", file=sys.stderr,
                   flush=True)
 
-    replays = ens.run(a.replay_steps, progress=progress)
+    replays = ens.run(a.replay_steps, progress=progress, retune_below=a.ens_retune_below)
     torch.cuda.synchronize()
     ens_s = time.perf_counter() - t0
+    retunes = list(ens.retunes)
     steps = [int(s.ctrl_dict()["steps"]) for s in ens.solvers]
     cells = [s.layout.n_valid for s in ens.solvers]
     work = float(sum(n * k for n, k in zip(cells, steps)))
@@ -386,6 +392,8 @@ def run_ensemble(a, cases, dev, sim_seconds, baselines):
     return {"scenarios": len(cases), "sizes": [c.mask.shape[0] - 2 for c in cases],
             "sim_seconds": sim_seconds, "dtype": "f32", "order": 2, "steps_per_replay": a.replay_steps,
             "tile_rows": ens_rows, "tile_rows_tuning_ms_per_step": {str(k): v for k, v in ens_tuning.items()},
+            "retune_below": a.ens_retune_below,
+            "retunes": [{"scenarios": n, "cells": c, "tile_rows": r, "seconds": t} for n, c, r, t in retunes],
             "step": "two-pass" if a.ens_two_pass else "single-pass",
             "cell_updates": work, "ensemble_s": ens_s, "ensemble_cell_updates_per_s": work / ens_s,
             "steps_min": min(steps), "steps_max": max(steps),
@@ -435,6 +443,8 @@ def ensemble_mode(a):
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     bl = ("standalone_graphs",) if a.no_baselines else ("standalone_graphs", "godunov", "compact")
+    if a.ensemble_only:
+        bl = ()
     line = {"mode": "ensemble", **run_ensemble(a, cases, dev, a.sim_seconds, bl), "generation_s": gen_s}
     print(json.dumps(line), flush=True)
 
@@ -680,17 +690,8 @@ def main():
     best = fused if fused["total_ms"] < two["total_ms"] else two
     ms_step = best["total_ms"] / best["steps"]
     value = cells * best["steps"] / (best["total_ms"] / 1e3)
-    per_kernel = dict(best["kernels_ms"])
-    dom = max((k for k in per_kernel if k in B), key=per_kernel.get)
-    assert per_kernel[dom] <= ms_step, "the dominant kernel's time exceeds the step it belongs to"
-    achieved = B[dom] * cells / (per_kernel[dom] / 1e3) / 1e9
+    per_kernel, roofline = roofline_fields(best["kernels_ms"], ms_step, esize, field, cells, workload_tag(a))
     two_pass = (B["k_flux"] + B["k_update"]) * value / 1e9
-    names = kernel_names(esize)
-    kname = names[dom]
-    per_kernel = {names.get(k, k): v for k, v in per_kernel.items()}
-    valu = profiled_valu(kname, workload_tag(a))
-    vpeak, vsrc = valu_peak()
-    valu_frac = None if valu is None else valu * cells / (per_kernel[kname] / 1e3) / vpeak
     t_wall = best["wall_s"]
 
     cpu = None
@@ -718,19 +719,7 @@ def main():
                    "tile_rows_tuning_ms": {p: {str(k): v for k, v in t.items()} for p, t in tuned.items()},
                    "mass_sums": a.mass_sums,
                    "sim_time_s": ctrl["t"]},
-        # priced on the HBM roofline (no MFMA work on this path); `limiter`
-        # names the resource the kernel is closer to saturating: the HBM
-        # fraction above or the VALU issue fraction from the counted
-        # instructions per cell
-        "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": profiled_traffic(kname, workload_tag(a)),
-                     "alg_bytes_per_cell": B[dom], "kernel_ms": per_kernel[kname],
-                     "timed_with": "HIP events around the kernel of every tenth timed step",
-                     "valu_lane_instr_per_cell": valu,
-                     "valu_issue_frac": valu_frac,
-                     "limiter": "valu" if valu_frac is not None and valu_frac > achieved / HBM_PEAK_GBS else "hbm",
-                     "valu_peak_lane_instr_per_s": vpeak, "valu_peak_source": vsrc},
+        "roofline": roofline,
         "kernels_ms": per_kernel,
         "path": best["path"],
         "paths_ms_per_step": {"two_pass": two["total_ms"] / two["steps"], "fused": fused["total_ms"] / fused["steps"]},
@@ -745,6 +734,33 @@ def main():
     if not a.no_adjoint:
         line["adjoint"] = run_adjoint(a, fp64=False)
     print(json.dumps(line), flush=True)
+
+
+def roofline_fields(kernels_ms: dict, ms_step: float, esize: int, rain_field: bool, cells: int, tag: str):
+    """(kernel times keyed by kernel name, the line's roofline object) for the
+    timed path's kernels (keys as time_two_pass / time_fused give them).  The
+    object is priced on the HBM roofline (no MFMA work on this path);
+    `limiter` names the resource the dominant kernel is closer to
+    saturating: its HBM fraction or its VALU issue fraction (the counted
+    instructions per cell from profiles/ against the measured issue rate)."""
+    B = alg_bytes(esize, rain_field=rain_field)
+    dom = max((k for k in kernels_ms if k in B), key=kernels_ms.get)
+    assert kernels_ms[dom] <= ms_step, "the dominant kernel's time exceeds the step it belongs to"
+    achieved = B[dom] * cells / (kernels_ms[dom] / 1e3) / 1e9
+    names = kernel_names(esize)
+    kname = names[dom]
+    per_kernel = {names.get(k, k): v for k, v in kernels_ms.items()}
+    valu = profiled_valu(kname, tag)
+    vpeak, vsrc = valu_peak()
+    valu_frac = None if valu is None else valu * cells / (per_kernel[kname] / 1e3) / vpeak
+    return per_kernel, {
+        "bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(kname, tag),
+        "alg_bytes_per_cell": B[dom], "kernel_ms": per_kernel[kname],
+        "timed_with": "HIP events around the kernel of every tenth timed step",
+        "valu_lane_instr_per_cell": valu, "valu_issue_frac": valu_frac,
+        "limiter": "valu" if valu_frac is not None and valu_frac > achieved / HBM_PEAK_GBS else "hbm",
+        "valu_peak_lane_instr_per_s": vpeak, "valu_peak_source": vsrc}
 
 
 def physical_cores():

@@ -85,6 +85,7 @@ struct WalkEnv {
   float dx, dt;
   float cfl_dx;
   bool t_zero;  // t == 0: every valid cell is listed
+  int n_rates;  // station rates in LDS (the table's stations at this t)
 };
 
 template <class DP> __device__ inline BcValues<float> walk_bc_values(DP a, int slot, float t) {
@@ -336,7 +337,7 @@ __device__ inline void walk_row(const WalkEnv& env, DP a, const WRow& cur, const
     }
     if (FUSED) {
       CellIn<float> in = in_row;
-      if (a->rain_mode == HP_RAIN_STATION) in.rain = (in.st >= 0 && in.st < kMaxStations) ? s_rate[in.st] : 0.0f;
+      if (a->rain_mode == HP_RAIN_STATION) in.rain = (in.st >= 0 && in.st < env.n_rates) ? s_rate[in.st] : 0.0f;
       float hn = cur.h, qxn = cur.qx, qyn = cur.qy, hm = in.hmax;
       float dF = 0.0f;
       if (a->infil_ks && listed) {  // infiltration over the flux list (SWE_CUDA.py:439-443)
@@ -409,10 +410,15 @@ __device__ __forceinline__ void walk_unit(DP a, int bx, int by, const Partials& 
   env.dt_dx = env.dt / env.dx;
   env.cfl_dx = float(a->cfl) * float(a->dx);
   env.t_zero = float(a->ctrl->t) == 0.0f;
+  env.n_rates = 0;
   if (FUSED && a->rain_mode == HP_RAIN_STATION) {
-    // every station's rate at this step's t, in LDS
+    // the table's station rates at this step's t, in LDS (as many as the
+    // table has stations, at most 1024, which the host enforces: a cell
+    // naming another station, or any cell once the table has ended, gets
+    // no rain, as station_rate gives)
     const RainRows rr = rain_rows_at(a, a->ctrl->t);
-    for (int st = lane; st < kMaxStations; st += kWalkLanes) s_rate[st] = station_rate<float>(rr, st, a->rain_scale);
+    env.n_rates = rr.zero ? 0 : min(rr.n_st, kMaxStations);
+    for (int st = lane; st < env.n_rates; st += kWalkLanes) s_rate[st] = station_rate<float>(rr, st, a->rain_scale);
     __syncthreads();
   }
 

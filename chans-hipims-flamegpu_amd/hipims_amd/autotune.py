@@ -70,11 +70,13 @@ def tune(s: GridSolver, reps: int = 20, apply: bool = True, path: str = "flux") 
     return best
 
 
-def tune_ensemble(solvers: list[GridSolver], single_pass: bool = True, steps: int = 10) -> int:
+def tune_ensemble(solvers: list[GridSolver], single_pass: bool = True, steps: int = 10,
+                  max_rows: int | None = None) -> int:
     """Best unit height for one ensemble launch over `solvers` (all must share
-    dtype and order): each legal height is timed over `steps` ensemble steps
-    (after two untimed ones) and every domain's state is restored
-    afterwards.  Sets the winner on every solver and returns it."""
+    dtype and order): each legal height (up to `max_rows` when given) is
+    timed over `steps` ensemble steps (after two untimed ones) and every
+    domain's state is restored afterwards.  Sets the winner on every solver
+    and returns it."""
     from .ensemble import Ensemble
 
     if not solvers:
@@ -84,6 +86,8 @@ def tune_ensemble(solvers: list[GridSolver], single_pass: bool = True, steps: in
     saved = [(s.U.clone(), s.h_max.clone(), s.ctrl.clone()) for s in solvers]
     timings = {}
     for ty in CHOICES[dt]:
+        if max_rows is not None and ty > max_rows and ty != min(CHOICES[dt]):
+            continue
         for s in solvers:
             s.set_tile_rows(ty)
         e = Ensemble(solvers, single_pass=single_pass)

@@ -15,10 +15,11 @@ with one host check per replay.
 from __future__ import annotations
 
 import ctypes
+import time
 
 import torch
 
-from . import _lib
+from . import _lib, autotune
 from ._lib import HpEnsemble, HpGridDesc, check, lib
 from .grid import GridSolver
 
@@ -92,15 +93,24 @@ class Ensemble:
         return int(self.active_mask().sum())
 
     def run(self, steps_per_replay: int = 50, max_replays: int = 10_000_000, compact: float = 0.75,
-            progress=None) -> int:
+            progress=None, retune_below: float = 0.25) -> int:
         """Replay the captured graph until every scenario has stopped; returns
         the number of replays.  Stopped scenarios cost a launched-and-exiting
         block per tile, so once the active count falls to `compact` of the
         scenarios in the launch maps, the maps are rebuilt over the active
-        ones and the graph is captured again.  `progress(replays, active)` is
-        called after each host check."""
+        ones and the graph is captured again.  At such a rebuild, once the
+        active scenarios hold less than `retune_below` of the cells the unit
+        height was last chosen for, the height is timed again over them
+        (autotune.tune_ensemble, heights up to the current one): the height
+        that fills the GPU with many domains leaves it idle with a few small
+        ones.  The states do not depend on the height, only the FP64 mass
+        sums' rounding does; 0 keeps the height.  `self.retunes` records
+        (scenarios, cells, height, seconds) of each retuning.
+        `progress(replays, active)` is called after each host check."""
         if self.graph is None or self.graph_steps != steps_per_replay:
             self.capture(steps_per_replay)
+        tuned_cells = sum(self.solvers[m].layout.n_valid for m in self.members)
+        self.retunes = []
         for k in range(max_replays):
             act = self.active_mask()
             n_act = int(act.sum())
@@ -109,7 +119,16 @@ class Ensemble:
             if n_act == 0:
                 return k
             if n_act <= compact * len(self.members):
-                self._build([m for m in self.members if bool(act[m])])
+                keep = [m for m in self.members if bool(act[m])]
+                cells = sum(self.solvers[m].layout.n_valid for m in keep)
+                if cells < retune_below * tuned_cells:
+                    t0 = time.perf_counter()
+                    now = self.solvers[keep[0]].tile_rows
+                    best = autotune.tune_ensemble([self.solvers[m] for m in keep], single_pass=self.single_pass,
+                                                  max_rows=now if now > 0 else None)
+                    self.retunes.append((len(keep), cells, best, time.perf_counter() - t0))
+                    tuned_cells = cells
+                self._build(keep)
                 self.capture(steps_per_replay)
             self.replay()
         raise RuntimeError("ensemble did not finish within max_replays")

@@ -111,6 +111,49 @@ def test_stop_time_freezes_each_domain(cuda, single_pass):
     say(f"  replays {replays}, steps {[s.ctrl_dict()['steps'] for s in ss]}")
 
 
+def test_compaction_with_retuned_unit_height_matches_standalone(cuda):
+    """Staggered stop times make run() rebuild its launch maps (4 -> 3 -> 2
+    or fewer active), choosing the unit height again each time; every domain's
+    state, h_max, t, dt and step count equal a standalone single-pass run
+    to the same stop time, bitwise (the per-cell arithmetic does not depend
+    on the unit height)."""
+    say("[compaction with retuned unit height]")
+    # early steps of the dry start are tens of seconds long; 300 s apart, the
+    # stops fall in different replays of two steps
+    stops = [300.0, 600.0, 900.0, 900.0]
+    ens_s = [scenario(*sp, torch.float32) for sp in SPECS]
+    alone = [scenario(*sp, torch.float32) for sp in SPECS]
+    for s, a, t in zip(ens_s, alone, stops):
+        s.set_t_stop(t)
+        a.set_t_stop(t)
+    for s in ens_s:
+        s.set_tile_rows(16)
+    e = Ensemble(ens_s)
+    heights = []
+
+    def progress(k, n_act):
+        heights.append((n_act, len(e.members), ens_s[-1].tile_rows))
+
+    # retune_below=1: the height is timed again at every rebuild
+    e.run(steps_per_replay=2, progress=progress, retune_below=1.0)
+    torch.cuda.synchronize()
+    # rebuilt at least twice: over 3 domains, then over 2 or fewer
+    assert {4, 3} <= {m for _, m, _ in heights} and len(e.members) <= 2, heights
+    assert len(e.retunes) >= 2, e.retunes
+    for a in alone:
+        while float(a.ctrl[_lib.CTRL_T]) < float(a.ctrl[_lib.CTRL_T_STOP]):
+            a.step_fused(2)
+    torch.cuda.synchronize()
+    keys = ("t", "dt", "steps")
+    for k, (a, b) in enumerate(zip(ens_s, alone)):
+        for x, y in zip(state(a)[:4], state(b)[:4]):
+            assert torch.equal(x, y), f"domain {k} differs"
+        da, db = a.ctrl_dict(), b.ctrl_dict()
+        assert all(da[f] == db[f] for f in keys), (k, da, db)
+        assert da["t"] >= stops[k]
+    say(f"  (active, members, unit height) per check: {sorted(set(heights))}; retunes {e.retunes}")
+
+
 def test_single_pass_ensemble_rejects_odd_step_counts(cuda):
     ss = [scenario(*sp, torch.float32) for sp in SPECS[:2]]
     with pytest.raises(_lib.HipimsError, match="even number of steps"):
