@@ -962,12 +962,12 @@ static int64_t update_blocks(const hp_grid_desc* g) {
   return (total + kChunk - 1) / kChunk;
 }
 
-// Rows per flux work unit: FP32 column-walk units of 60 columns x 2, 4, 8,
-// 12 or 16 rows (default 8); FP64 LDS tiles of 64 x 4 or 64 x 8 cells
+// Rows per flux work unit: FP32 column-walk units of 60 columns x 2, 4, 6,
+// 8, 10, 12 or 16 rows (default 8); FP64 LDS tiles of 64 x 4 or 64 x 8 cells
 // (default 4).  Every launch of a domain's units takes its height from here.
 static int flux_rows(const hp_grid_desc* g) {
   const int r = g->tile_rows;
-  if (g->dtype == HP_F32) return (r == 2 || r == 4 || r == 12 || r == 16) ? r : 8;
+  if (g->dtype == HP_F32) return (r == 2 || r == 4 || r == 6 || r == 10 || r == 12 || r == 16) ? r : 8;
   return r == 8 ? 8 : 4;
 }
 
@@ -1043,6 +1043,8 @@ static int launch_walk_o(const hp_grid_desc* g, hipStream_t st, int flags) {
   switch (flux_rows(g)) {
     case 2: return launch_walk_r<ORDER, 2, FUSED>(g, st, flags);
     case 4: return launch_walk_r<ORDER, 4, FUSED>(g, st, flags);
+    case 6: return launch_walk_r<ORDER, 6, FUSED>(g, st, flags);
+    case 10: return launch_walk_r<ORDER, 10, FUSED>(g, st, flags);
     case 12: return launch_walk_r<ORDER, 12, FUSED>(g, st, flags);
     case 16: return launch_walk_r<ORDER, 16, FUSED>(g, st, flags);
     default: return launch_walk_r<ORDER, 8, FUSED>(g, st, flags);
@@ -1185,6 +1187,8 @@ static void launch_walk_multi_o(const hp_ensemble* e, int parity, hipStream_t st
   switch (e->tile_rows) {
     case 2: return launch_walk_multi<ORDER, 2, FUSED>(e, parity, st);
     case 4: return launch_walk_multi<ORDER, 4, FUSED>(e, parity, st);
+    case 6: return launch_walk_multi<ORDER, 6, FUSED>(e, parity, st);
+    case 10: return launch_walk_multi<ORDER, 10, FUSED>(e, parity, st);
     case 12: return launch_walk_multi<ORDER, 12, FUSED>(e, parity, st);
     case 16: return launch_walk_multi<ORDER, 16, FUSED>(e, parity, st);
     default: return launch_walk_multi<ORDER, 8, FUSED>(e, parity, st);

@@ -376,7 +376,7 @@ __device__ inline void walk_row(const WalkEnv& env, DP a, const WRow& cur, const
 template <int ORDER, int R, bool FUSED, class DP>
 __device__ __forceinline__ void walk_unit(DP a, int bx, int by, const Partials& part, int64_t pidx, bool save_flux,
                                  bool time_control, bool force_copy) {
-  static_assert(R == 2 || R % 4 == 0, "rows per unit: 2 or a multiple of the four-row window");
+  static_assert(R >= 2 && R % 2 == 0, "rows per unit: even (the four-row window's first two rows end a unit)");
   __shared__ float s_rate[FUSED ? kMaxStations : 1];
   __shared__ double s_sums[FUSED ? 3 * kWalkLanes : 1];
   const int lane = (int)threadIdx.x;
@@ -445,7 +445,7 @@ __device__ __forceinline__ void walk_unit(DP a, int bx, int by, const Partials& 
                            time_control);
     walk_row<ORDER, FUSED>(env, a, D, A, B, C, k, gx, y0 + r + 1, own, need_x, col_in, s_rate, save_flux,
                            time_control);
-    if (R == 2) break;
+    if (r + 2 == R) break;  // R = 4k + 2: the unit ends halfway through the window
     walk_row<ORDER, FUSED>(env, a, A, B, C, D, k, gx, y0 + r + 2, own, need_x, col_in, s_rate, save_flux,
                            time_control);
     walk_row<ORDER, FUSED>(env, a, B, C, D, A, k, gx, y0 + r + 3, own, need_x, col_in, s_rate, save_flux,

@@ -1,7 +1,7 @@
 """Flux unit-height autotuner.
 
 Float domains run the flux pass and the single-pass step as column walks:
-one wave per unit of 60 columns x R rows, R in {2, 4, 8, 12, 16}.  A taller
+one wave per unit of 60 columns x R rows, R in {2, 4, 6, 8, 10, 12, 16}.  A taller
 unit re-loads fewer halo rows and solves fewer duplicate y-edges (four
 prologue rows and one edge per unit), a shorter one gives more waves, which
 matters when the last round of waves on the 256 CUs would run part-empty.
@@ -19,7 +19,7 @@ import torch
 
 from .grid import GridSolver
 
-CHOICES = {torch.float32: (2, 4, 8, 12, 16), torch.float64: (4, 8)}
+CHOICES = {torch.float32: (2, 4, 6, 8, 10, 12, 16), torch.float64: (4, 8)}
 _cache: dict = {}
 
 

@@ -145,7 +145,7 @@ typedef struct hp_grid_desc {
   int32_t cell0_x, cell0_y; /* first valid cell in row-major order (-1: none) */
   int32_t n_classes;        /* entries of manning */
   int32_t rain_mode;        /* hp_rain_mode */
-  int32_t tile_rows;        /* flux unit height: float 2, 4, 8, 12 or 16, double 4 or 8 (other: 8 / 4) */
+  int32_t tile_rows;        /* flux unit height: float 2, 4, 6, 8, 10, 12 or 16, double 4 or 8 (other: 8 / 4) */
   int32_t mass_sums;        /* 1: keep hp_ctrl's FP64 volume / rain / boundary-flux sums up to
                                date (diagnostics the reference does not compute); 0: skip them */
   /* state planes (dtype) */
