@@ -30,7 +30,9 @@ frozen, so the discrete order-2 model has no derivative to follow there and
 both adjoints grow without bound.  Long-horizon calibration therefore uses
 order 1 (the reference server's default); order-2 gradients, exact or with
 frozen slopes, hold over short horizons (tested against central
-differences).
+differences, and for frozen slopes against one-sided differences: that
+model has kinks exactly at the run's own state, where the adjoint follows
+the branch the run took).
 """
 from __future__ import annotations
 

@@ -294,10 +294,11 @@ int hp_ensemble_step(const hp_ensemble* e, int32_t nsteps, void* stream);
  * flux_ready 1: dh/dqx/dqy already hold the step's flux (saved when the
  * step was recomputed), so hp_adjoint_step does not recompute it.
  * slope_mode 0 differentiates the minmod-limited MUSCL slopes exactly (the
- * exact derivative of the step; over long horizons its frozen-branch
- * linearisation grows, as that of MUSCL with forward Euler does); 1 holds
- * the limited slopes fixed (the derivative of the first-order flux at the
- * reconstructed states, stable over long horizons, an approximation).
+ * exact derivative of the step); 1 holds the limited slopes at the run's
+ * own values (the derivative of the model whose faces are the cell value
+ * plus those slopes, one-sided where the run sits on a branch point of that
+ * model).  Order-2 runs amplify rounding differences, so in either mode the
+ * derivative holds over a few hundred steps, not over long horizons.
  * gauge_of (may be null): an int32 plane like h holding, at each gauge cell,
  * the gauge's index g (-1 elsewhere; each cell at most one gauge); with it
  * the step first adds the squared-misfit cotangent 2 (loss_h[g] -

@@ -202,45 +202,41 @@ def test_overlapped_recompute_matches_serial(cuda):
     assert all(torch.equal(a, b) for a, b in zip(lam[False], lam[True]))
 
 
-@pytest.mark.xfail(reason="open: the frozen-slope adjoint removes the slope terms of the exact order-2 adjoint "
-                          "only to within 1-2 % of their size (gradient errors 1e-3 to 1e-2 against central "
-                          "differences of the frozen-slope model from 3 steps on); its edge reverse matches the "
-                          "frozen edge exactly (CPU test), so the residual enters where the steps are assembled; "
-                          "cause not found",
-                   strict=False)
-def test_order2_frozen_slope_adjoint_matches_fd_of_frozen_slope_forward(oracle_lib, cuda):
-    """Order 2, 200 steps in segments of 50, FP64: the frozen-slope adjoint
-    (slope terms of the MUSCL faces held at the unperturbed run's) is the
-    exact derivative of the forward model whose faces are the cell value plus
-    the unperturbed run's limited slope.  The oracle runs that model (beside
-    the unperturbed trajectory, on the recorded schedule, against the same
-    observations), and its central differences at eps 1e-6 must match the
-    adjoint to 1e-4 of the largest component.
+def _frozen_case(oracle_lib, K, ck, warm, n_gauges, seed_gauges=1004):
+    """A 96^2 order-2 FP64 calibration with frozen slopes whose reverse sweep
+    runs on the oracle's trajectory (the oracle's states replace the
+    checkpoints, its gauge depths the recorded ones; the HIP path recomputes
+    only the steps of each segment from them), and the oracle's frozen-slope
+    loss as a function of the parameters on the recorded schedule.
 
     Order-2 runs amplify rounding differences (a 1e-15 perturbation reaches
-    1e-2 within a few hundred steps on this case), so the reverse sweep runs
-    on the oracle's trajectory: its states replace the checkpoints and its
-    gauge depths the recorded ones, and the HIP path recomputes only the 50
-    steps of each segment from them.  The horizon is 200 steps because
-    beyond about 300 even the frozen-slope model's central differences
-    depend on eps (its branch switches: HLLC regions, hydrostatic clamps,
-    wet/dry), so no derivative of an order-2 run is defined there to check
-    against."""
-    say("[adjoint order 2, frozen slopes, fp64, 200 steps, ck 50, vs FD of the frozen-slope forward]")
-    K, ck = 200, 50
+    1e-2 within a few hundred steps on this case), hence the oracle's
+    trajectory.  `warm` > 0 starts from the oracle's state after that many
+    steps (a wet state) instead of the dry start."""
     c = gen.rainfall_runoff(96, 5.0, seed=4242, rain_mmh=150.0, relief=8.0)
+    first_dt = 0.5
+    if warm:
+        o = oracle_lib.OracleSim(c.mask, c.h, c.qx, c.qy, c.z, c.dx, cfl=0.5, order=2, first_dt=0.5,
+                                 manning=c.manning, landuse=c.landuse, rain_table=c.rain_table,
+                                 station=np.zeros_like(c.mask))
+        o.run(warm)
+        c.h, c.qx, c.qy = o.raster("h_c", 0.0), o.raster("qx_c", 0.0), o.raster("qy_c", 0.0)
+        first_dt = o.dt
 
     def solver():
-        s = GridSolver(c.mask, c.h, c.qx, c.qy, c.z, c.dx, cfl=0.5, order=2, dtype=torch.float64, first_dt=0.5,
+        s = GridSolver(c.mask, c.h, c.qx, c.qy, c.z, c.dx, cfl=0.5, order=2, dtype=torch.float64, first_dt=first_dt,
                        manning=c.manning, landuse=c.landuse)
         s.set_station_rain(c.rain_table)
         return s
-    pilot = solver()
-    pilot.step(K)
-    h = pilot.raster(pilot.h)
-    rng = np.random.Generator(np.random.PCG64(1004))
+    if warm:
+        h = c.h
+    else:
+        pilot = solver()
+        pilot.step(K)
+        h = pilot.raster(pilot.h)
+    rng = np.random.Generator(np.random.PCG64(seed_gauges))
     wet = np.argwhere((h > 1e-3) & (c.mask > 0))
-    gauges = wet[rng.choice(len(wet), size=min(16, len(wet)), replace=False)]
+    gauges = wet[rng.choice(len(wet), size=min(n_gauges, len(wet)), replace=False)]
     cal = Calibration(solver(), gauges, K, ck, slopes="frozen")
     th0 = cal.theta0()
     pert = torch.tensor(np.random.Generator(np.random.PCG64(7)).choice([0.8, 1.2], size=th0.numel() - 1))
@@ -248,54 +244,124 @@ def test_order2_frozen_slope_adjoint_matches_fd_of_frozen_slope_forward(oracle_l
     th_true[:-1] *= pert
     cal.obs = cal.simulate(th_true)
     cal.run(th0)
-
     sched = cal.sched[:K].cpu().numpy()
     obs = cal.obs[1:].cpu().numpy()
     compact_of = np.cumsum(c.mask > 0).reshape(c.mask.shape) - 1
-    gauges_c = compact_of[gauges[:, 0], gauges[:, 1]].astype(np.int32)
     C = th0.numel() - 1
 
     def sim(theta):
         th = np.asarray(theta, dtype=np.float64)
         rt = c.rain_table.copy()
         rt[:, 1:] *= th[C]
-        return oracle_lib.OracleSim(c.mask, c.h, c.qx, c.qy, c.z, c.dx, cfl=0.5, order=2, first_dt=0.5,
+        return oracle_lib.OracleSim(c.mask, c.h, c.qx, c.qy, c.z, c.dx, cfl=0.5, order=2, first_dt=first_dt,
                                     manning=th[:C], landuse=c.landuse, rain_table=rt,
                                     station=np.zeros_like(c.mask))
 
-    # the oracle's trajectory on the recorded schedule: checkpoint states and
-    # gauge depths, loaded into the calibration in place of its own
-    o = sim(th0.numpy())
-    hist = np.zeros((K, len(gauges_c)))
-    s = cal.s
-    for k in range(K):
-        if k % ck == 0:
-            for v, arr in enumerate((o.h_c, o.qx_c, o.qy_c)):
-                s.load_compact(cal.ckpts[k][v], torch.from_numpy(arr))
-        o.ctrl[0:2] = sched[k]
-        o.run(1)
-        hist[k] = o.h_c[gauges_c]
-    cal.tail = None
-    cal.hist[1:] = torch.from_numpy(hist).to(cal.hist)
-    grad = cal.gradient().numpy()
-    loss = float(((hist - obs) ** 2).sum())
+    def gradient(gsel):
+        """The adjoint gradient of the loss over the gauges `gsel` (indices
+        into `gauges`), and that loss as a function of theta."""
+        gauges_c = compact_of[gauges[gsel, 0], gauges[gsel, 1]].astype(np.int32)
+        cal_g = Calibration(solver(), gauges[gsel], K, ck, slopes="frozen", obs=cal.obs[:, gsel].clone())
+        cal_g.run(th0)
+        o = sim(th0.numpy())
+        hist = np.zeros((K, len(gauges_c)))
+        for k in range(K):
+            if k % ck == 0:
+                for v, arr in enumerate((o.h_c, o.qx_c, o.qy_c)):
+                    cal_g.s.load_compact(cal_g.ckpts[k][v], torch.from_numpy(arr))
+            o.ctrl[0:2] = sched[k]
+            o.run(1)
+            hist[k] = o.h_c[gauges_c]
+        cal_g.tail = None
+        cal_g.hist[1:] = torch.from_numpy(hist).to(cal_g.hist)
+        grad = cal_g.gradient().numpy()
+        ob = obs[:, gsel]
+        loss = float(((hist - ob) ** 2).sum())
 
-    def frozen_loss(theta):
-        return oracle_lib.frozen_slope_loss(sim(th0.numpy()), sim(theta), sched, gauges_c, obs)
+        def frozen_loss(theta):
+            return oracle_lib.frozen_slope_loss(sim(th0.numpy()), sim(theta), sched, gauges_c, ob)
+        assert abs(frozen_loss(th0.numpy()) - loss) <= 1e-12 * loss
+        return grad, frozen_loss
 
-    l_cpu = frozen_loss(th0.numpy())
-    say(f"  loss {loss:.12e} (frozen-slope model at theta0 {l_cpu:.12e})")
-    assert abs(l_cpu - loss) <= 1e-12 * loss
-    fd = np.zeros(th0.numel())
-    for j in range(th0.numel()):
-        d = 1e-6 * abs(float(th0[j]))
-        tp, tm = th0.numpy().copy(), th0.numpy().copy()
+    return th0, gauges, gradient
+
+
+def _one_sided(frozen_loss, th0, eps):
+    """Forward, backward and central differences of frozen_loss at th0."""
+    t0 = th0.numpy()
+    L0 = frozen_loss(t0)
+    fwd, bwd = np.zeros(t0.size), np.zeros(t0.size)
+    for j in range(t0.size):
+        d = eps * abs(float(t0[j]))
+        tp, tm = t0.copy(), t0.copy()
         tp[j] += d
         tm[j] -= d
-        fd[j] = (frozen_loss(tp) - frozen_loss(tm)) / (2 * d)
+        fwd[j] = (frozen_loss(tp) - L0) / d
+        bwd[j] = (L0 - frozen_loss(tm)) / d
+    return fwd, bwd, (fwd + bwd) / 2
+
+
+# The frozen-slope model (each MUSCL face = the cell's value plus the
+# unperturbed run's limited slope, the unperturbed run's dry-cell momentum
+# rule) has kinks exactly at the unperturbed state: where that state sits on
+# a branch point of the step (both faces of an edge meeting the max of the
+# hydrostatic step, a clamp bound met exactly, ...) the exact model keeps
+# the tie under a perturbation (its faces move with their cells' neighbours),
+# the frozen one breaks it.  Its one-sided derivatives then differ, by up to
+# 4 % over 200 steps on the case below, and central differences return their
+# mean; the adjoint follows the branch the unperturbed run took.  So the
+# checks are: where the model is smooth (two steps) the adjoint matches
+# central differences; at a kink (three steps, one gauge) it equals one of
+# the one-sided derivatives; over 200 steps every component lies between
+# its two one-sided differences.
+
+def test_order2_frozen_slope_adjoint_matches_central_differences_where_smooth(oracle_lib, cuda):
+    say("[adjoint order 2, frozen slopes, fp64, 2 steps from a wet state, 16 gauges]")
+    th0, gauges, gradient = _frozen_case(oracle_lib, K=2, ck=10, warm=300, n_gauges=16)
+    grad, frozen_loss = gradient(np.arange(len(gauges)))
+    fwd, bwd, fd = _one_sided(frozen_loss, th0, 1e-7)
+    spread = float(np.abs(fwd - bwd).max() / np.abs(fd).max())  # O(eps) where smooth
     err = float(np.abs(grad - fd).max() / np.abs(grad).max())
-    say(f"  adjoint {grad}\n  fd      {fd}\n  max rel err {err:.2e}")
-    assert np.isfinite(grad).all() and err <= 1e-4
+    say(f"  adjoint {grad}\n  fd      {fd}\n  one-sided spread {spread:.1e}, max rel err {err:.2e}")
+    assert spread < 1e-5 and err <= 1e-6
+
+
+def test_order2_frozen_slope_adjoint_takes_a_side_at_a_kink(oracle_lib, cuda):
+    """Three steps, the one gauge of the 16 whose central differences
+    disagree with the adjoint: the forward and backward differences of the
+    frozen-slope loss differ by 0.4 % at every eps from 1e-5 to 1e-7 (a kink
+    at the unperturbed state), and the adjoint equals one of them."""
+    say("[adjoint order 2, frozen slopes, fp64, 3 steps, kink at gauge (71, 18)]")
+    th0, gauges, gradient = _frozen_case(oracle_lib, K=3, ck=10, warm=300, n_gauges=16)
+    g = [k for k in range(len(gauges)) if tuple(gauges[k]) == (71, 18)]
+    assert g, gauges
+    grad, frozen_loss = gradient(np.array(g))
+    sides = [_one_sided(frozen_loss, th0, e) for e in (1e-5, 1e-7)]
+    for fwd, bwd, _ in sides:
+        assert abs(fwd[0] - bwd[0]) > 1e-3 * abs(fwd[0])
+    fwd, bwd, fd = sides[1]
+    to_side = min(abs(grad[0] - fwd[0]), abs(grad[0] - bwd[0])) / abs(grad[0])
+    say(f"  adjoint {grad[0]:.10e}, forward {fwd[0]:.10e}, backward {bwd[0]:.10e}, central {fd[0]:.10e}; "
+        f"nearest side {to_side:.1e}")
+    assert to_side <= 1e-6
+    assert abs(grad[0] - fd[0]) > 1e-3 * abs(grad[0])
+
+
+def test_order2_frozen_slope_adjoint_lies_between_one_sided_differences(oracle_lib, cuda):
+    """Order 2, 200 steps from the dry start in segments of 50, 16 gauges:
+    every component of the frozen-slope adjoint lies between the forward and
+    the backward difference of the frozen-slope loss (eps 1e-7), within 1e-4
+    of the largest component.  (Beyond about 300 steps even the one-sided
+    differences depend on eps: no derivative is defined to check against.)"""
+    say("[adjoint order 2, frozen slopes, fp64, 200 steps, ck 50, between one-sided differences]")
+    th0, gauges, gradient = _frozen_case(oracle_lib, K=200, ck=50, warm=0, n_gauges=16)
+    grad, frozen_loss = gradient(np.arange(len(gauges)))
+    fwd, bwd, fd = _one_sided(frozen_loss, th0, 1e-7)
+    tol = 1e-4 * np.abs(grad).max()
+    lo, hi = np.minimum(fwd, bwd) - tol, np.maximum(fwd, bwd) + tol
+    say(f"  adjoint  {grad}\n  forward  {fwd}\n  backward {bwd}\n  central  {fd}")
+    assert np.isfinite(grad).all()
+    assert ((grad >= lo) & (grad <= hi)).all(), (grad - lo, hi - grad)
 
 
 def _oracle_loss(o, sched, gauges_c, obs):

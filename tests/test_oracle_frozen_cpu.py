@@ -111,3 +111,48 @@ def test_order2_trajectory_amplifies_rounding_order1_does_not(oracle_lib):
             pert.run(1)
         worst[order] = float(np.abs(pert.h_c - base.h_c).max() / base.h_c.max())
     assert worst[1] < 1e-12 and worst[2] > 1e-4, worst
+
+
+def test_frozen_slope_model_has_a_kink_at_the_base_state(oracle_lib):
+    """From a wet state (300 oracle steps on a 96^2 case), three steps, the
+    loss at one gauge: the forward and backward differences of the
+    frozen-slope loss in the first Manning coefficient differ by 0.4 % at
+    every eps from 1e-5 to 1e-7 (a kink exactly at the unperturbed
+    parameters, where central differences return the mean of the two
+    sides), while over two steps the difference shrinks with eps (smooth)."""
+    c = gen.rainfall_runoff(96, 5.0, seed=4242, rain_mmh=150.0, relief=8.0)
+    o = oracle_lib.OracleSim(c.mask, c.h, c.qx, c.qy, c.z, c.dx, cfl=0.5, order=2, first_dt=0.5,
+                             manning=c.manning, landuse=c.landuse, rain_table=c.rain_table,
+                             station=np.zeros_like(c.mask))
+    o.run(300)
+    h0, qx0, qy0, dt0 = o.raster("h_c", 0.0), o.raster("qx_c", 0.0), o.raster("qy_c", 0.0), o.dt
+    n_classes = len(c.manning)
+    th0 = np.concatenate([c.manning, [1.0]])
+
+    def sim(theta):
+        rt = c.rain_table.copy()
+        rt[:, 1:] *= theta[n_classes]
+        return oracle_lib.OracleSim(c.mask, h0, qx0, qy0, c.z, c.dx, cfl=0.5, order=2, first_dt=dt0,
+                                    manning=theta[:n_classes], landuse=c.landuse, rain_table=rt,
+                                    station=np.zeros_like(c.mask))
+    compact_of = np.cumsum(c.mask > 0).reshape(c.mask.shape) - 1
+    gauge = np.array([compact_of[71, 18]], dtype=np.int32)
+    spreads = {}
+    for K in (2, 3):
+        base = sim(th0)
+        sched = np.zeros((K, 2))
+        for k in range(K):
+            sched[k] = base.ctrl[0:2]
+            base.run(1)
+        obs = np.zeros((K, 1))
+        L0 = oracle_lib.frozen_slope_loss(sim(th0), sim(th0), sched, gauge, obs)
+        for eps in (1e-5, 1e-7):
+            d = eps * th0[0]
+            tp, tm = th0.copy(), th0.copy()
+            tp[0] += d
+            tm[0] -= d
+            fwd = (oracle_lib.frozen_slope_loss(sim(th0), sim(tp), sched, gauge, obs) - L0) / d
+            bwd = (L0 - oracle_lib.frozen_slope_loss(sim(th0), sim(tm), sched, gauge, obs)) / d
+            spreads[K, eps] = abs(fwd - bwd) / abs(fwd)
+    assert spreads[3, 1e-5] > 3e-3 and spreads[3, 1e-7] > 3e-3, spreads
+    assert spreads[2, 1e-7] < 0.1 * spreads[2, 1e-5] and spreads[2, 1e-7] < 1e-5, spreads
