@@ -94,6 +94,40 @@ def test_rainfall_runoff_fp64_lockstep(oracle_lib, cuda, order):
     say(f"  worst rel err {worst:.2e}, t = {o.time:.1f} s, wet cells {(o.h_c > 1e-6).sum()}")
 
 
+@pytest.mark.parametrize("order", [1, 2])
+def test_wetting_front_fp64_lockstep(oracle_lib, cuda, order):
+    """A block of water released onto a dry, rough, sloped bed (open east
+    edge): over 120 steps about 1 500 cells go from dry to wet (counted on
+    the oracle after each step), every step within 1e-10 of the oracle; order
+    2 also produces negative depths, which the reference keeps."""
+    say(f"[wetting front 64^2 order {order}]")
+    n = 64
+    rng = np.random.Generator(np.random.PCG64(31))
+    yy, xx = np.mgrid[0:n, 0:n]
+    z = 0.04 * (n - xx) + 0.05 * rng.standard_normal((n, n)) + 0.3 * np.exp(-((yy - 40) ** 2 + (xx - 30) ** 2) / 60.0)
+    h = np.zeros((n, n))
+    h[8:24, 4:20] = 0.6
+    mask, H, QX, QY, Z = box(n, n, wall=40, east=90, z=z, h=h)
+    c = gen.Case("wetting_front", mask, H, QX, QY, Z, 1.0, np.zeros_like(mask, dtype=np.int64), np.array([0.03]),
+                 0.5, 0.05)
+    g, o = make_pair(oracle_lib, c, order)
+    prev = o.h_c > 1e-6
+    wetting, worst = 0, 0.0
+    for k in range(120):
+        sync_to(g, o)
+        g.step(1)
+        o.run(1)
+        e = compare(g, o, f"front o{order} step {k}")
+        worst = max(worst, max(e.values()))
+        now = o.h_c > 1e-6
+        wetting += int((now & ~prev).sum())
+        prev = now
+    say(f"  worst rel err {worst:.2e}, wetting transitions {wetting}, wet {int(prev.sum())}, min h {o.h_c.min():.2e}")
+    assert wetting >= 1000
+    if order == 2:
+        assert o.h_c.min() < 0.0
+
+
 def bc_case(n=40, seed=9):
     z, h, qx, qy = random_state(n, n + 7, seed=seed, wet=False, rough=0.3)
     mask, H, QX, QY, Z = box(n, n + 7, wall=30, z=z, h=h, qx=qx, qy=qy)
