@@ -19,6 +19,9 @@ FrameCell<double> cell(const double* p) {
 
 extern "C" {
 
+// h^(-1/3) as the FP64 friction forms it (seed + two Newton steps).
+double ec_rcbrt(double h) { return Prec<double>::rcbrt_friction(h); }
+
 // Interior edge: water levels are the cells' h + z.  out: fh, fn, ft, bed_c, bed_r.
 int ec_edge(int order, const double* cells, double dx, double dt, double* out) {
   const FrameCell<double> l = cell(cells), c = cell(cells + 4), r = cell(cells + 8), rr = cell(cells + 12);

@@ -31,7 +31,22 @@ def ec():
     L.ec_edge_vjp_frozen.argtypes = [_P, ctypes.c_double, ctypes.c_double, _P, _P]
     L.ec_friction.argtypes = [ctypes.c_double] * 5 + [_P]
     L.ec_friction_vjp.argtypes = [ctypes.c_double] * 5 + [_P, _P]
+    L.ec_rcbrt.argtypes = [ctypes.c_double]
+    L.ec_rcbrt.restype = ctypes.c_double
     return L
+
+
+def test_fp64_friction_cube_root_is_accurate_to_a_few_ulp(ec):
+    """The FP64 friction's h^(-1/3) (single-precision seed, two Newton
+    steps) against the double pow over every depth a wet cell can hold
+    (1e-6 m .. 1e4 m, log-spaced and random): within 1e-15 relative, and
+    h * y^3 = 1 to the same."""
+    rng = np.random.Generator(np.random.PCG64(17))
+    hs = np.concatenate([np.logspace(-6, 4, 4001), 10.0 ** rng.uniform(-6, 4, 20000)])
+    y = np.array([ec.ec_rcbrt(float(h)) for h in hs])
+    ref = np.power(hs, -1.0 / 3.0)
+    assert np.abs(y / ref - 1.0).max() <= 1e-15
+    assert np.abs(hs * y ** 3 - 1.0).max() <= 2e-15
 
 
 def P(a):

@@ -312,8 +312,9 @@ def _one_sided(frozen_loss, th0, eps):
 # mean; the adjoint follows the branch the unperturbed run took.  So the
 # checks are: where the model is smooth (two steps) the adjoint matches
 # central differences; at a kink (three steps, one gauge) it equals one of
-# the one-sided derivatives; over 200 steps every component lies between
-# its two one-sided differences.
+# the one-sided derivatives; over 200 steps, with several kinks along each
+# direction, it lies within the model's one-sided spread of the central
+# difference.
 
 def test_order2_frozen_slope_adjoint_matches_central_differences_where_smooth(oracle_lib, cuda):
     say("[adjoint order 2, frozen slopes, fp64, 2 steps from a wet state, 16 gauges]")
@@ -347,21 +348,29 @@ def test_order2_frozen_slope_adjoint_takes_a_side_at_a_kink(oracle_lib, cuda):
     assert abs(grad[0] - fd[0]) > 1e-3 * abs(grad[0])
 
 
-def test_order2_frozen_slope_adjoint_lies_between_one_sided_differences(oracle_lib, cuda):
-    """Order 2, 200 steps from the dry start in segments of 50, 16 gauges:
-    every component of the frozen-slope adjoint lies between the forward and
-    the backward difference of the frozen-slope loss (eps 1e-7), within 1e-4
-    of the largest component.  (Beyond about 300 steps even the one-sided
-    differences depend on eps: no derivative is defined to check against.)"""
-    say("[adjoint order 2, frozen slopes, fp64, 200 steps, ck 50, between one-sided differences]")
+def test_order2_frozen_slope_adjoint_within_the_kink_spread_over_200_steps(oracle_lib, cuda):
+    """Order 2, 200 steps from the dry start in segments of 50, 16 gauges.
+    Over this horizon the frozen-slope model crosses several kinks along
+    each parameter direction: the adjoint takes one side at each, which in
+    general equals neither the forward nor the backward difference (each
+    takes the same side at all of them), so no difference defines the
+    derivative to check against.  The bound: every component lies within
+    the model's largest one-sided spread (forward minus backward, eps 1e-7)
+    of the central difference, plus 1e-4 of the largest component - about
+    1 % of the largest component on this case, so an adjoint that is off by
+    more than that anywhere fails.  (Beyond about 300 steps even the
+    one-sided differences depend on eps.)"""
+    say("[adjoint order 2, frozen slopes, fp64, 200 steps, ck 50, within the kink spread]")
     th0, gauges, gradient = _frozen_case(oracle_lib, K=200, ck=50, warm=0, n_gauges=16)
     grad, frozen_loss = gradient(np.arange(len(gauges)))
     fwd, bwd, fd = _one_sided(frozen_loss, th0, 1e-7)
-    tol = 1e-4 * np.abs(grad).max()
-    lo, hi = np.minimum(fwd, bwd) - tol, np.maximum(fwd, bwd) + tol
-    say(f"  adjoint  {grad}\n  forward  {fwd}\n  backward {bwd}\n  central  {fd}")
+    spread = float(np.abs(fwd - bwd).max())
+    bound = spread + 1e-4 * np.abs(grad).max()
+    say(f"  adjoint  {grad}\n  forward  {fwd}\n  backward {bwd}\n  central  {fd}\n"
+        f"  largest spread {spread:.2e} ({spread / np.abs(grad).max():.1e} of the largest component)")
     assert np.isfinite(grad).all()
-    assert ((grad >= lo) & (grad <= hi)).all(), (grad - lo, hi - grad)
+    assert spread <= 0.02 * np.abs(grad).max()
+    assert (np.abs(grad - fd) <= bound).all(), (np.abs(grad - fd), bound)
 
 
 def _oracle_loss(o, sched, gauges_c, obs):
