@@ -21,9 +21,13 @@
 //   CFL accelerator ... timeControl_Kernel.cu:186-189
 //
 // Precision policy (template parameter T):
-//   double: every expression is evaluated in the reference's order with IEEE
-//           division and square root, so results agree with the reference
-//           update to rounding.
+//   double: expressions are evaluated in the reference's order with IEEE
+//           division and square root, except two that agree with the
+//           reference's forms to a few ulp at a fraction of the instructions:
+//           the friction's h^(-1/3) (a float seed and two Newton steps in
+//           place of pow) and the star celerity |base| (in place of
+//           sqrt(g * base^2 / g)); results agree with the reference update to
+//           rounding.
 //   float:  the reconstruction works relative to the cell's own bed (a local
 //           datum) so thin films are not lost in the rounding of a 50 m
 //           water level; the implicit-friction quotient uses its
@@ -91,16 +95,13 @@ template <> struct Prec<double> {
   }
   // (x / dx) * dt, the reference's order of operations
   __host__ __device__ static inline double scale(double x, double dx, double dt, double) { return (x / dx) * dt; }
-  // star-region celerity sqrt(g h*) with h* = base^2 / g, as the reference forms it
-  __host__ __device__ static inline double star_celerity(double base, double g) {
-    const double h_star = base * base / g;
-    return __builtin_sqrt(g * h_star);
-  }
-  // cotangent of base from that of the celerity a* = sqrt(g h*): through
-  // h* = base^2 / g, zero where a* = 0
-  __host__ __device__ static inline double star_celerity_adj(double base, double g, double a_star, double a_astar) {
-    const double a_hstar = a_star > 0.0 ? a_astar * g / (2.0 * a_star) : 0.0;
-    return a_hstar * 2.0 * base / g;
+  // star-region celerity sqrt(g h*) with h* = base^2 / g (the reference's
+  // form, a division and a square root per edge) is |base|; the two differ
+  // by rounding only (at most 2 ulp)
+  __host__ __device__ static inline double star_celerity(double base, double) { return fabs(base); }
+  // d|base| / d base = sign(base), 0 at base = 0
+  __host__ __device__ static inline double star_celerity_adj(double base, double, double, double a_astar) {
+    return base > 0.0 ? a_astar : (base < 0.0 ? -a_astar : 0.0);
   }
 };
 template <> struct Prec<float> {
@@ -124,7 +125,7 @@ template <> struct Prec<float> {
   }
   // x * (dt / dx) with the quotient computed once per block
   __device__ static inline float scale(float x, float, float, float dt_dx) { return x * dt_dx; }
-  // sqrt(g * base^2 / g) is |base| exactly; no square root needed
+  // sqrt(g * base^2 / g) is |base|; no square root needed
   __device__ static inline float star_celerity(float base, float) { return fabsf(base); }
   // d|base| / d base = sign(base), 0 at base = 0
   __device__ static inline float star_celerity_adj(float base, float, float, float a_astar) {
