@@ -83,8 +83,9 @@ __host__ __device__ inline bool solve_edge_vjp(const FrameCell<T>& c, const Fram
   }
   // ---- forward: velocities and hydrostatic reconstruction ----
   const bool dryL0 = lt_hs(eL.h), dryR0 = lt_hs(eR.h);
-  const T uL0 = dryL0 ? T(0) : P::div(eL.qn, eL.h), vL0 = dryL0 ? T(0) : P::div(eL.qt, eL.h);
-  const T uR0 = dryR0 ? T(0) : P::div(eR.qn, eR.h), vR0 = dryR0 ? T(0) : P::div(eR.qt, eR.h);
+  const T rhL = P::div(T(1), eL.h), rhR = P::div(T(1), eR.h);  // as hllc_edge forms them
+  const T uL0 = dryL0 ? T(0) : eL.qn * rhL, vL0 = dryL0 ? T(0) : eL.qt * rhL;
+  const T uR0 = dryR0 ? T(0) : eR.qn * rhR, vR0 = dryR0 ? T(0) : eR.qt * rhR;
   const T wL0 = eL.h + eL.z, wR0 = eR.h + eR.z;
   const bool maxC = etaC >= etaR;  // fmax(etaC, etaR) took etaC
   const bool minC = etaC <= etaR;  // fmin(etaC, etaR) took etaC
@@ -155,7 +156,8 @@ __host__ __device__ inline bool solve_edge_vjp(const FrameCell<T>& c, const Fram
       const T qnfR = uR0 * qnR + T(0.5) * g * hR * hR;
       const T Nh = sR * qnL - sL * qnR + sL * sR * (hR - hL);
       const T Nn = sR * qnfL - sL * qnfR + sL * sR * (qnR - qnL);
-      const T fh = P::div(Nh, D), fn = P::div(Nn, D);
+      const T rD = P::div(T(1), D);
+      const T fh = Nh * rD, fn = Nn * rD;
       const T b_fh_tot = b_fh + b_ft * vstar;
       if (useL) a_vL0 += b_ft * fh; else a_vR0 += b_ft * fh;
       T a_D = P::div(-(b_fh_tot * fh + b_fn * fn), D);

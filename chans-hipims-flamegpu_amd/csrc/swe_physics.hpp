@@ -25,9 +25,10 @@
 //           division and square root, except two that agree with the
 //           reference's forms to a few ulp at a fraction of the instructions:
 //           the friction's h^(-1/3) (a float seed and two Newton steps in
-//           place of pow) and the star celerity |base| (in place of
-//           sqrt(g * base^2 / g)); results agree with the reference update to
-//           rounding.
+//           place of pow), the star celerity |base| (in place of
+//           sqrt(g * base^2 / g)) and the edge terms scaled by dt / dx formed
+//           once (in place of (x / dx) * dt per term); results agree with the
+//           reference update to rounding.
 //   float:  the reconstruction works relative to the cell's own bed (a local
 //           datum) so thin films are not lost in the rounding of a 50 m
 //           water level; the implicit-friction quotient uses its
@@ -93,8 +94,10 @@ template <> struct Prec<double> {
     y = y * (4.0 - h * y * y * y) * (1.0 / 3.0);
     return y;
   }
-  // (x / dx) * dt, the reference's order of operations
-  __host__ __device__ static inline double scale(double x, double dx, double dt, double) { return (x / dx) * dt; }
+  // x * (dt / dx), the quotient formed once by the caller (the reference
+  // forms (x / dx) * dt per term: five divisions per edge; the two differ by
+  // rounding only)
+  __host__ __device__ static inline double scale(double x, double, double, double dt_dx) { return x * dt_dx; }
   // star-region celerity sqrt(g h*) with h* = base^2 / g (the reference's
   // form, a division and a square root per edge) is |base|; the two differ
   // by rounding only (at most 2 ulp)
@@ -240,10 +243,13 @@ __host__ __device__ inline void hllc_edge(FrameCell<T>& eL, FrameCell<T>& eR, T 
   // the branches would produce.
   using P = Prec<T>;
   const T g = T(kG);
-  const T uL0 = lt_hs(eL.h) ? T(0) : P::div(eL.qn, eL.h);
-  const T vL0 = lt_hs(eL.h) ? T(0) : P::div(eL.qt, eL.h);
-  const T uR0 = lt_hs(eR.h) ? T(0) : P::div(eR.qn, eR.h);
-  const T vR0 = lt_hs(eR.h) ? T(0) : P::div(eR.qt, eR.h);
+  // one reciprocal per face serves both velocities (in FP64 two IEEE
+  // divisions become one; in FP32 the hardware reciprocal was shared anyway)
+  const T rhL = P::div(T(1), eL.h), rhR = P::div(T(1), eR.h);
+  const T uL0 = lt_hs(eL.h) ? T(0) : eL.qn * rhL;
+  const T vL0 = lt_hs(eL.h) ? T(0) : eL.qt * rhL;
+  const T uR0 = lt_hs(eR.h) ? T(0) : eR.qn * rhR;
+  const T vR0 = lt_hs(eR.h) ? T(0) : eR.qt * rhR;
 
   // hydrostatic reconstruction
   T wL = eL.h + eL.z;
@@ -281,8 +287,9 @@ __host__ __device__ inline void hllc_edge(FrameCell<T>& eL, FrameCell<T>& eR, T 
   const T sM = P::div(sL * hR * (uR0 - sR) - sR * hL * (uL0 - sL), hR * (uR0 - sR) - hL * (uL0 - sL));
   const T qnfL = uL0 * qnL + T(0.5) * g * hL * hL;
   const T qnfR = uR0 * qnR + T(0.5) * g * hR * hR;
-  const T fh_s = P::div(sR * qnL - sL * qnR + sL * sR * (hR - hL), sR - sL);
-  const T fn_s = P::div(sR * qnfL - sL * qnfR + sL * sR * (qnR - qnL), sR - sL);
+  const T rS = P::div(T(1), sR - sL);
+  const T fh_s = (sR * qnL - sL * qnR + sL * sR * (hR - hL)) * rS;
+  const T fn_s = (sR * qnfL - sL * qnfR + sL * sR * (qnR - qnL)) * rS;
   const bool star_l = sL < T(0) && T(0) <= sM;
   const T ft_s = fh_s * (star_l ? vL0 : vR0);
   const bool left = T(0) <= sL;
