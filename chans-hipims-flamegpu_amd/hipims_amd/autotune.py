@@ -71,19 +71,21 @@ def tune(s: GridSolver, reps: int = 20, apply: bool = True, path: str = "flux") 
 
 
 def tune_ensemble(solvers: list[GridSolver], single_pass: bool = True, steps: int = 10,
-                  max_rows: int | None = None) -> int:
+                  max_rows: int | None = None, restore: bool = True) -> int:
     """Best unit height for one ensemble launch over `solvers` (all must share
     dtype and order): each legal height (up to `max_rows` when given) is
-    timed over `steps` ensemble steps (after two untimed ones) and every
-    domain's state is restored afterwards.  Sets the winner on every solver
-    and returns it."""
+    timed over `steps` ensemble steps (after two untimed ones).  With
+    `restore` every domain's state is put back afterwards; without it the
+    steps stand as the run's own (the states do not depend on the height),
+    so a run that retunes on the way loses no work to it.  Sets the winner
+    on every solver and returns it."""
     from .ensemble import Ensemble
 
     if not solvers:
         raise ValueError("no domains")
     dt = solvers[0].dtype
     steps += steps & 1  # the single-pass ensemble steps in pairs
-    saved = [(s.U.clone(), s.h_max.clone(), s.ctrl.clone()) for s in solvers]
+    saved = [(s.U.clone(), s.h_max.clone(), s.ctrl.clone()) for s in solvers] if restore else []
     timings = {}
     for ty in CHOICES[dt]:
         if max_rows is not None and ty > max_rows and ty != min(CHOICES[dt]):

@@ -103,8 +103,9 @@ class Ensemble:
         height was last chosen for, the height is timed again over them
         (autotune.tune_ensemble, heights up to the current one): the height
         that fills the GPU with many domains leaves it idle with a few small
-        ones.  The states do not depend on the height, only the FP64 mass
-        sums' rounding does; 0 keeps the height.  `self.retunes` records
+        ones; its timed steps count as the run's own.  The states do not
+        depend on the height, only the FP64 mass sums' rounding does; 0
+        keeps the height.  `self.retunes` records
         (scenarios, cells, height, seconds) of each retuning.
         `progress(replays, active)` is called after each host check."""
         if self.graph is None or self.graph_steps != steps_per_replay:
@@ -124,8 +125,9 @@ class Ensemble:
                 if cells < retune_below * tuned_cells:
                     t0 = time.perf_counter()
                     now = self.solvers[keep[0]].tile_rows
+                    # the timed steps are the run's own (no state restored)
                     best = autotune.tune_ensemble([self.solvers[m] for m in keep], single_pass=self.single_pass,
-                                                  max_rows=now if now > 0 else None)
+                                                  max_rows=now if now > 0 else None, restore=False)
                     self.retunes.append((len(keep), cells, best, time.perf_counter() - t0))
                     tuned_cells = cells
                 self._build(keep)
