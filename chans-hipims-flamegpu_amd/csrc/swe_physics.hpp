@@ -22,13 +22,14 @@
 //
 // Precision policy (template parameter T):
 //   double: expressions are evaluated in the reference's order with IEEE
-//           division and square root, except two that agree with the
-//           reference's forms to a few ulp at a fraction of the instructions:
-//           the friction's h^(-1/3) (a float seed and two Newton steps in
-//           place of pow), the star celerity |base| (in place of
-//           sqrt(g * base^2 / g)) and the edge terms scaled by dt / dx formed
-//           once (in place of (x / dx) * dt per term); results agree with the
-//           reference update to rounding.
+//           division and square root, except four forms that agree with the
+//           reference's to a few ulp at a fraction of the instructions: the
+//           friction's h^(-1/3) (a float seed and two Newton steps in place of
+//           pow), the star celerity |base| (in place of sqrt(g * base^2 / g)),
+//           the edge terms scaled by dt / dx formed once (in place of
+//           (x / dx) * dt per term) and one reciprocal per face for both
+//           velocities and per edge for both star-region fluxes; results
+//           agree with the reference update to rounding.
 //   float:  the reconstruction works relative to the cell's own bed (a local
 //           datum) so thin films are not lost in the rounding of a 50 m
 //           water level; the implicit-friction quotient uses its
