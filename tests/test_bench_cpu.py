@@ -62,3 +62,25 @@ def test_kernel_longer_than_its_step_is_refused(bench):
     with pytest.raises(AssertionError):
         bench.roofline_fields({"k_step_fused": 0.09, "k_finalize+gaps": 0.0}, 0.08, 4, False, CELLS,
                               "rainfall_runoff_2048_fp32_order2")
+
+
+def test_more_than_one_gpu_is_refused_before_any_gpu_work():
+    """`bench.py --gpus 2` (what a multi-GPU launch would pass) exits with
+    status 2 and one JSON line naming the reason, before importing torch."""
+    import json
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in json.loads(r.stdout.strip().splitlines()[-1])["error"]
+
+
+def test_default_run_is_the_contract_one(bench, monkeypatch):
+    """No flags: one GPU, 1000 timed steps after 50 warm-up steps, the 2048^2
+    FP32 order-2 rainfall-runoff workload, the ensemble and adjoint objects
+    included."""
+    monkeypatch.setattr("sys.argv", ["bench.py"])
+    a = bench.parse()
+    assert (a.gpus, a.steps, a.warmup, a.size, a.case, a.dtype, a.order) == (1, 1000, 50, 2048, "runoff", "fp32", 2)
+    assert a.ensemble_seconds > 0 and not a.no_adjoint
